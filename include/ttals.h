@@ -1,0 +1,167 @@
+/*
+ * ttals.h -- C ABI of libttals.so, the MI355X (gfx950) TT-ALS engine.
+ *
+ * The reference (PGelss/scikit_tt) is pure Python/numpy; its "FFI" for the
+ * hot path is the Python call surface scikit_tt.solvers.sle.als
+ * (scikit_tt/solvers/sle.py:10), scikit_tt.solvers.evp.als
+ * (scikit_tt/solvers/evp.py:17-27) and the TT methods they use.  Each entry
+ * point below replaces one of those functions (cited per declaration); the
+ * Python package scikit_tt_amd binds them with ctypes.
+ *
+ * Conventions
+ *   - All numeric arrays are float64, C-order, in DEVICE memory (HIP), except
+ *     the small integer shape arrays, which are HOST memory.
+ *   - TT cores have shape (r_i, m_i, n_i, r_{i+1}) like the reference
+ *     (scikit_tt/tensor_train.py:14-21); a TT "vector" has n_i = 1.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls are
+ *     asynchronous on that stream unless documented as synchronous.
+ *   - Return value: 0 on success, a negative TTALS_E* code on an argument or
+ *     runtime error (message in ttals_last_error()), or a positive
+ *     TTALS_S* status of the numerical result.
+ *   - Thread safety: one ttals_ctx per host thread; a context owns its
+ *     device workspace and must not be used concurrently.
+ */
+#ifndef TTALS_H
+#define TTALS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* argument / runtime errors (negative) */
+#define TTALS_E_ARG -1      /* bad argument (ValueError in Python) */
+#define TTALS_E_SHAPE -2    /* inconsistent TT shapes (ValueError) */
+#define TTALS_E_HIP -3      /* HIP runtime error */
+#define TTALS_E_OOM -4      /* device allocation failed (MemoryError) */
+#define TTALS_E_UNSUPPORTED -5 /* option not implemented (NotImplementedError) */
+
+/* numerical status of a solve (positive) */
+#define TTALS_S_SINGULAR 1    /* zero pivot in a local LU (numpy.linalg.LinAlgError) */
+#define TTALS_S_NOCONV 2      /* some local Krylov solve hit its iteration cap */
+#define TTALS_S_UNSUPPORTED 3 /* evp: sigma inside the spectrum of a large local problem */
+#define TTALS_S_NONFINITE 4   /* NaN/Inf in a local solution */
+
+typedef struct ttals_ctx ttals_ctx;
+
+/* A tensor train.  Shape arrays are host memory; cores are device pointers.
+ * For outputs, `ranks` receives the final ranks and `cores` must have the
+ * capacity of the corresponding input cores. */
+typedef struct ttals_tt {
+  int order;
+  int* row_dims;   /* [order] */
+  int* col_dims;   /* [order] */
+  int* ranks;      /* [order + 1] */
+  double** cores;  /* [order] device pointers */
+} ttals_tt;
+
+typedef struct ttals_result {
+  int status;              /* 0 or a TTALS_S_* code */
+  int sweeps;              /* ALS iterations performed */
+  int unconverged;         /* local Krylov solves that missed the tolerance */
+  int max_iters;           /* most Krylov iterations in one local solve */
+  long long krylov_iters;  /* total Krylov iterations (matvecs) */
+  long long local_solves;  /* local problems solved */
+  double lambda;           /* evp: eigenvalue returned */
+  double worst_residual;   /* sle: worst final relative residual of a local Krylov solve */
+} ttals_result;
+
+/* ---- context ------------------------------------------------------------ */
+int ttals_ctx_create(int device, ttals_ctx** out);
+void ttals_ctx_destroy(ttals_ctx* ctx);
+const char* ttals_last_error(ttals_ctx* ctx);
+int ttals_version(void);
+
+/* Step trace (diagnostics): when cap > 0, every later whole-solver call
+ * records 6 doubles per local solve of its FIRST system into the device
+ * buffer buf: position, forward(1)/backward(0), local size, Krylov
+ * iterations, eigenvalue (evp), final relative residual (sle Krylov).
+ * No counterpart in the reference (its solvers have no tracing). */
+int ttals_set_trace(ttals_ctx* ctx, double* buf, long long cap);
+
+/* ---- whole solvers (synchronous: return when the results are on the host) */
+
+/* Replaces scikit_tt.solvers.sle.als (scikit_tt/solvers/sle.py:10-95) for
+ * `count` independent systems in ONE launch (one workgroup per system).
+ * solver: 0 = 'solve', 1 = 'lu' (both: LU for local size <= 96, matrix-free
+ * CG/GMRES above).  tol/maxit: local Krylov relative tolerance / cap.
+ * x_out[s] receives the solution (cores in place, final ranks). */
+int ttals_sle_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                        const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
+                        ttals_result* res);
+
+/* Replaces scikit_tt.solvers.evp.als (scikit_tt/solvers/evp.py:17-179) with
+ * number_ev=1, operator_gevp=None, previous=[] for `count` systems in ONE
+ * launch.  solver: 0 = 'eig' (eigenvalue nearest sigma), 1 = 'eigh'
+ * (largest).  The returned TT is the best-so-far iterate (evp.py:151-155). */
+int ttals_evp_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                        const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
+                        int maxit, ttals_result* res);
+
+/* ---- hot-path pieces (one workgroup; for unit parity) --------------------
+ * Shapes: x core (a,n,b); operator core (P,n,n,Q); op stacks L (a,P,a),
+ * R (b,Q,b); rhs core (p,n,q); rhs stacks Lb (p,a), Rb (q,b). */
+
+/* scikit_tt/solvers/sle.py:194-219: Lout (b,Q,b) from L (a,P,a) */
+int ttals_stack_left_op(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                        const double* Acore, const double* x, double* Lout);
+/* scikit_tt/solvers/sle.py:250-276: Rout (a,P,a) from R (b,Q,b) */
+int ttals_stack_right_op(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* R,
+                         const double* Acore, const double* x, double* Rout);
+/* scikit_tt/solvers/sle.py:222-247: Lbout (q,b) from Lb (p,a) */
+int ttals_stack_left_rhs(ttals_ctx* ctx, void* stream, int a, int p, int n, int q, int b, const double* Lb,
+                         const double* Bcore, const double* x, double* Lbout);
+/* scikit_tt/solvers/sle.py:279-305: Rbout (p,a) from Rb (q,b) */
+int ttals_stack_right_rhs(ttals_ctx* ctx, void* stream, int a, int p, int n, int q, int b, const double* Rb,
+                          const double* Bcore, const double* x, double* Rbout);
+/* scikit_tt/solvers/sle.py:308-347: dense local matrix (N x N, N = a*n*b),
+ * rows (a',m,b'), cols (a,n,b) */
+int ttals_micro_matrix(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                       const double* Acore, const double* R, double* M);
+/* matrix-free y = M v with M as in ttals_micro_matrix */
+int ttals_micro_matvec(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                       const double* Acore, const double* R, const double* v, double* y);
+/* scikit_tt/solvers/sle.py:393-430: local rhs (a,n,b) */
+int ttals_micro_rhs(ttals_ctx* ctx, void* stream, int a, int p, int n, int q, int b, const double* Lb,
+                    const double* Bcore, const double* Rb, double* f);
+/* scikit_tt/solvers/sle.py:502-509: solve M v = f (v holds the initial
+ * guess on entry).  method: 0 auto (LU if N<=96, else CG if sym, GMRES),
+ * 1 LU (N<=96), 2 CG, 3 GMRES.  iters (host) receives the Krylov count. */
+int ttals_local_solve(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                      const double* Acore, const double* R, const double* f, double* v, int method, int sym,
+                      double tol, int maxit, int* iters);
+/* scikit_tt/solvers/evp.py:411-443 for symmetric local matrices: extreme
+ * eigenpair (mode 0 lowest, 1 highest, 2 nearest sigma).  v: start vector
+ * in, unit eigenvector out; lambda (host) receives the eigenvalue. */
+int ttals_local_eig(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                    const double* Acore, const double* R, double* v, int mode, double sigma, double tol, int maxit,
+                    double* lambda, int* iters);
+/* scikit_tt/solvers/sle.py:517-525 (economic QR): Householder QR of the
+ * row-major (m x c) matrix Y; Q (m x k) and R (k x c), k = min(m, c). */
+int ttals_qr(ttals_ctx* ctx, void* stream, int m, int c, const double* Y, double* Q, double* R);
+
+/* ---- TT algebra (replaces TT methods; asynchronous) ---------------------- */
+
+/* scikit_tt/tensor_train.py:447-481: core product
+ * (r1,m,n,r2) x (s1,n,p,s2) -> (r1*s1, m, p, r2*s2) */
+int ttals_core_product(void* stream, int r1, int m, int n, int r2, int s1, int p, int s2, const double* A,
+                       const double* X, double* out);
+/* scikit_tt/tensor_train.py:313-333: write `src` (r0,m,n,r1) scaled by
+ * `scale` into the block [o0:o0+r0, :, :, o1:o1+r1] of dst (R0,m,n,R1) */
+int ttals_core_embed(void* stream, int r0, int mn, int r1, const double* src, double scale, int R0, int R1, int o0,
+                     int o1, double* dst);
+/* scikit_tt/tensor_train.py:1334-1430 (p=2): Euclidean/Frobenius norm of a
+ * TT (operator cores are vectorised).  Synchronous; result on the host. */
+int ttals_tt_norm(ttals_ctx* ctx, void* stream, const ttals_tt* t, double* out);
+
+/* scikit_tt/tensor_train.py:1092-1205 (dir 0, ortho_left over cores
+ * start..end) and 1207-1310 (dir 1, ortho_right over cores start..end
+ * descending) without truncation: Householder QR sweep, in place; the
+ * cores keep their capacity and t->ranks receives the new ranks (the
+ * reference's thin-SVD ranks min(r_i m_i n_i, r_{i+1}) / min(r_i,
+ * m_i n_i r_{i+1})).  Synchronous. */
+int ttals_tt_ortho(ttals_ctx* ctx, void* stream, const ttals_tt* t, int dir, int start, int end);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TTALS_H */

@@ -1,0 +1,137 @@
+"""ctypes binding of libttals.so (the C ABI declared in include/ttals.h).
+
+The shared library is built in-tree (``python -m scikit_tt_amd.build``) and
+loaded from this package directory.  There is no fallback: if the library or
+a HIP device is missing, every compute call raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libttals.so')
+
+E_ARG, E_SHAPE, E_HIP, E_OOM, E_UNSUPPORTED = -1, -2, -3, -4, -5
+S_SINGULAR, S_NOCONV, S_UNSUPPORTED, S_NONFINITE = 1, 2, 3, 4
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class TTDesc(ctypes.Structure):
+    _fields_ = [('order', ctypes.c_int),
+                ('row_dims', c_int_p),
+                ('col_dims', c_int_p),
+                ('ranks', c_int_p),
+                ('cores', ctypes.POINTER(ctypes.c_void_p))]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [('status', ctypes.c_int),
+                ('sweeps', ctypes.c_int),
+                ('unconverged', ctypes.c_int),
+                ('max_iters', ctypes.c_int),
+                ('krylov_iters', ctypes.c_longlong),
+                ('local_solves', ctypes.c_longlong),
+                ('lambda_', ctypes.c_double),
+                ('worst_residual', ctypes.c_double)]
+
+
+# exported symbol -> (restype, argtypes)
+_vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+_TTp = ctypes.POINTER(TTDesc)
+_Rp = ctypes.POINTER(Result)
+SIGNATURES = {
+    'ttals_version': (_i, []),
+    'ttals_ctx_create': (_i, [_i, ctypes.POINTER(_vp)]),
+    'ttals_ctx_destroy': (None, [_vp]),
+    'ttals_last_error': (ctypes.c_char_p, [_vp]),
+    'ttals_sle_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _Rp]),
+    'ttals_evp_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i, _Rp]),
+    'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_stack_left_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_stack_right_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_micro_matrix': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_micro_matvec': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    'ttals_micro_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    'ttals_local_solve': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _i, c_int_p]),
+    'ttals_local_eig': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _d, _d, _i, c_double_p,
+                             c_int_p]),
+    'ttals_qr': (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    'ttals_core_product': (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    'ttals_core_embed': (_i, [_vp, _i, _i, _i, _vp, _d, _i, _i, _i, _i, _vp]),
+    'ttals_tt_norm': (_i, [_vp, _vp, _TTp, c_double_p]),
+    'ttals_tt_ortho': (_i, [_vp, _vp, _TTp, _i, _i, _i]),
+    'ttals_set_trace': (_i, [_vp, _vp, ctypes.c_longlong]),
+}
+
+_lock = threading.Lock()
+_lib = None
+_ctx = {}
+
+
+def load(path=LIB_PATH):
+    """Load libttals.so and declare every exported signature."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise ImportError(
+                    'libttals.so not found at %s: build it with `python -m scikit_tt_amd.build`' % path)
+            lib = ctypes.CDLL(path)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def context(device):
+    """Per-device engine context (workspace owner), created on first use."""
+    lib = load()
+    with _lock:
+        c = _ctx.get(device)
+        if c is None:
+            h = ctypes.c_void_p()
+            rc = lib.ttals_ctx_create(int(device), ctypes.byref(h))
+            if rc != 0:
+                raise RuntimeError('ttals_ctx_create(%d) failed with %d' % (device, rc))
+            c = h
+            _ctx[device] = c
+    return c
+
+
+def check(rc, ctx=None, what='ttals'):
+    """Map a negative C-ABI return code to the Python exception the reference raises."""
+    if rc >= 0:
+        return rc
+    msg = what
+    if ctx is not None:
+        err = load().ttals_last_error(ctx)
+        if err:
+            msg = '%s: %s' % (what, err.decode())
+    if rc in (E_ARG, E_SHAPE):
+        raise ValueError(msg)
+    if rc == E_OOM:
+        raise MemoryError(msg)
+    if rc == E_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError('%s (code %d)' % (msg, rc))
+
+
+class DescPack:
+    """Keeps the ctypes arrays behind a TTDesc alive."""
+
+    def __init__(self, order, row_dims, col_dims, ranks, ptrs):
+        self.rows = (ctypes.c_int * order)(*row_dims)
+        self.cols = (ctypes.c_int * order)(*col_dims)
+        self.ranks = (ctypes.c_int * (order + 1))(*ranks)
+        self.ptrs = (ctypes.c_void_p * order)(*ptrs)
+        self.desc = TTDesc(order, self.rows, self.cols, self.ranks, self.ptrs)
+
+    def final_ranks(self):
+        return [int(r) for r in np.ctypeslib.as_array(self.ranks)]

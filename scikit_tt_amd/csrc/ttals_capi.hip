@@ -1,0 +1,942 @@
+// libttals.so: kernels + C ABI of the MI355X TT-ALS engine (see include/ttals.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ttals_engine.hpp"
+#include "../../include/ttals.h"
+
+using namespace ttals;
+
+#define TTALS_VERSION 100
+
+// ---------------------------------------------------------------------------
+// kernels
+
+template <class ACC>
+__global__ __launch_bounds__(NT) void k_als(const SysDesc* descs) {
+  __shared__ ACC lds[GEMM_LDS_DOUBLES];
+  __shared__ SolveLDS S;
+  als_solve<ACC>(descs[blockIdx.x], lds, S);
+}
+
+struct PieceArgs {
+  int op, a, P, n, Q, b;
+  const double* L;
+  const double* A;
+  const double* R;   // or Lb/Rb/x depending on op
+  const double* x;
+  const double* f;
+  double* out;
+  double* v;
+  double* t1;
+  double* t2;
+  double* V;
+  double* res;       // [4] scalars out
+  int method, sym, maxit, mode;
+  double tol, sigma;
+};
+
+enum PieceOp { P_LEFT_OP, P_RIGHT_OP, P_LEFT_RHS, P_RIGHT_RHS, P_MICRO_DENSE, P_MATVEC, P_MICRO_RHS, P_SOLVE,
+               P_EIG, P_QR };
+
+__global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
+  __shared__ double lds[GEMM_LDS_DOUBLES];
+  __shared__ SolveLDS S;
+  switch (A.op) {
+    case P_LEFT_OP:
+      stack_left_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, A.t2, lds);
+      break;
+    case P_RIGHT_OP:
+      stack_right_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.R, A.A, A.x, A.out, A.t1, A.t2, lds);
+      break;
+    case P_LEFT_RHS:
+      stack_left_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, lds);
+      break;
+    case P_RIGHT_RHS:
+      stack_right_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.R, A.A, A.x, A.out, A.t1, lds);
+      break;
+    case P_MICRO_DENSE:
+      micro_dense<double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.a * A.n * A.b);
+      break;
+    case P_MATVEC:
+      local_matvec<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.x, A.out, A.t1, A.t2, lds);
+      break;
+    case P_MICRO_RHS:
+      micro_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.t1, lds);
+      break;
+    case P_SOLVE: {
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2};
+      const int N = op.size();
+      int its = 0, r = 0;
+      int method = A.method;
+      if (method == 0) method = (N <= NDENSE) ? 1 : (A.sym ? 2 : 3);
+      if (method == 1) {
+        micro_dense<double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.V, N);
+        wg_copy(A.f, A.v, N);
+        __syncthreads();
+        r = wg_lu_solve(A.V, N, A.v, reinterpret_cast<int*>(A.V + (long)N * N), S.red) ? 0 : 1;
+      } else {
+        if (method == 2) r = wg_cg<double>(op, A.f, A.v, A.V, A.tol, A.maxit, lds, S, &its);
+        if (method == 3 || r == 1) r = wg_gmres<double>(op, A.f, A.v, A.V, A.tol, A.maxit, KMAX, lds, S, &its);
+      }
+      if (threadIdx.x == 0) {
+        A.res[0] = r;
+        A.res[1] = its;
+      }
+      break;
+    }
+    case P_EIG: {
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2};
+      int its = 0;
+      double lam = 0.0;
+      const int r = wg_lanczos<double>(op, A.v, A.V, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      if (threadIdx.x == 0) {
+        A.res[0] = r;
+        A.res[1] = its;
+        A.res[2] = lam;
+      }
+      break;
+    }
+    case P_QR: {
+      // m = a, c = b; input x (m x c); out = Q (m x k); v = R (k x c); t1 = work
+      const int m = A.a, c = A.b;
+      double* W = A.t1;
+      for (long e = threadIdx.x; e < (long)m * c; e += NT) W[e] = A.x[e];
+      __syncthreads();
+      wg_householder(W, m, c, A.t2, S.red);
+      wg_form_q(W, m, c, A.t2, A.out);
+      const int k = m < c ? m : c;
+      for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+        const int i = (int)(e / c), j = (int)(e % c);
+        A.v[e] = (j >= i) ? W[(long)i * c + j] : 0.0;
+      }
+      break;
+    }
+  }
+}
+
+// TT core product (r1,m,n,r2) x (s1,n,p,s2) -> (r1*s1, m, p, r2*s2):
+// out[(i,k), mm, pp, (j,l)] = sum_nn A[i,mm,nn,j] X[k,nn,pp,l]
+__global__ void k_core_product(int r1, int m, int n, int r2, int s1, int p, int s2, const double* __restrict__ A,
+                               const double* __restrict__ X, double* __restrict__ out) {
+  const long total = (long)r1 * s1 * m * p * r2 * s2;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    long t = e;
+    const int l = (int)(t % s2); t /= s2;
+    const int j = (int)(t % r2); t /= r2;
+    const int pp = (int)(t % p); t /= p;
+    const int mm = (int)(t % m); t /= m;
+    const int k = (int)(t % s1); t /= s1;
+    const int i = (int)t;
+    double s = 0.0;
+    for (int nn = 0; nn < n; ++nn)
+      s += A[(((long)i * m + mm) * n + nn) * r2 + j] * X[(((long)k * n + nn) * p + pp) * s2 + l];
+    out[e] = s;
+  }
+}
+
+// dst[o0 + i, mn, o1 + j] = scale * src[i, mn, j]
+__global__ void k_core_embed(int r0, int mn, int r1, const double* __restrict__ src, double scale, int R0, int R1,
+                             int o0, int o1, double* __restrict__ dst) {
+  const long total = (long)r0 * mn * r1;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(e % r1);
+    const long t = e / r1;
+    const int q = (int)(t % mn), i = (int)(t / mn);
+    dst[((long)(o0 + i) * mn + q) * R1 + (o1 + j)] = scale * src[e];
+  }
+}
+
+// ||t||_2 by a right-to-left QR sweep: only the R factors are needed
+// (the norm is invariant under the orthogonal factors).  Cores are working
+// copies with dims (r, mn, r'); they are overwritten.
+struct NormArgs {
+  int d;
+  const int* mn;
+  const int* r;
+  double* const* c;   // working copies
+  double* W;          // >= max_i mn_i * r_{i+1} * r_i
+  double* Rc;         // >= r_i^2
+  double* T;          // >= max_i r_{i-1} * mn_{i-1} * r_i
+  double* tau;        // >= max rank
+  double* out;
+};
+__global__ __launch_bounds__(NT) void k_tt_norm(NormArgs A) {
+  __shared__ double lds[GEMM_LDS_DOUBLES];
+  __shared__ SolveLDS S;
+  int right = A.r[A.d];
+  for (int i = A.d - 1; i >= 1; --i) {
+    const int a = A.r[i], q = A.mn[i], b = right;
+    const int m = q * b, c = a;
+    // W (m x c) = core_i^T, core_i is (a x q*b)
+    for (long e = threadIdx.x; e < (long)m * c; e += NT) {
+      const int ii = (int)(e / c), jj = (int)(e % c);
+      A.W[e] = A.c[i][(long)jj * m + ii];
+    }
+    __syncthreads();
+    wg_householder(A.W, m, c, A.tau, S.red);
+    const int k = m < c ? m : c;
+    for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+      const int ii = (int)(e / c), jj = (int)(e % c);
+      A.Rc[e] = (jj >= ii) ? A.W[(long)ii * c + jj] : 0.0;
+    }
+    __syncthreads();
+    // core_{i-1} (rows x a) <- core_{i-1} * R^T  (rows x k)
+    const int rows = A.r[i - 1] * A.mn[i - 1];
+    wg_gemm<double, double>(rows, k, a, crm<double>(A.c[i - 1], a), ctr<double>(A.Rc, c), mrm<double>(A.T, k), lds);
+    wg_copy(A.T, A.c[i - 1], rows * k);
+    __syncthreads();
+    right = k;
+  }
+  const long n0 = (long)A.r[0] * A.mn[0] * right;
+  const double nrm = wg_nrm2(A.c[0], (int)n0, S.red);
+  if (threadIdx.x == 0) A.out[0] = nrm;
+}
+
+// Left/right orthonormalisation sweep of a TT by Householder QR (one
+// workgroup).  Cores are updated in place (capacity of the input ranks);
+// ranks (device int[d+1]) are updated.  dims: mn[i] = m_i * n_i.
+struct OrthoArgs {
+  int d, dir, start, end;  // dir 0: left (i = start..end), 1: right (i = start..end descending)
+  const int* mn;
+  int* r;
+  double* const* c;
+  double* W;   // >= max core size
+  double* Qm;  // >= max core size
+  double* Rc;  // >= rmax^2
+  double* T;   // >= max core size
+  double* tau; // >= rmax
+};
+__global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
+  __shared__ double lds[GEMM_LDS_DOUBLES];
+  __shared__ SolveLDS S;
+  if (A.dir == 0) {
+    for (int i = A.start; i <= A.end; ++i) {
+      const int a = A.r[i], q = A.mn[i], b = A.r[i + 1];
+      const int m = a * q, c = b;
+      wg_copy(A.c[i], A.W, m * c);
+      __syncthreads();
+      wg_householder(A.W, m, c, A.tau, S.red);
+      wg_form_q(A.W, m, c, A.tau, A.Qm);
+      const int k = m < c ? m : c;
+      for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+        const int ii = (int)(e / c), jj = (int)(e % c);
+        A.Rc[e] = (jj >= ii) ? A.W[(long)ii * c + jj] : 0.0;
+      }
+      wg_copy(A.Qm, A.c[i], m * k);
+      __syncthreads();
+      // core_{i+1} (b x rest) <- R (k x b) * core_{i+1}
+      const int rest = A.mn[i + 1] * A.r[i + 2];
+      wg_gemm<double, double>(k, rest, b, crm<double>(A.Rc, c), crm<double>(A.c[i + 1], rest), mrm<double>(A.T, rest),
+                              lds);
+      wg_copy(A.T, A.c[i + 1], k * rest);
+      if (threadIdx.x == 0) A.r[i + 1] = k;
+      __syncthreads();
+    }
+  } else {
+    for (int i = A.start; i >= A.end; --i) {
+      const int a = A.r[i], q = A.mn[i], b = A.r[i + 1];
+      const int m = q * b, c = a;
+      for (long e = threadIdx.x; e < (long)m * c; e += NT) {
+        const int ii = (int)(e / c), jj = (int)(e % c);
+        A.W[e] = A.c[i][(long)jj * m + ii];
+      }
+      __syncthreads();
+      wg_householder(A.W, m, c, A.tau, S.red);
+      wg_form_q(A.W, m, c, A.tau, A.Qm);
+      const int k = m < c ? m : c;
+      for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+        const int ii = (int)(e / c), jj = (int)(e % c);
+        A.Rc[e] = (jj >= ii) ? A.W[(long)ii * c + jj] : 0.0;
+      }
+      // core_i (k x m) = Q^T
+      for (long e = threadIdx.x; e < (long)k * m; e += NT) {
+        const int rr = (int)(e / m), cc = (int)(e % m);
+        A.c[i][e] = A.Qm[(long)cc * k + rr];
+      }
+      __syncthreads();
+      const int rows = A.r[i - 1] * A.mn[i - 1];
+      wg_gemm<double, double>(rows, k, a, crm<double>(A.c[i - 1], a), ctr<double>(A.Rc, c), mrm<double>(A.T, k),
+                              lds);
+      wg_copy(A.T, A.c[i - 1], rows * k);
+      if (threadIdx.x == 0) A.r[i] = k;
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct ttals_ctx {
+  int device = 0;
+  void* dws = nullptr;
+  size_t dws_bytes = 0;
+  char* hst = nullptr;       // pinned staging
+  size_t hst_bytes = 0;
+  double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
+  long trace_cap = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(ttals_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(ttals_ctx* c, hipError_t e, const char* where) {
+  return fail(c, TTALS_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, call)                                  \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+  } while (0)
+
+int ensure_dws(ttals_ctx* c, size_t bytes) {
+  if (bytes <= c->dws_bytes) return 0;
+  if (c->dws) (void)hipFree(c->dws);
+  c->dws = nullptr;
+  c->dws_bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 20);
+  hipError_t e = hipMalloc(&c->dws, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, TTALS_E_OOM, "device workspace allocation of " + std::to_string(want) + " bytes failed");
+  }
+  c->dws_bytes = want;
+  return 0;
+}
+
+int ensure_hst(ttals_ctx* c, size_t bytes) {
+  if (bytes <= c->hst_bytes) return 0;
+  if (c->hst) (void)hipHostFree(c->hst);
+  c->hst = nullptr;
+  c->hst_bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 16);
+  hipError_t e = hipHostMalloc((void**)&c->hst, want, hipHostMallocDefault);
+  if (e != hipSuccess) return hip_fail(c, e, "hipHostMalloc");
+  c->hst_bytes = want;
+  return 0;
+}
+
+inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte layout planner: first pass counts, second pass hands out offsets.
+struct Layout {
+  size_t bytes = 0;
+  size_t take(size_t n) {
+    size_t o = bytes;
+    bytes = al256(bytes + n);
+    return o;
+  }
+};
+
+struct SysPlan {
+  int d = 0;
+  std::vector<int> n, rx, ra, rb;
+  long nmax = 0, tmax = 0, rmax = 0;
+  size_t xsz_total = 0;
+  // metadata (host staging offsets)
+  size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_Loff, m_Roff, m_Lboff, m_Rboff, m_desc;
+  // device workspace offsets (doubles region)
+  size_t w_L, w_R, w_Lb, w_Rb, w_f, w_g, w_t1, w_t2, w_V, w_dense, w_qw, w_qq, w_tau, w_rf, w_best, w_out;
+  std::vector<long> Loff, Roff, Lboff, Rboff;
+};
+
+int check_tt(ttals_ctx* c, const ttals_tt& t, bool vector, const char* what) {
+  if (t.order < 1 || !t.row_dims || !t.col_dims || !t.ranks || !t.cores)
+    return fail(c, TTALS_E_ARG, std::string(what) + ": incomplete TT descriptor");
+  for (int i = 0; i < t.order; ++i) {
+    if (t.row_dims[i] < 1 || t.col_dims[i] < 1 || t.ranks[i] < 1 || !t.cores[i])
+      return fail(c, TTALS_E_SHAPE, std::string(what) + ": bad dimension, rank or core pointer");
+    if (vector && t.col_dims[i] != 1) return fail(c, TTALS_E_SHAPE, std::string(what) + ": expected a TT vector");
+  }
+  if (t.ranks[t.order] < 1) return fail(c, TTALS_E_SHAPE, std::string(what) + ": bad rank");
+  return 0;
+}
+
+// kind 0: sle (b given), 1: evp
+int plan_system(ttals_ctx* c, int kind, const ttals_tt& A, const ttals_tt& x0, const ttals_tt* b, SysPlan& P) {
+  int rc;
+  if ((rc = check_tt(c, A, false, "operator"))) return rc;
+  if ((rc = check_tt(c, x0, true, "initial_guess"))) return rc;
+  if (b && (rc = check_tt(c, *b, true, "right_hand_side"))) return rc;
+  const int d = A.order;
+  if (x0.order != d || (b && b->order != d)) return fail(c, TTALS_E_SHAPE, "orders of the TTs differ");
+  if (x0.ranks[0] != 1 || x0.ranks[d] != 1 || A.ranks[0] != 1 || A.ranks[d] != 1 ||
+      (b && (b->ranks[0] != 1 || b->ranks[d] != 1)))
+    return fail(c, TTALS_E_SHAPE, "outer TT ranks must be 1");
+  P.d = d;
+  P.n.resize(d);
+  P.rx.resize(d + 1);
+  P.ra.resize(d + 1);
+  P.rb.assign(d + 1, 1);
+  for (int i = 0; i < d; ++i) {
+    if (A.row_dims[i] != A.col_dims[i])
+      return fail(c, TTALS_E_SHAPE, "operator must be square core by core (row_dims == col_dims)");
+    if (x0.row_dims[i] != A.col_dims[i]) return fail(c, TTALS_E_SHAPE, "initial guess dims != operator col dims");
+    if (b && b->row_dims[i] != A.row_dims[i]) return fail(c, TTALS_E_SHAPE, "rhs dims != operator row dims");
+    P.n[i] = A.row_dims[i];
+  }
+  for (int i = 0; i <= d; ++i) {
+    P.rx[i] = x0.ranks[i];
+    P.ra[i] = A.ranks[i];
+    if (b) P.rb[i] = b->ranks[i];
+  }
+  P.nmax = P.tmax = P.rmax = 1;
+  P.Loff.resize(d);
+  P.Roff.resize(d);
+  P.Lboff.resize(d);
+  P.Rboff.resize(d);
+  long lo = 0, bo = 0;
+  P.xsz_total = 0;
+  for (int i = 0; i < d; ++i) {
+    const long a = P.rx[i], bb = P.rx[i + 1], n = P.n[i];
+    const long N = a * n * bb;
+    P.nmax = std::max(P.nmax, N);
+    const long mx = std::max(std::max((long)P.ra[i], (long)P.ra[i + 1]), std::max((long)P.rb[i], (long)P.rb[i + 1]));
+    P.tmax = std::max(P.tmax, N * mx);
+    P.rmax = std::max(P.rmax, std::max(a, bb));
+    P.Loff[i] = P.Roff[i] = lo;
+    lo += a * P.ra[i] * a;
+    P.Lboff[i] = P.Rboff[i] = bo;
+    bo += (long)P.rb[i] * a;
+    P.xsz_total += (size_t)N;
+  }
+  // stacks sized by the initial ranks (ranks never grow)
+  P.w_L = (size_t)lo;   // temporarily: element counts
+  P.w_Lb = (size_t)bo;
+  (void)kind;
+  return 0;
+}
+
+template <class T>
+T* hptr(ttals_ctx* c, size_t off) { return reinterpret_cast<T*>(c->hst + off); }
+
+// Builds descriptors for `count` systems, runs k_als (one workgroup per
+// system), copies ranks and scalars back and synchronises the stream.
+int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals_tt* A, const ttals_tt* x0,
+              const ttals_tt* b, const ttals_tt* xo, int repeats, int solver, double tol, int maxit, double sigma,
+              double conv_eps, ttals_result* res) {
+  if (count < 1 || !A || !x0 || !xo || !res) return fail(c, TTALS_E_ARG, "bad batch arguments");
+  if (kind == 0 && !b) return fail(c, TTALS_E_ARG, "missing right-hand side");
+  if (repeats < 0) return fail(c, TTALS_E_ARG, "repeats must be >= 0");
+  if (!(tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
+  if (maxit < 1) return fail(c, TTALS_E_ARG, "maxit must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<SysPlan> plans(count);
+  Layout hl, dl;  // host metadata layout (mirrored on device), device workspace layout
+  const size_t m_descs = hl.take(sizeof(SysDesc) * count);
+  for (int s = 0; s < count; ++s) {
+    SysPlan& P = plans[s];
+    int rc = plan_system(c, kind, A[s], x0[s], kind == 0 ? &b[s] : nullptr, P);
+    if (rc) return fail(c, rc, "system " + std::to_string(s) + ": " + c->err);
+    if (check_tt(c, xo[s], true, "x_out") || xo[s].order != P.d)
+      return fail(c, TTALS_E_ARG, "system " + std::to_string(s) + ": bad output descriptor");
+    const int d = P.d;
+    P.m_n = hl.take(sizeof(int) * d);
+    P.m_rx = hl.take(sizeof(int) * (d + 1));
+    P.m_ra = hl.take(sizeof(int) * (d + 1));
+    P.m_rb = hl.take(sizeof(int) * (d + 1));
+    P.m_brx = hl.take(sizeof(int) * (d + 1));
+    P.m_xc = hl.take(sizeof(void*) * d);
+    P.m_ac = hl.take(sizeof(void*) * d);
+    P.m_bc = hl.take(sizeof(void*) * d);
+    P.m_bestc = hl.take(sizeof(void*) * d);
+    P.m_Loff = hl.take(sizeof(long) * d);
+    P.m_Roff = hl.take(sizeof(long) * d);
+    P.m_Lboff = hl.take(sizeof(long) * d);
+    P.m_Rboff = hl.take(sizeof(long) * d);
+    const size_t nst = P.w_L, nbst = P.w_Lb;
+    P.w_L = dl.take(8 * std::max<size_t>(nst, 1));
+    P.w_R = dl.take(8 * std::max<size_t>(nst, 1));
+    P.w_Lb = dl.take(8 * std::max<size_t>(nbst, 1));
+    P.w_Rb = dl.take(8 * std::max<size_t>(nbst, 1));
+    P.w_f = dl.take(8 * P.nmax);
+    P.w_g = dl.take(8 * P.nmax);
+    P.w_t1 = dl.take(8 * P.tmax);
+    P.w_t2 = dl.take(8 * P.tmax);
+    P.w_V = dl.take(8 * (size_t)(KMAX + 1) * P.nmax);
+    P.w_dense = dl.take(8 * (size_t)(NDENSE * NDENSE + NDENSE));
+    P.w_qw = dl.take(8 * P.nmax);
+    P.w_qq = dl.take(8 * P.nmax);
+    P.w_tau = dl.take(8 * P.rmax);
+    P.w_rf = dl.take(8 * P.rmax * P.rmax);
+    P.w_best = dl.take(kind == 1 ? 8 * P.xsz_total : 8);
+    P.w_out = dl.take(8 * 8);
+  }
+  const size_t meta_bytes = hl.bytes;
+  // results region (ranks + scalars) in the staging buffer after the metadata
+  Layout rl;
+  rl.bytes = meta_bytes;
+  std::vector<size_t> r_rx(count), r_out(count);
+  for (int s = 0; s < count; ++s) {
+    r_rx[s] = rl.take(sizeof(int) * (plans[s].d + 1));
+    r_out[s] = rl.take(8 * 8);
+  }
+  int rc;
+  if ((rc = ensure_hst(c, rl.bytes))) return rc;
+  if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
+  char* dbase = reinterpret_cast<char*>(c->dws);
+  char* dmeta = dbase + dl.bytes;  // metadata mirrored after the workspace
+  auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
+  auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
+  SysDesc* hdesc = hptr<SysDesc>(c, m_descs);
+  for (int s = 0; s < count; ++s) {
+    SysPlan& P = plans[s];
+    const int d = P.d;
+    std::memcpy(hptr<int>(c, P.m_n), P.n.data(), sizeof(int) * d);
+    std::memcpy(hptr<int>(c, P.m_rx), P.rx.data(), sizeof(int) * (d + 1));
+    std::memcpy(hptr<int>(c, P.m_ra), P.ra.data(), sizeof(int) * (d + 1));
+    std::memcpy(hptr<int>(c, P.m_rb), P.rb.data(), sizeof(int) * (d + 1));
+    std::memcpy(hptr<int>(c, P.m_brx), P.rx.data(), sizeof(int) * (d + 1));
+    double** xcp = hptr<double*>(c, P.m_xc);
+    const double** acp = hptr<const double*>(c, P.m_ac);
+    const double** bcp = hptr<const double*>(c, P.m_bc);
+    double** bestp = hptr<double*>(c, P.m_bestc);
+    size_t boff = 0;
+    for (int i = 0; i < d; ++i) {
+      xcp[i] = xo[s].cores[i];
+      acp[i] = A[s].cores[i];
+      bcp[i] = kind == 0 ? b[s].cores[i] : nullptr;
+      bestp[i] = wptr(P.w_best) + (kind == 1 ? boff : 0);
+      boff += (size_t)P.rx[i] * P.n[i] * P.rx[i + 1];
+    }
+    std::memcpy(hptr<long>(c, P.m_Loff), P.Loff.data(), sizeof(long) * d);
+    std::memcpy(hptr<long>(c, P.m_Roff), P.Roff.data(), sizeof(long) * d);
+    std::memcpy(hptr<long>(c, P.m_Lboff), P.Lboff.data(), sizeof(long) * d);
+    std::memcpy(hptr<long>(c, P.m_Rboff), P.Rboff.data(), sizeof(long) * d);
+    SysDesc& D = hdesc[s];
+    std::memset(&D, 0, sizeof(D));
+    D.d = d;
+    D.kind = kind;
+    D.solver = solver;
+    D.repeats = repeats;
+    D.ortho_x0 = (kind == 0) ? 1 : 0;
+    D.sym = -1;
+    D.gm_restart = KMAX;
+    D.maxit = maxit;
+    D.tol = tol;
+    D.sigma = sigma;
+    D.conv_eps = conv_eps;
+    D.n = (const int*)dptr(P.m_n);
+    D.rx = (int*)dptr(P.m_rx);
+    D.ra = (const int*)dptr(P.m_ra);
+    D.rb = (const int*)dptr(P.m_rb);
+    D.best_rx = (int*)dptr(P.m_brx);
+    D.xc = (double* const*)dptr(P.m_xc);
+    D.ac = (const double* const*)dptr(P.m_ac);
+    D.bc = (const double* const*)dptr(P.m_bc);
+    D.bestc = (double* const*)dptr(P.m_bestc);
+    D.Ls = wptr(P.w_L);
+    D.Rs = wptr(P.w_R);
+    D.Lbs = wptr(P.w_Lb);
+    D.Rbs = wptr(P.w_Rb);
+    D.Loff = (const long*)dptr(P.m_Loff);
+    D.Roff = (const long*)dptr(P.m_Roff);
+    D.Lboff = (const long*)dptr(P.m_Lboff);
+    D.Rboff = (const long*)dptr(P.m_Rboff);
+    D.f = wptr(P.w_f);
+    D.g = wptr(P.w_g);
+    D.t1 = wptr(P.w_t1);
+    D.t2 = wptr(P.w_t2);
+    D.V = wptr(P.w_V);
+    D.dense = wptr(P.w_dense);
+    D.qw = wptr(P.w_qw);
+    D.qq = wptr(P.w_qq);
+    D.tau = wptr(P.w_tau);
+    D.rf = wptr(P.w_rf);
+    D.out = wptr(P.w_out);
+    D.trace = (c->trace && s == 0) ? c->trace : nullptr;
+    D.trace_cap = c->trace_cap;
+    // x_out <- x0 (the engine works in place on x_out)
+    for (int i = 0; i < d; ++i) {
+      const size_t bytes = 8 * (size_t)P.rx[i] * P.n[i] * P.rx[i + 1];
+      if (xo[s].cores[i] != x0[s].cores[i])
+        HIPCHK(c, hipMemcpyAsync(xo[s].cores[i], x0[s].cores[i], bytes, hipMemcpyDeviceToDevice, stream));
+    }
+  }
+  HIPCHK(c, hipMemcpyAsync(dmeta, c->hst, meta_bytes, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_als<double>, dim3(count), dim3(NT), 0, stream, (const SysDesc*)dptr(m_descs));
+  HIPCHK(c, hipGetLastError());
+  for (int s = 0; s < count; ++s) {
+    HIPCHK(c, hipMemcpyAsync(c->hst + r_rx[s], dptr(plans[s].m_rx), sizeof(int) * (plans[s].d + 1),
+                             hipMemcpyDeviceToHost, stream));
+    HIPCHK(c, hipMemcpyAsync(c->hst + r_out[s], wptr(plans[s].w_out), 8 * 8, hipMemcpyDeviceToHost, stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(stream));
+  int worst = 0;
+  for (int s = 0; s < count; ++s) {
+    const int* rr = hptr<int>(c, r_rx[s]);
+    for (int i = 0; i <= plans[s].d; ++i) xo[s].ranks[i] = rr[i];
+    const double* o = hptr<double>(c, r_out[s]);
+    ttals_result& R = res[s];
+    R.lambda = o[0];
+    R.sweeps = (int)o[1];
+    R.status = (int)o[2];
+    R.krylov_iters = (long long)o[3];
+    R.unconverged = (int)o[4];
+    R.local_solves = (long long)o[5];
+    R.max_iters = (int)o[6];
+    R.worst_residual = o[7];
+    if (R.status == ST_OK && R.unconverged > 0) R.status = ST_NOCONV;
+    if (R.status != ST_OK && (worst == 0 || (R.status != ST_NOCONV && worst == ST_NOCONV))) worst = R.status;
+  }
+  return worst;
+}
+
+// scratch for the piece entry points: 2*t + (KMAX+2)*N doubles
+int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double** V, double** res) {
+  const long nv = std::max<long>((KMAX + 2) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
+  const size_t bytes = 8 * (size_t)(2 * t + nv + 16) + 4096;
+  int rc = ensure_dws(c, bytes);
+  if (rc) return rc;
+  double* base = reinterpret_cast<double*>(c->dws);
+  *t1 = base;
+  *t2 = base + t;
+  *V = base + 2 * t;
+  *res = base + 2 * t + nv;
+  return 0;
+}
+
+int launch_piece(ttals_ctx* c, hipStream_t st, PieceArgs& a) {
+  hipLaunchKernelGGL(k_piece, dim3(1), dim3(NT), 0, st, a);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+bool dims_ok(std::initializer_list<int> v) {
+  for (int x : v)
+    if (x < 1) return false;
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int ttals_version(void) { return TTALS_VERSION; }
+
+int ttals_ctx_create(int device, ttals_ctx** out) {
+  if (!out) return TTALS_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TTALS_E_ARG;
+  ttals_ctx* c = new ttals_ctx();
+  c->device = device;
+  *out = c;
+  return 0;
+}
+
+void ttals_ctx_destroy(ttals_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->dws) (void)hipFree(c->dws);
+  if (c->hst) (void)hipHostFree(c->hst);
+  delete c;
+}
+
+const char* ttals_last_error(ttals_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ttals_sle_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                        const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
+                        ttals_result* res) {
+  if (!c) return TTALS_E_ARG;
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "sle solver must be 'solve' or 'lu'");
+  return run_batch(c, (hipStream_t)stream, 0, count, A, x0, b, x_out, repeats, solver, tol, maxit, 0.0, 0.0, res);
+}
+
+int ttals_evp_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                        const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
+                        int maxit, ttals_result* res) {
+  if (!c) return TTALS_E_ARG;
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "evp solver must be 'eig' or 'eigh'");
+  return run_batch(c, (hipStream_t)stream, 1, count, A, x0, nullptr, x_out, repeats, solver, tol, maxit, sigma,
+                   conv_eps, res);
+}
+
+#define PIECE_PROLOGUE(tsize, Nsize)                                  \
+  if (!c) return TTALS_E_ARG;                                         \
+  double *t1, *t2, *V, *res;                                          \
+  int rc = piece_scratch(c, (tsize), (Nsize), &t1, &t2, &V, &res);    \
+  if (rc) return rc;                                                  \
+  PieceArgs pa;                                                       \
+  std::memset(&pa, 0, sizeof(pa));                                    \
+  pa.t1 = t1;                                                         \
+  pa.t2 = t2;                                                         \
+  pa.V = V;                                                           \
+  pa.res = res;
+
+int ttals_stack_left_op(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                        const double* Acore, const double* x, double* Lout) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * b * std::max(P, Q), 1)
+  pa.op = P_LEFT_OP;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.L = L; pa.A = Acore; pa.x = x; pa.out = Lout;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_stack_right_op(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* R,
+                         const double* Acore, const double* x, double* Rout) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * b * std::max(P, Q), 1)
+  pa.op = P_RIGHT_OP;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.R = R; pa.A = Acore; pa.x = x; pa.out = Rout;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_stack_left_rhs(ttals_ctx* c, void* stream, int a, int p, int n, int q, int b, const double* Lb,
+                         const double* Bcore, const double* x, double* Lbout) {
+  if (!dims_ok({a, p, n, q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * b * std::max(p, q), 1)
+  pa.op = P_LEFT_RHS;
+  pa.a = a; pa.P = p; pa.n = n; pa.Q = q; pa.b = b;
+  pa.L = Lb; pa.A = Bcore; pa.x = x; pa.out = Lbout;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_stack_right_rhs(ttals_ctx* c, void* stream, int a, int p, int n, int q, int b, const double* Rb,
+                          const double* Bcore, const double* x, double* Rbout) {
+  if (!dims_ok({a, p, n, q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * b * std::max(p, q), 1)
+  pa.op = P_RIGHT_RHS;
+  pa.a = a; pa.P = p; pa.n = n; pa.Q = q; pa.b = b;
+  pa.R = Rb; pa.A = Bcore; pa.x = x; pa.out = Rbout;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_micro_matrix(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                       const double* Acore, const double* R, double* M) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE(1, 1)
+  pa.op = P_MICRO_DENSE;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.L = L; pa.A = Acore; pa.R = R; pa.out = M;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_micro_matvec(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                       const double* Acore, const double* R, const double* v, double* y) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * b * std::max(P, Q), 1)
+  pa.op = P_MATVEC;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.L = L; pa.A = Acore; pa.R = R; pa.x = v; pa.out = y;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_micro_rhs(ttals_ctx* c, void* stream, int a, int p, int n, int q, int b, const double* Lb,
+                    const double* Bcore, const double* Rb, double* f) {
+  if (!dims_ok({a, p, n, q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)a * n * std::max(p, q) * b, 1)
+  pa.op = P_MICRO_RHS;
+  pa.a = a; pa.P = p; pa.n = n; pa.Q = q; pa.b = b;
+  pa.L = Lb; pa.A = Bcore; pa.R = Rb; pa.out = f;
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_local_solve(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                      const double* Acore, const double* R, const double* f, double* v, int method, int sym,
+                      double tol, int maxit, int* iters) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  const long N = (long)a * n * b;
+  if (method < 0 || method > 3) return fail(c, TTALS_E_ARG, "bad method");
+  if (method == 1 && N > NDENSE) return fail(c, TTALS_E_ARG, "dense LU limited to local size 96");
+  if (!(tol > 0.0) || maxit < 1) return fail(c, TTALS_E_ARG, "bad tol/maxit");
+  PIECE_PROLOGUE(N * std::max(P, Q), N)
+  pa.op = P_SOLVE;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.L = L; pa.A = Acore; pa.R = R; pa.f = f; pa.v = v;
+  pa.method = method; pa.sym = sym; pa.tol = tol; pa.maxit = maxit;
+  if ((rc = launch_piece(c, (hipStream_t)stream, pa))) return rc;
+  double h[2];
+  HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  HIPCHK(c, hipStreamSynchronize((hipStream_t)stream));
+  if (iters) *iters = (int)h[1];
+  const int r = (int)h[0];
+  if (r == 0) return 0;
+  if (method == 1 || (method == 0 && N <= NDENSE)) return ST_SINGULAR;
+  return ST_NOCONV;
+}
+
+int ttals_local_eig(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int b, const double* L,
+                    const double* Acore, const double* R, double* v, int mode, double sigma, double tol, int maxit,
+                    double* lambda, int* iters) {
+  if (!dims_ok({a, P, n, Q, b})) return fail(c, TTALS_E_ARG, "bad dims");
+  if (mode < 0 || mode > 2 || !(tol > 0.0) || maxit < 1) return fail(c, TTALS_E_ARG, "bad mode/tol/maxit");
+  const long N = (long)a * n * b;
+  PIECE_PROLOGUE(N * std::max(P, Q), N)
+  pa.op = P_EIG;
+  pa.a = a; pa.P = P; pa.n = n; pa.Q = Q; pa.b = b;
+  pa.L = L; pa.A = Acore; pa.R = R; pa.v = v;
+  pa.mode = mode; pa.sigma = sigma; pa.tol = tol; pa.maxit = maxit;
+  if ((rc = launch_piece(c, (hipStream_t)stream, pa))) return rc;
+  double h[3];
+  HIPCHK(c, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  HIPCHK(c, hipStreamSynchronize((hipStream_t)stream));
+  if (iters) *iters = (int)h[1];
+  if (lambda) *lambda = h[2];
+  const int r = (int)h[0];
+  return r == 0 ? 0 : (r == 3 ? ST_UNSUPPORTED : ST_NOCONV);
+}
+
+int ttals_qr(ttals_ctx* c, void* stream, int m, int cdim, const double* Y, double* Q, double* R) {
+  if (!dims_ok({m, cdim})) return fail(c, TTALS_E_ARG, "bad dims");
+  PIECE_PROLOGUE((long)m * cdim + std::max(m, cdim), 1)
+  pa.op = P_QR;
+  pa.a = m; pa.b = cdim;
+  pa.x = Y; pa.out = Q; pa.v = R;
+  pa.t2 = t1 + (long)m * cdim;  // tau after W
+  return launch_piece(c, (hipStream_t)stream, pa);
+}
+
+int ttals_core_product(void* stream, int r1, int m, int n, int r2, int s1, int p, int s2, const double* A,
+                       const double* X, double* out) {
+  if (!dims_ok({r1, m, n, r2, s1, p, s2}) || !A || !X || !out) return TTALS_E_ARG;
+  const long total = (long)r1 * s1 * m * p * r2 * s2;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_core_product, dim3(blocks), dim3(256), 0, (hipStream_t)stream, r1, m, n, r2, s1, p, s2, A,
+                     X, out);
+  return hipGetLastError() == hipSuccess ? 0 : TTALS_E_HIP;
+}
+
+int ttals_core_embed(void* stream, int r0, int mn, int r1, const double* src, double scale, int R0, int R1, int o0,
+                     int o1, double* dst) {
+  if (!dims_ok({r0, mn, r1, R0, R1}) || o0 < 0 || o1 < 0 || o0 + r0 > R0 || o1 + r1 > R1 || !src || !dst)
+    return TTALS_E_ARG;
+  const long total = (long)r0 * mn * r1;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_core_embed, dim3(blocks), dim3(256), 0, (hipStream_t)stream, r0, mn, r1, src, scale, R0, R1,
+                     o0, o1, dst);
+  return hipGetLastError() == hipSuccess ? 0 : TTALS_E_HIP;
+}
+
+int ttals_tt_norm(ttals_ctx* c, void* stream, const ttals_tt* t, double* out) {
+  if (!c || !t || !out) return TTALS_E_ARG;
+  int rc = check_tt(c, *t, false, "tt");
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int d = t->order;
+  hipStream_t st = (hipStream_t)stream;
+  // workspace: copies of the cores + W + Rc + T + tau + out + metadata
+  Layout L;
+  std::vector<size_t> coff(d);
+  long wmax = 1, rmax = 1, tmax = 1;
+  for (int i = 0; i < d; ++i) {
+    const long mn = (long)t->row_dims[i] * t->col_dims[i];
+    const long sz = (long)t->ranks[i] * mn * t->ranks[i + 1];
+    coff[i] = L.take(8 * sz);
+    wmax = std::max(wmax, sz);
+    rmax = std::max(rmax, (long)std::max(t->ranks[i], t->ranks[i + 1]));
+    tmax = std::max(tmax, sz);
+  }
+  const size_t oW = L.take(8 * wmax), oR = L.take(8 * rmax * rmax), oT = L.take(8 * tmax), otau = L.take(8 * rmax),
+               oout = L.take(64);
+  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), optr = L.take(sizeof(void*) * d);
+  if ((rc = ensure_dws(c, L.bytes))) return rc;
+  if ((rc = ensure_hst(c, L.bytes))) return rc;
+  char* db = reinterpret_cast<char*>(c->dws);
+  int* hmn = reinterpret_cast<int*>(c->hst + omn);
+  int* hrr = reinterpret_cast<int*>(c->hst + orr);
+  double** hp = reinterpret_cast<double**>(c->hst + optr);
+  for (int i = 0; i < d; ++i) {
+    hmn[i] = t->row_dims[i] * t->col_dims[i];
+    hp[i] = reinterpret_cast<double*>(db + coff[i]);
+    const size_t bytes = 8 * (size_t)t->ranks[i] * hmn[i] * t->ranks[i + 1];
+    HIPCHK(c, hipMemcpyAsync(hp[i], t->cores[i], bytes, hipMemcpyDeviceToDevice, st));
+  }
+  for (int i = 0; i <= d; ++i) hrr[i] = t->ranks[i];
+  HIPCHK(c, hipMemcpyAsync(db + omn, c->hst + omn, L.bytes - omn, hipMemcpyHostToDevice, st));
+  NormArgs na;
+  na.d = d;
+  na.mn = reinterpret_cast<const int*>(db + omn);
+  na.r = reinterpret_cast<const int*>(db + orr);
+  na.c = reinterpret_cast<double* const*>(db + optr);
+  na.W = reinterpret_cast<double*>(db + oW);
+  na.Rc = reinterpret_cast<double*>(db + oR);
+  na.T = reinterpret_cast<double*>(db + oT);
+  na.tau = reinterpret_cast<double*>(db + otau);
+  na.out = reinterpret_cast<double*>(db + oout);
+  hipLaunchKernelGGL(k_tt_norm, dim3(1), dim3(NT), 0, st, na);
+  HIPCHK(c, hipGetLastError());
+  double h = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&h, na.out, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  *out = h;
+  return 0;
+}
+
+int ttals_set_trace(ttals_ctx* c, double* buf, long long cap) {
+  if (!c || cap < 0 || (cap > 0 && !buf)) return TTALS_E_ARG;
+  c->trace = cap > 0 ? buf : nullptr;
+  c->trace_cap = cap;
+  return 0;
+}
+
+int ttals_tt_ortho(ttals_ctx* c, void* stream, const ttals_tt* t, int dir, int start, int end) {
+  if (!c || !t) return TTALS_E_ARG;
+  int rc = check_tt(c, *t, false, "tt");
+  if (rc) return rc;
+  const int d = t->order;
+  if (dir == 0 && (start < 0 || end > d - 2 || start > end + 1)) return fail(c, TTALS_E_ARG, "bad index range");
+  if (dir == 1 && (start > d - 1 || end < 1 || start < end - 1)) return fail(c, TTALS_E_ARG, "bad index range");
+  if (dir != 0 && dir != 1) return fail(c, TTALS_E_ARG, "bad direction");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  long cmax = 1, rmax = 1;
+  for (int i = 0; i < d; ++i) {
+    cmax = std::max(cmax, (long)t->ranks[i] * t->row_dims[i] * t->col_dims[i] * t->ranks[i + 1]);
+    rmax = std::max(rmax, (long)std::max(t->ranks[i], t->ranks[i + 1]));
+  }
+  Layout L;
+  const size_t oW = L.take(8 * cmax), oQ = L.take(8 * cmax), oR = L.take(8 * rmax * rmax), oT = L.take(8 * cmax),
+               otau = L.take(8 * rmax);
+  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), optr = L.take(sizeof(void*) * d);
+  if ((rc = ensure_dws(c, L.bytes))) return rc;
+  if ((rc = ensure_hst(c, L.bytes))) return rc;
+  char* db = reinterpret_cast<char*>(c->dws);
+  int* hmn = reinterpret_cast<int*>(c->hst + omn);
+  int* hrr = reinterpret_cast<int*>(c->hst + orr);
+  double** hp = reinterpret_cast<double**>(c->hst + optr);
+  for (int i = 0; i < d; ++i) {
+    hmn[i] = t->row_dims[i] * t->col_dims[i];
+    hp[i] = t->cores[i];
+  }
+  for (int i = 0; i <= d; ++i) hrr[i] = t->ranks[i];
+  HIPCHK(c, hipMemcpyAsync(db + omn, c->hst + omn, L.bytes - omn, hipMemcpyHostToDevice, st));
+  OrthoArgs oa;
+  oa.d = d;
+  oa.dir = dir;
+  oa.start = start;
+  oa.end = end;
+  oa.mn = reinterpret_cast<const int*>(db + omn);
+  oa.r = reinterpret_cast<int*>(db + orr);
+  oa.c = reinterpret_cast<double* const*>(db + optr);
+  oa.W = reinterpret_cast<double*>(db + oW);
+  oa.Qm = reinterpret_cast<double*>(db + oQ);
+  oa.Rc = reinterpret_cast<double*>(db + oR);
+  oa.T = reinterpret_cast<double*>(db + oT);
+  oa.tau = reinterpret_cast<double*>(db + otau);
+  hipLaunchKernelGGL(k_tt_ortho, dim3(1), dim3(NT), 0, st, oa);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->hst + orr, db + orr, sizeof(int) * (d + 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  for (int i = 0; i <= d; ++i) t->ranks[i] = hrr[i];
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,453 @@
+// Device building blocks of the TT-ALS engine (gfx950, wave64).
+//
+// Every routine here is executed by ONE whole workgroup of NT threads and
+// works on operands in global memory (HBM, L2-resident at these sizes),
+// staging tiles through LDS.  Routines end with a workgroup barrier so the
+// next routine sees their global writes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttals {
+
+constexpr int NT = 256;            // threads per workgroup: 4 x wave64
+constexpr int NWAVE = NT / 64;
+
+// ---------------------------------------------------------------------------
+// index maps: a matrix row (or column) index i is split into
+// (i / div, i % div) and mapped to  (i / div) * hi + (i % div) * lo.
+// This expresses every reshape/transpose of the interface-stack
+// contractions as a plain GEMM operand without copies.
+struct IMap {
+  int div, hi, lo;
+};
+__host__ __device__ inline IMap plain(int stride) { return IMap{1 << 30, 0, stride}; }
+__host__ __device__ inline IMap split(int div, int hi, int lo) { return IMap{div, hi, lo}; }
+__device__ __forceinline__ long imap(const IMap& m, int i) {
+  int h = i / m.div;
+  return (long)h * m.hi + (long)(i - h * m.div) * m.lo;
+}
+
+template <class T>
+struct CView {  // read-only matrix view
+  const T* p;
+  IMap r, c;
+  __device__ __forceinline__ T at(int i, int j) const { return p[imap(r, i) + imap(c, j)]; }
+};
+template <class T>
+struct MView {  // writable matrix view
+  T* p;
+  IMap r, c;
+  __device__ __forceinline__ T& at(int i, int j) const { return p[imap(r, i) + imap(c, j)]; }
+};
+
+template <class T>
+__device__ __forceinline__ CView<T> cview(const T* p, IMap r, IMap c) { return CView<T>{p, r, c}; }
+template <class T>
+__device__ __forceinline__ MView<T> mview(T* p, IMap r, IMap c) { return MView<T>{p, r, c}; }
+// row-major (rows x ld) matrix
+template <class T>
+__device__ __forceinline__ CView<T> crm(const T* p, int ld) { return CView<T>{p, plain(ld), plain(1)}; }
+template <class T>
+__device__ __forceinline__ MView<T> mrm(T* p, int ld) { return MView<T>{p, plain(ld), plain(1)}; }
+// transpose of a row-major (rows x ld) matrix
+template <class T>
+__device__ __forceinline__ CView<T> ctr(const T* p, int ld) { return CView<T>{p, plain(1), plain(ld)}; }
+
+// ---------------------------------------------------------------------------
+// reductions (fixed order: deterministic for a given shape)
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over the workgroup; every thread gets the result.  `red` is LDS
+// scratch of at least NWAVE doubles.
+__device__ __forceinline__ double wg_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NWAVE; ++i) s += red[i];
+  return s;
+}
+
+template <class T>
+__device__ double wg_dot(const T* x, const T* y, int n, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) s += (double)x[i] * (double)y[i];
+  return wg_sum(s, red);
+}
+
+template <class T>
+__device__ double wg_nrm2(const T* x, int n, double* red) {
+  // scaled to avoid overflow/underflow of the squares
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) mx = fmax(mx, fabs((double)x[i]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = mx;
+  __syncthreads();
+  mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < NWAVE; ++i) mx = fmax(mx, red[i]);
+  if (mx == 0.0 || !isfinite(mx)) return mx == 0.0 ? 0.0 : mx;
+  const double inv = 1.0 / mx;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    double t = (double)x[i] * inv;
+    s += t * t;
+  }
+  return mx * sqrt(wg_sum(s, red));
+}
+
+// y := a*x + b*y  (elementwise, no barrier)
+template <class T>
+__device__ __forceinline__ void wg_axpby(double a, const T* x, double b, T* y, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+template <class T, class U>
+__device__ __forceinline__ void wg_copy(const T* x, U* y, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) y[i] = (U)x[i];
+}
+template <class T>
+__device__ __forceinline__ void wg_fill(T* y, T v, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) y[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup GEMM  C[M x N] = A[M x K] * B[K x N]  (T = storage/compute type,
+// accumulation in ACC).  64x64 output tiles, K-steps of 16 staged in LDS;
+// each thread owns a 4x4 register micro-tile.
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GEMM_LDS_DOUBLES = GBK * (GBM + 1) + GBK * (GBN + 1);
+
+template <class T, class ACC>
+__device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
+  ACC* As = lds;                      // [GBK][GBM+1]
+  ACC* Bs = lds + GBK * (GBM + 1);    // [GBK][GBN+1]
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  // contiguous direction of each operand decides the load order (coalescing)
+  const bool a_kfast = (A.c.lo == 1 && A.c.div >= K);
+  const bool b_nfast = (B.c.lo == 1 && B.c.div >= N);
+  for (int m0 = 0; m0 < M; m0 += GBM) {
+    for (int n0 = 0; n0 < N; n0 += GBN) {
+      ACC acc[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = (ACC)0;
+      for (int k0 = 0; k0 < K; k0 += GBK) {
+        for (int e = tid; e < GBM * GBK; e += NT) {
+          int mi, ki;
+          if (a_kfast) { ki = e % GBK; mi = e / GBK; } else { mi = e % GBM; ki = e / GBM; }
+          const int gm = m0 + mi, gk = k0 + ki;
+          As[ki * (GBM + 1) + mi] = (gm < M && gk < K) ? (ACC)A.at(gm, gk) : (ACC)0;
+        }
+        for (int e = tid; e < GBN * GBK; e += NT) {
+          int ni, ki;
+          if (b_nfast) { ni = e % GBN; ki = e / GBN; } else { ki = e % GBK; ni = e / GBK; }
+          const int gn = n0 + ni, gk = k0 + ki;
+          Bs[ki * (GBN + 1) + ni] = (gn < N && gk < K) ? (ACC)B.at(gk, gn) : (ACC)0;
+        }
+        __syncthreads();
+        const int kmax = (K - k0) < GBK ? (K - k0) : GBK;
+        for (int kk = 0; kk < kmax; ++kk) {
+          ACC a[4], b[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) a[r] = As[kk * (GBM + 1) + ty + 16 * r];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) b[c] = Bs[kk * (GBN + 1) + tx + 16 * c];
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + ty + 16 * r;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int gn = n0 + tx + 16 * c;
+          if (gn < N) C.at(gm, gn) = (T)acc[r][c];
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Interface-stack contractions and the local matvec.
+//
+// Shapes: x core (a, n, b); operator core A (P, m, n, Q) with m == n;
+// op stacks L (a, P, a') / R (b, Q, b'); rhs core B (p, m, q); rhs stacks
+// Lb (p, a') / Rb (q, b').  All row-major.  t1/t2 are scratch of
+// max(P*a*n*b, a*n*Q*b) elements.
+
+// stage 1 shared by the matvec and the left stack:
+//   T1[a', b, P, n] = sum_a L[a, P, a'] v[a, n, b]
+template <class T, class ACC>
+__device__ void stage_left1(int a, int P, int n, int b, const T* L, const T* v, T* t1, ACC* lds) {
+  const int a2 = a;
+  wg_gemm<T, ACC>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
+                  mview<T>(t1, split(a2, n, b * P * n), split(b, 1, P * n)), lds);
+}
+
+// y[a', m, b'] = sum L[a,P,a'] A[P,m,n,Q] R[b,Q,b'] v[a,n,b]
+template <class T, class ACC>
+__device__ void local_matvec(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R,
+                             const T* v, T* y, T* t1, T* t2, ACC* lds) {
+  stage_left1<T, ACC>(a, P, n, b, L, v, t1, lds);
+  // T2[a', m, b, Q] = sum_{P,n} T1[(a',b),(P,n)] A[P,m,n,Q]
+  wg_gemm<T, ACC>(a * b, n * Q, P * n, crm<T>(t1, P * n),
+                  cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+                  mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)), lds);
+  // y[(a',m), b'] = T2[(a',m),(b,Q)] R[(b,Q), b']
+  wg_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b), lds);
+}
+
+// Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']
+template <class T, class ACC>
+__device__ void stack_left_op(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* x,
+                              T* Lout, T* t1, T* t2, ACC* lds) {
+  stage_left1<T, ACC>(a, P, n, b, L, x, t1, lds);
+  // T2[b, Q, a', m] = sum_{P,n} T1[(a',b),(P,n)] A[P,m,n,Q]
+  wg_gemm<T, ACC>(a * b, n * Q, P * n, crm<T>(t1, P * n),
+                  cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+                  mview<T>(t2, split(b, n, Q * a * n), split(Q, 1, a * n)), lds);
+  // Lout[(b,Q), b'] = T2[(b,Q),(a',m)] x[(a',m), b']
+  wg_gemm<T, ACC>(b * Q, b, a * n, crm<T>(t2, a * n), crm<T>(x, b), mrm<T>(Lout, b), lds);
+}
+
+// Rout[a, P, a'] = sum x[a,n,b] A[P,m,n,Q] R[b,Q,b'] x[a',m,b']
+template <class T, class ACC>
+__device__ void stack_right_op(int a, int P, int n, int Q, int b, const T* R, const T* Ac, const T* x,
+                               T* Rout, T* t1, T* t2, ACC* lds) {
+  // U1[a', b, m, Q] = sum_b' x[(a',m), b'] R[(b,Q), b']
+  wg_gemm<T, ACC>(a * n, b * Q, b, crm<T>(x, b), ctr<T>(R, b),
+                  mview<T>(t1, split(n, b * n * Q, Q), split(Q, n * Q, 1)), lds);
+  // U2[n, b, P, a'] = sum_{m,Q} U1[(a',b),(m,Q)] A[P,m,n,Q]
+  wg_gemm<T, ACC>(a * b, P * n, n * Q, crm<T>(t1, n * Q),
+                  cview<T>(Ac, split(Q, n * Q, 1), split(n, n * n * Q, Q)),
+                  mview<T>(t2, split(b, 1, P * a), split(n, a, b * P * a)), lds);
+  // Rout[a, (P,a')] = x[a,(n,b)] U2[(n,b),(P,a')]
+  wg_gemm<T, ACC>(a, P * a, n * b, crm<T>(x, n * b), crm<T>(t2, P * a), mrm<T>(Rout, P * a), lds);
+}
+
+// Lbout[q, b'] = sum Lb[p,a'] B[p,m,q] x[a',m,b']
+template <class T, class ACC>
+__device__ void stack_left_rhs(int a, int p, int n, int q, int b, const T* Lb, const T* Bc, const T* x,
+                               T* Lbout, T* t1, ACC* lds) {
+  // W[q, a', m] = sum_p Lb[p,a'] B[p,(m,q)]
+  wg_gemm<T, ACC>(a, n * q, p, ctr<T>(Lb, a), crm<T>(Bc, n * q),
+                  mview<T>(t1, plain(n), split(q, 1, a * n)), lds);
+  wg_gemm<T, ACC>(q, b, a * n, crm<T>(t1, a * n), crm<T>(x, b), mrm<T>(Lbout, b), lds);
+}
+
+// Rbout[p, a'] = sum B[p,m,q] Rb[q,b'] x[a',m,b']
+template <class T, class ACC>
+__device__ void stack_right_rhs(int a, int p, int n, int q, int b, const T* Rb, const T* Bc, const T* x,
+                                T* Rbout, T* t1, ACC* lds) {
+  // W[m, q, a'] = sum_b' x[(a',m), b'] Rb[q, b']
+  wg_gemm<T, ACC>(a * n, q, b, crm<T>(x, b), ctr<T>(Rb, b),
+                  mview<T>(t1, split(n, 1, q * a), plain(a)), lds);
+  wg_gemm<T, ACC>(p, a, n * q, crm<T>(Bc, n * q), crm<T>(t1, a), mrm<T>(Rbout, a), lds);
+}
+
+// f[a', m, b'] = sum Lb[p,a'] B[p,m,q] Rb[q,b']
+template <class T, class ACC>
+__device__ void micro_rhs(int a, int p, int n, int q, int b, const T* Lb, const T* Bc, const T* Rb, T* f,
+                          T* t1, ACC* lds) {
+  wg_gemm<T, ACC>(a, n * q, p, ctr<T>(Lb, a), crm<T>(Bc, n * q), mrm<T>(t1, n * q), lds);
+  wg_gemm<T, ACC>(a * n, b, q, crm<T>(t1, q), crm<T>(Rb, b), mrm<T>(f, b), lds);
+}
+
+// Dense local matrix M[(a',m,b'), (a,n,b)] = sum_{P,Q} L[a,P,a'] A[P,m,n,Q] R[b,Q,b']
+// written with leading dimension ldm (used for small local systems).
+template <class T>
+__device__ void micro_dense(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R, T* Mout,
+                            int ldm) {
+  const int N = a * n * b;
+  const long NN = (long)N * N;
+  for (long e = threadIdx.x; e < NN; e += NT) {
+    const int row = (int)(e / N), col = (int)(e - (long)row * N);
+    const int a2 = row / (n * b), m = (row / b) % n, b2 = row % b;
+    const int a1 = col / (n * b), nn = (col / b) % n, b1 = col % b;
+    double s = 0.0;
+    for (int pp = 0; pp < P; ++pp) {
+      const double l = (double)L[(a1 * P + pp) * a + a2];
+      if (l == 0.0) continue;
+      for (int qq = 0; qq < Q; ++qq)
+        s += l * (double)Ac[((pp * n + m) * n + nn) * Q + qq] * (double)R[(b1 * Q + qq) * b + b2];
+    }
+    Mout[(long)row * ldm + col] = (T)s;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR of a row-major m x c matrix W (ld = c), in place.
+// On exit W holds R on/above the diagonal and the reflectors below it
+// (unit leading entry implied); tau[0..k) with k = min(m, c).
+__device__ void wg_householder(double* W, int m, int c, double* tau, double* red) {
+  const int k = m < c ? m : c;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = 0; j < k; ++j) {
+    double s2 = 0.0;
+    for (int i = j + 1 + threadIdx.x; i < m; i += NT) {
+      const double t = W[(long)i * c + j];
+      s2 += t * t;
+    }
+    s2 = wg_sum(s2, red);
+    const double alpha = W[(long)j * c + j];
+    double t;
+    if (s2 == 0.0) {
+      t = 0.0;  // H_j = I (LAPACK convention for an already-reduced column)
+    } else {
+      const double nrm = sqrt(alpha * alpha + s2);
+      const double beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int i = j + 1 + threadIdx.x; i < m; i += NT) W[(long)i * c + j] *= sc;
+      __syncthreads();
+      if (threadIdx.x == 0) W[(long)j * c + j] = beta;
+    }
+    if (threadIdx.x == 0) tau[j] = t;
+    __syncthreads();
+    if (t != 0.0) {
+      // apply H_j to columns j+1..c-1: one wave per column
+      for (int col = j + 1 + w; col < c; col += NWAVE) {
+        double s = 0.0;
+        for (int i = j + lane; i < m; i += 64) {
+          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+          s += v * W[(long)i * c + col];
+        }
+        s = wave_sum(s) * t;
+        for (int i = j + lane; i < m; i += 64) {
+          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+          W[(long)i * c + col] -= v * s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Explicit Q (m x k, row-major, ld = k) from the reflectors in W.
+__device__ void wg_form_q(const double* W, int m, int c, const double* tau, double* Qm) {
+  const int k = m < c ? m : c;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (long e = threadIdx.x; e < (long)m * k; e += NT) {
+    const int i = (int)(e / k), j = (int)(e % k);
+    Qm[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const double t = tau[j];
+    if (t != 0.0) {
+      for (int col = j + w; col < k; col += NWAVE) {
+        double s = 0.0;
+        for (int i = j + lane; i < m; i += 64) {
+          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+          s += v * Qm[(long)i * k + col];
+        }
+        s = wave_sum(s) * t;
+        for (int i = j + lane; i < m; i += 64) {
+          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+          Qm[(long)i * k + col] -= v * s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense LU with partial pivoting of an N x N row-major matrix in LDS/global
+// memory (ld = N) and solve with right-hand side f (overwritten by the
+// solution).  Returns false if an exactly zero pivot is met.
+__device__ bool wg_lu_solve(double* Mx, int N, double* f, int* piv, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ int s_ip[NWAVE];
+  __shared__ double s_iv[NWAVE];
+  bool ok = true;
+  for (int k = 0; k < N; ++k) {
+    // pivot search: max |M[i][k]|, smallest index on ties
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + threadIdx.x; i < N; i += NT) {
+      const double v = fabs(Mx[(long)i * N + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    __syncthreads();
+    if (lane == 0) { s_iv[w] = best; s_ip[w] = bi; }
+    __syncthreads();
+    best = s_iv[0];
+    bi = s_ip[0];
+    for (int q = 1; q < NWAVE; ++q)
+      if (s_iv[q] > best || (s_iv[q] == best && s_ip[q] < bi)) { best = s_iv[q]; bi = s_ip[q]; }
+    if (threadIdx.x == 0) piv[k] = bi;
+    if (best == 0.0) { ok = false; }
+    if (bi != k) {
+      for (int j = threadIdx.x; j < N; j += NT) {
+        const double t = Mx[(long)k * N + j];
+        Mx[(long)k * N + j] = Mx[(long)bi * N + j];
+        Mx[(long)bi * N + j] = t;
+      }
+      if (threadIdx.x == 0) { const double t = f[k]; f[k] = f[bi]; f[bi] = t; }
+    }
+    __syncthreads();
+    if (best != 0.0) {
+      const double inv = 1.0 / Mx[(long)k * N + k];
+      for (int i = k + 1 + threadIdx.x; i < N; i += NT) Mx[(long)i * N + k] *= inv;
+      __syncthreads();
+      const int rem = N - k - 1;
+      for (long e = threadIdx.x; e < (long)rem * rem; e += NT) {
+        const int i = k + 1 + (int)(e / rem), j = k + 1 + (int)(e % rem);
+        Mx[(long)i * N + j] -= Mx[(long)i * N + k] * Mx[(long)k * N + j];
+      }
+    }
+    __syncthreads();
+  }
+  if (!ok) return false;
+  // forward substitution (unit lower), then back substitution
+  for (int k = 0; k < N; ++k) {
+    const double fk = f[k];
+    for (int i = k + 1 + threadIdx.x; i < N; i += NT) f[i] -= Mx[(long)i * N + k] * fk;
+    __syncthreads();
+  }
+  for (int k = N - 1; k >= 0; --k) {
+    if (threadIdx.x == 0) f[k] /= Mx[(long)k * N + k];
+    __syncthreads();
+    const double fk = f[k];
+    for (int i = threadIdx.x; i < k; i += NT) f[i] -= Mx[(long)i * N + k] * fk;
+    __syncthreads();
+  }
+  return true;
+}
+
+}  // namespace ttals

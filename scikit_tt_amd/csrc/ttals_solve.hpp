@@ -1,0 +1,453 @@
+// Local solvers of the TT-ALS engine: matrix-free CG / GMRES(m) for the
+// local linear system and restarted Lanczos for the extreme local
+// eigenpair.  One workgroup per local problem; vectors in global memory.
+#pragma once
+#include "ttals_dev.hpp"
+
+namespace ttals {
+
+// The local operator M = (L x A x R) restricted to one core position.
+template <class T>
+struct LocalOp {
+  int a, P, n, Q, b;
+  const T* L;
+  const T* A;
+  const T* R;
+  T* t1;
+  T* t2;
+  __device__ int size() const { return a * n * b; }
+};
+
+template <class T, class ACC>
+__device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y, ACC* lds) {
+  local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);
+}
+
+// LDS scratch used by the solvers (besides the GEMM tiles).
+constexpr int KMAX = 48;  // Krylov basis size (GMRES restart / Lanczos)
+struct SolveLDS {
+  double red[NWAVE];
+  double h[KMAX + 2];
+  double H[(KMAX + 1) * KMAX];
+  double cs[KMAX + 1], sn[KMAX + 1], g[KMAX + 2];
+  double alpha[KMAX + 1], beta[KMAX + 1], s[KMAX + 1];
+  double scal[4];
+  int flag[4];
+};
+
+// h[i] = <V_i, w> for i < cnt, one wave per dot product.
+__device__ void wg_multidot(const double* V, long ldv, int cnt, const double* w, int N, double* h) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = wv; i < cnt; i += NWAVE) {
+    double s = 0.0;
+    const double* vi = V + (long)i * ldv;
+    for (int e = lane; e < N; e += 64) s += vi[e] * w[e];
+    s = wave_sum(s);
+    if (lane == 0) h[i] = s;
+  }
+  __syncthreads();
+}
+
+// w -= sum_i h[i] V_i   (no barrier)
+__device__ __forceinline__ void wg_multiaxpy(const double* V, long ldv, int cnt, const double* h, double* w,
+                                             int N) {
+  for (int e = threadIdx.x; e < N; e += NT) {
+    double s = 0.0;
+    for (int i = 0; i < cnt; ++i) s += h[i] * V[(long)i * ldv + e];
+    w[e] -= s;
+  }
+}
+
+// Iterated classical Gram-Schmidt with the DGKS criterion: another pass is
+// made while a pass shrinks w by more than a factor 0.7 (at most 4 passes),
+// so that even a near-breakdown residual made of rounding noise leaves
+// orthogonal to the basis.  Accumulates the coefficients in hsum; returns
+// ||w|| after orthogonalisation and the norm before it in *w0.
+__device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, double* h, double* hsum,
+                          double* red, double* w0) {
+  for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] = 0.0;
+  double prev = wg_nrm2(w, N, red);
+  *w0 = prev;
+  double nrm = prev;
+  for (int pass = 0; pass < 4; ++pass) {
+    wg_multidot(V, ldv, cnt, w, N, h);
+    wg_multiaxpy(V, ldv, cnt, h, w, N);
+    for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] += h[i];
+    __syncthreads();
+    nrm = wg_nrm2(w, N, red);
+    if (pass >= 1 && nrm > 0.7 * prev) break;
+    prev = nrm;
+  }
+  return nrm;
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate gradients.  x holds the initial guess on entry.  Returns 0 on
+// convergence, 1 if p^T M p <= 0 was met (not SPD: caller switches to
+// GMRES), 2 if maxit was reached.
+template <class ACC>
+__device__ int wg_cg(const LocalOp<double>& op, const double* f, double* x, double* work, double tol, int maxit,
+                     ACC* lds, SolveLDS& S, int* iters) {
+  const int N = op.size();
+  double* r = work;
+  double* p = work + N;
+  double* q = work + 2 * N;
+  const double fn = wg_nrm2(f, N, S.red);
+  if (fn == 0.0) {
+    wg_fill(x, 0.0, N);
+    __syncthreads();
+    return 0;
+  }
+  for (int restart = 0; restart < 3; ++restart) {
+    apply_op<double, ACC>(op, x, q, lds);
+    for (int e = threadIdx.x; e < N; e += NT) r[e] = f[e] - q[e], p[e] = r[e];
+    double rr = wg_dot(r, r, N, S.red);
+    if (sqrt(rr) <= tol * fn) return 0;
+    while (*iters < maxit) {
+      apply_op<double, ACC>(op, p, q, lds);
+      const double pq = wg_dot(p, q, N, S.red);
+      if (!(pq > 0.0)) return 1;
+      const double al = rr / pq;
+      for (int e = threadIdx.x; e < N; e += NT) x[e] += al * p[e], r[e] -= al * q[e];
+      const double rn = wg_dot(r, r, N, S.red);
+      ++*iters;
+      if (sqrt(rn) <= tol * fn) break;
+      const double be = rn / rr;
+      rr = rn;
+      for (int e = threadIdx.x; e < N; e += NT) p[e] = r[e] + be * p[e];
+      __syncthreads();
+    }
+    // true residual check (the recurrence drifts)
+    apply_op<double, ACC>(op, x, q, lds);
+    for (int e = threadIdx.x; e < N; e += NT) r[e] = f[e] - q[e];
+    const double tr = wg_nrm2(r, N, S.red);
+    if (tr <= 10.0 * tol * fn) return 0;
+    if (*iters >= maxit) return 2;
+  }
+  return 2;
+}
+
+// ---------------------------------------------------------------------------
+// Restarted GMRES(m) with CGS2 orthogonalisation.  V: (m+1) x N scratch.
+template <class ACC>
+__device__ int wg_gmres(const LocalOp<double>& op, const double* f, double* x, double* V, double tol, int maxit,
+                        int m, ACC* lds, SolveLDS& S, int* iters) {
+  const int N = op.size();
+  if (m > KMAX) m = KMAX;
+  if (m > N) m = N;
+  const double fn = wg_nrm2(f, N, S.red);
+  if (fn == 0.0) {
+    wg_fill(x, 0.0, N);
+    __syncthreads();
+    return 0;
+  }
+  for (;;) {
+    double* v0 = V;
+    apply_op<double, ACC>(op, x, v0, lds);
+    for (int e = threadIdx.x; e < N; e += NT) v0[e] = f[e] - v0[e];
+    __syncthreads();
+    const double beta = wg_nrm2(v0, N, S.red);
+    if (beta <= tol * fn) return 0;
+    if (*iters >= maxit) return 2;
+    const double ib = 1.0 / beta;
+    for (int e = threadIdx.x; e < N; e += NT) v0[e] *= ib;
+    if (threadIdx.x == 0) {
+      S.g[0] = beta;
+      for (int i = 1; i <= m; ++i) S.g[i] = 0.0;
+    }
+    __syncthreads();
+    int jend = 0;
+    double res = beta;
+    for (int j = 0; j < m; ++j) {
+      double* w = V + (long)(j + 1) * N;
+      apply_op<double, ACC>(op, V + (long)j * N, w, lds);
+      double w0;
+      double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.red, &w0);
+      if (!(hn > 1e-14 * w0)) hn = 0.0;  // (lucky) breakdown: Krylov space invariant
+      if (hn > 0.0) {
+        const double ih = 1.0 / hn;
+        for (int e = threadIdx.x; e < N; e += NT) w[e] *= ih;
+      }
+      if (threadIdx.x == 0) {
+        double* col = S.H + j * (KMAX + 1);  // column j, rows 0..j+1
+        col[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+          const double t = S.cs[i] * col[i] + S.sn[i] * col[i + 1];
+          col[i + 1] = -S.sn[i] * col[i] + S.cs[i] * col[i + 1];
+          col[i] = t;
+        }
+        const double h0 = col[j], h1 = col[j + 1];
+        const double rr = sqrt(h0 * h0 + h1 * h1);
+        const double c = rr == 0.0 ? 1.0 : h0 / rr, s = rr == 0.0 ? 0.0 : h1 / rr;
+        S.cs[j] = c;
+        S.sn[j] = s;
+        col[j] = rr;
+        col[j + 1] = 0.0;
+        S.g[j + 1] = -s * S.g[j];
+        S.g[j] = c * S.g[j];
+        S.scal[0] = fabs(S.g[j + 1]);
+      }
+      __syncthreads();
+      res = S.scal[0];
+      ++*iters;
+      jend = j + 1;
+      if (res <= tol * fn || hn == 0.0 || *iters >= maxit) break;
+    }
+    // y = H^{-1} g  (upper triangular jend x jend), x += V y
+    if (threadIdx.x == 0) {
+      for (int i = jend - 1; i >= 0; --i) {
+        double t = S.g[i];
+        for (int k = i + 1; k < jend; ++k) t -= S.H[k * (KMAX + 1) + i] * S.h[k];
+        const double d = S.H[i * (KMAX + 1) + i];
+        S.h[i] = d != 0.0 ? t / d : 0.0;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < N; e += NT) {
+      double t = 0.0;
+      for (int i = 0; i < jend; ++i) t += S.h[i] * V[(long)i * N + e];
+      x[e] += t;
+    }
+    __syncthreads();
+    if (res <= tol * fn) {
+      // confirm with the true residual once
+      double* rv = V;
+      apply_op<double, ACC>(op, x, rv, lds);
+      for (int e = threadIdx.x; e < N; e += NT) rv[e] = f[e] - rv[e];
+      __syncthreads();
+      if (wg_nrm2(rv, N, S.red) <= 10.0 * tol * fn) return 0;
+    }
+    if (*iters >= maxit) return 2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Extreme eigenpair of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
+// Sturm-count multisection by one wave (64 shifts per round), then inverse
+// iteration for the eigenvector.  `which`: 0 = lowest, 1 = highest, or an
+// explicit eigenvalue index via which_idx >= 0.
+
+__device__ __forceinline__ int sturm_count(const double* al, const double* be, int m, double x) {
+  int c = 0;
+  double q = al[0] - x;
+  if (q < 0.0) ++c;
+  for (int i = 1; i < m; ++i) {
+    const double qq = (q == 0.0) ? 1e-300 : q;
+    q = (al[i] - x) - be[i - 1] * be[i - 1] / qq;
+    if (q < 0.0) ++c;
+  }
+  return c;
+}
+
+// eigenvalue with index idx (0 = smallest) of T; executed by wave 0, result
+// broadcast through S.scal[1].
+__device__ void tri_eigval(const double* al, const double* be, int m, int idx, SolveLDS& S) {
+  if ((threadIdx.x >> 6) == 0) {
+    const int lane = threadIdx.x & 63;
+    double lo = 1e300, hi = -1e300;
+    for (int i = 0; i < m; ++i) {
+      const double r = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i < m - 1 ? fabs(be[i]) : 0.0);
+      lo = fmin(lo, al[i] - r);
+      hi = fmax(hi, al[i] + r);
+    }
+    const double span = fmax(hi - lo, 1e-300);
+    lo -= 1e-12 * span + 1e-300;
+    hi += 1e-12 * span + 1e-300;
+    // invariant: count(lo) <= idx < count(hi)
+    for (int round = 0; round < 40; ++round) {
+      const double h = (hi - lo) / 65.0;
+      const double xpt = lo + h * (lane + 1);
+      const int c = sturm_count(al, be, m, xpt);
+      // largest lane with count <= idx
+      unsigned long long ball = __ballot(c <= idx);
+      const int nle = __popcll(ball);  // counts are monotone in lane
+      const double nlo = nle > 0 ? lo + h * nle : lo;
+      const double nhi = nle < 64 ? lo + h * (nle + 1) : hi;
+      lo = nlo;
+      hi = nhi;
+      if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+    }
+    if (lane == 0) S.scal[1] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+}
+
+// eigenvector of T for eigenvalue th by inverse iteration (thread 0).
+__device__ void tri_eigvec(const double* al, const double* be, int m, double th, double* s, double* tmpd,
+                           double* tmpu) {
+  if (threadIdx.x == 0) {
+    double anorm = 0.0;
+    for (int i = 0; i < m; ++i) anorm = fmax(anorm, fabs(al[i]) + (i < m - 1 ? fabs(be[i]) : 0.0));
+    const double pert = 1e-15 * fmax(anorm, 1e-300);
+    for (int i = 0; i < m; ++i) s[i] = 1.0 / sqrt((double)m) * (1.0 + 0.01 * (double)((i * 7) % 5));
+    for (int it = 0; it < 3; ++it) {
+      // solve (T - th I) z = s with partial pivoting-free LDL (perturbed pivots)
+      double dprev = 0.0, uprev = 0.0;
+      for (int i = 0; i < m; ++i) {
+        double di = al[i] - th;
+        if (i > 0) {
+          const double l = be[i - 1] / dprev;
+          di -= l * be[i - 1];
+          s[i] -= l * uprev;
+        }
+        if (fabs(di) < pert) di = (di < 0.0 ? -pert : pert);
+        tmpd[i] = di;
+        dprev = di;
+        uprev = s[i];
+      }
+      for (int i = m - 1; i >= 0; --i) {
+        double t = s[i];
+        if (i < m - 1) t -= be[i] * s[i + 1];
+        s[i] = t / tmpd[i];
+      }
+      double nn = 0.0;
+      for (int i = 0; i < m; ++i) nn += s[i] * s[i];
+      nn = 1.0 / sqrt(nn);
+      for (int i = 0; i < m; ++i) s[i] *= nn;
+    }
+    (void)tmpu;
+  }
+  __syncthreads();
+}
+
+// Restarted Lanczos with full re-orthogonalisation (CGS2) for one extreme
+// eigenpair of the symmetric local operator.
+//   mode 0: lowest, 1: highest, 2: nearest sigma (lowest/highest chosen
+//   from the first Ritz spectrum; the choice is checked at the end).
+// v: start vector on entry, unit eigenvector on exit.  V: (KMAX+1) x N.
+// Returns 0 converged, 2 not converged, 3 sigma lies inside the spectrum of
+// a problem too large for the Krylov space (unsupported).
+template <class ACC>
+__device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int mode, double sigma, double tol,
+                          int maxit, int kmax, ACC* lds, SolveLDS& S, double* lambda, int* iters) {
+  const int N = op.size();
+  if (kmax > KMAX) kmax = KMAX;
+  double vn = wg_nrm2(v, N, S.red);
+  if (!(vn > 0.0) || !isfinite(vn)) {
+    for (int e = threadIdx.x; e < N; e += NT) v[e] = 1.0 + 0.1 * (double)((e * 2654435761u) % 97) / 97.0;
+    __syncthreads();
+    vn = wg_nrm2(v, N, S.red);
+  }
+  int target = (mode == 1) ? 1 : 0;
+  bool decided = (mode != 2);
+  int status = 2;
+  double th = 0.0;
+  for (int restart = 0; restart < 1000; ++restart) {
+    const double iv = 1.0 / vn;
+    for (int e = threadIdx.x; e < N; e += NT) V[e] = v[e] * iv;
+    __syncthreads();
+    int m = 0;
+    bool conv = false;
+    bool full = false;
+    const int kk = kmax < N ? kmax : N;
+    for (int j = 0; j < kk; ++j) {
+      double* w = V + (long)(j + 1) * N;
+      apply_op<double, ACC>(op, V + (long)j * N, w, lds);
+      double w0;
+      double bn = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.red, &w0);
+      if (threadIdx.x == 0) {
+        S.alpha[j] = S.H[j];
+        S.beta[j] = 0.0;
+      }
+      __syncthreads();
+      ++*iters;
+      m = j + 1;
+      double anorm = 0.0;
+      for (int i = 0; i <= j; ++i)
+        anorm = fmax(anorm, fabs(S.alpha[i]) + (i > 0 ? fabs(S.beta[i - 1]) : 0.0));
+      // breakdown: the residual is at the rounding level of M v_j
+      const bool brk = !(bn > 1e-12 * fmax(w0, 1e-300));
+      full = (m == N);
+      if (brk && !full && m < kk) {
+        // invariant subspace: continue with a fresh direction orthogonal to
+        // the basis (T stays tridiagonal with a zero coupling)
+        for (int e = threadIdx.x; e < N; e += NT)
+          w[e] = (double)((((unsigned)e + 1u) * 2654435761u + ((unsigned)j + 1u) * 40503u) % 1009u) / 1009.0 - 0.5;
+        __syncthreads();
+        double r0;
+        bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.red, &r0);
+        if (bn > 0.0) {
+          const double ib = 1.0 / bn;
+          for (int e = threadIdx.x; e < N; e += NT) w[e] *= ib;
+        }
+        __syncthreads();
+        continue;
+      }
+      if (threadIdx.x == 0) S.beta[j] = brk ? 0.0 : bn;
+      __syncthreads();
+      const bool check = full || brk || (m == kk) || (m >= 4 && (m % 4) == 0) || *iters >= maxit;
+      if (check) {
+        if (!decided) {
+          tri_eigval(S.alpha, S.beta, m, 0, S);
+          const double tlo = S.scal[1];
+          tri_eigval(S.alpha, S.beta, m, m - 1, S);
+          const double thi = S.scal[1];
+          target = (sigma <= 0.5 * (tlo + thi)) ? 0 : 1;
+          decided = true;
+        }
+        int idx = target == 0 ? 0 : m - 1;
+        if (full && mode == 2) {
+          // the whole space is spanned: nearest sigma among all Ritz values;
+          // the Sturm count of sigma splits the spectrum
+          __syncthreads();
+          if (threadIdx.x == 0) S.flag[0] = sturm_count(S.alpha, S.beta, m, sigma);
+          __syncthreads();
+          const int c = S.flag[0];
+          double best = 0.0;
+          int bidx = 0;
+          if (c > 0) {
+            tri_eigval(S.alpha, S.beta, m, c - 1, S);
+            best = S.scal[1];
+            bidx = c - 1;
+          }
+          if (c < m) {
+            tri_eigval(S.alpha, S.beta, m, c, S);
+            const double up = S.scal[1];
+            if (c == 0 || fabs(up - sigma) < fabs(best - sigma)) bidx = c;
+          }
+          idx = bidx;
+        }
+        tri_eigval(S.alpha, S.beta, m, idx, S);
+        th = S.scal[1];
+        tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs, S.sn);
+        const double res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
+        if (res <= tol * fmax(fabs(th), 1e-3 * anorm)) { conv = true; break; }
+        if (*iters >= maxit) break;
+      }
+      const double ib = 1.0 / bn;
+      for (int e = threadIdx.x; e < N; e += NT) w[e] *= ib;
+      __syncthreads();
+    }
+    // Ritz vector
+    for (int e = threadIdx.x; e < N; e += NT) {
+      double t = 0.0;
+      for (int i = 0; i < m; ++i) t += S.s[i] * V[(long)i * N + e];
+      v[e] = t;
+    }
+    __syncthreads();
+    vn = wg_nrm2(v, N, S.red);
+    if (conv) {
+      const double iv2 = 1.0 / vn;
+      for (int e = threadIdx.x; e < N; e += NT) v[e] *= iv2;
+      __syncthreads();
+      status = 0;
+      if (mode == 2 && !full) {
+        // nearest-sigma with an extreme eigenvalue is only valid if sigma lies
+        // outside the spectrum on that side
+        if ((target == 0 && sigma > th) || (target == 1 && sigma < th)) status = 3;
+      }
+      break;
+    }
+    if (*iters >= maxit) {
+      const double iv2 = 1.0 / vn;
+      for (int e = threadIdx.x; e < N; e += NT) v[e] *= iv2;
+      __syncthreads();
+      status = 2;
+      break;
+    }
+  }
+  *lambda = th;
+  return status;
+}
+
+}  // namespace ttals

@@ -1,0 +1,75 @@
+"""Shared host plumbing for the fused ALS engine calls (libttals.so)."""
+import warnings
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..tensor_train import TT
+
+
+class SolveInfo:
+    """Per-system engine report: status, sweeps, Krylov iterations, eigenvalue."""
+
+    __slots__ = ('status', 'sweeps', 'unconverged', 'max_iters', 'krylov_iters', 'local_solves', 'eigenvalue',
+                 'worst_residual')
+
+    def __init__(self, r):
+        self.status = int(r.status)
+        self.sweeps = int(r.sweeps)
+        self.unconverged = int(r.unconverged)
+        self.krylov_iters = int(r.krylov_iters)
+        self.local_solves = int(r.local_solves)
+        self.max_iters = int(r.max_iters)
+        self.eigenvalue = float(r.lambda_)
+        self.worst_residual = float(r.worst_residual)
+
+    def __repr__(self):
+        return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in self.__slots__)
+
+
+def check_tts(*groups):
+    for g in groups:
+        for t in g:
+            if not isinstance(t, TT):
+                raise TypeError('Unsupported argument.')
+
+
+def descs(tts):
+    packs = [t._desc() for t in tts]
+    return packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
+
+
+def outputs(initial_guesses):
+    """Output cores with the capacity of the initial guesses (ranks never grow)."""
+    cores = [[torch.empty_like(c) for c in x.device_cores] for x in initial_guesses]
+    packs = [_lib.DescPack(x.order, x.row_dims, x.col_dims, x.ranks, [c.data_ptr() for c in oc])
+             for x, oc in zip(initial_guesses, cores)]
+    return cores, packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
+
+
+def finish(cores, pack, row_dims):
+    """View the engine's in-place cores with their final ranks."""
+    ranks = pack.final_ranks()
+    out = []
+    for i, c in enumerate(cores):
+        shape = (ranks[i], row_dims[i], 1, ranks[i + 1])
+        if tuple(c.shape) == shape:
+            out.append(c)
+        else:
+            out.append(c.reshape(-1)[:int(np.prod(shape))].reshape(shape).clone())
+    return TT._from_device(out)
+
+
+def raise_status(info, what):
+    """Map an engine status to the reference's exception (or a warning)."""
+    st = info.status
+    if st == _lib.S_SINGULAR:
+        raise np.linalg.LinAlgError('Singular matrix (%s)' % what)
+    if st == _lib.S_NONFINITE:
+        raise np.linalg.LinAlgError('non-finite local solution (%s)' % what)
+    if st == _lib.S_UNSUPPORTED:
+        raise NotImplementedError('%s: the eigenvalue nearest sigma lies inside the spectrum of a local problem '
+                                  'larger than the Krylov space' % what)
+    if st == _lib.S_NOCONV:
+        warnings.warn('%s: %d local Krylov solve(s) stopped at maxit' % (what, info.unconverged), RuntimeWarning)

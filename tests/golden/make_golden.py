@@ -1,0 +1,103 @@
+"""Write golden vectors for sle.als / evp.als with the reference itself.
+
+Run in the build container only (the reference is not available elsewhere):
+
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py /root/reference
+
+Only the two solvers are recorded here: their exact iterates (which local
+solution, which orthonormal factor, best-so-far tracking, the convergence
+rule and the iteration count) are pinned by nothing but the reference.  The
+TT algebra, stacks, micro matrices and local solvers are pinned by dense
+brute force and closed forms in tests/test_oracle.py instead.
+
+Each case stores its inputs (so the fixture does not depend on this repo's
+input builders), the returned cores and ranks, and gauge-invariant scalars.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def _save_tt(store, prefix, cores):
+    store[prefix + '_order'] = np.array(len(cores))
+    for i, c in enumerate(cores):
+        store['%s_%d' % (prefix, i)] = np.ascontiguousarray(c)
+
+
+def main(ref_root):
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    from scikit_tt.tensor_train import TT
+    import scikit_tt.tensor_train as rtt
+    import scikit_tt.solvers.sle as rsle
+    import scikit_tt.solvers.evp as revp
+    from scikit_tt_amd import workloads as W
+
+    out = {}
+
+    # 1. sle.als, -Laplacian d=10 n=8, b = ones, x0 N(0,1) rank 5 (rng 0), 4 sweeps, 'solve'
+    A, x0, b = W.laplace_small()
+    x = rsle.als(TT(A), TT(x0), TT(b), repeats=4, solver='solve')
+    s = {}
+    _save_tt(s, 'A', A)
+    _save_tt(s, 'x0', x0)
+    _save_tt(s, 'b', b)
+    _save_tt(s, 'x', x.cores)
+    s['ranks'] = np.array(x.ranks)
+    s['relres'] = np.array((TT(A).dot(x) - TT(b)).norm() / TT(b).norm())
+    s['norm'] = np.array(x.norm())
+    np.savez_compressed(os.path.join(HERE, 'sle_laplace_d10.npz'), **s)
+    out['sle_laplace_d10'] = float(s['relres'])
+
+    # 2. sle.als, indefinite Toeplitz operator (reference test pattern,
+    #    tests/test_sle.py:20-35, made smaller): toeplitz(1..64) as a TT
+    #    operator with [2]*6 modes, truncated TT-SVD; x0 = ones rank 3,
+    #    ortho_right; 2 sweeps, 'lu'
+    import scipy.linalg as sla
+    mat = sla.toeplitz(np.arange(1, 65), np.arange(1, 65)).astype(float)
+    op = TT(mat.reshape([2] * 12), threshold=1e-12)
+    x0t = rtt.ones(op.row_dims, [1] * 6, ranks=3).ortho_right()
+    bt = rtt.ones(op.row_dims, [1] * 6)
+    xt = rsle.als(op, x0t, bt, repeats=2, solver='lu')
+    s = {}
+    _save_tt(s, 'A', op.cores)
+    _save_tt(s, 'x0', x0t.cores)
+    _save_tt(s, 'b', bt.cores)
+    _save_tt(s, 'x', xt.cores)
+    s['ranks'] = np.array(xt.ranks)
+    s['dense_solution'] = np.linalg.solve(mat, np.ones(64))
+    np.savez_compressed(os.path.join(HERE, 'sle_toeplitz_d6.npz'), **s)
+
+    # 3. evp.als 'eig' sigma=-100 (lowest), TFIM d=8 rank 8, repeats=8,
+    #    conv_eps=1e-12 (the convergence rule may stop early)
+    for tag, d, r, reps, eps, solver, sigma in [('evp_tfim_d8_eig', 8, 8, 8, 1e-12, 'eig', -100.0),
+                                                ('evp_tfim_d8_eigh', 8, 6, 5, 0.0, 'eigh', 1.0),
+                                                ('evp_tfim_d12_conv', 12, 6, 10, 1e-9, 'eig', -100.0)]:
+        H, y0, J, g = W.tfim_chain(seed=3, d=d, rank=r)
+        y0t = TT(y0).ortho_right()
+        y0t = (1.0 / y0t.norm()) * y0t
+        lam, y, it = revp.als(TT(H), y0t, repeats=reps, conv_eps=eps, solver=solver, sigma=sigma)
+        s = {}
+        _save_tt(s, 'A', H)
+        _save_tt(s, 'x0', y0t.cores)
+        _save_tt(s, 'x', y.cores)
+        s['ranks'] = np.array(y.ranks)
+        s['eigenvalue'] = np.array(float(np.real(lam)))
+        s['iterations'] = np.array(it)
+        s['repeats'] = np.array(reps)
+        s['conv_eps'] = np.array(eps)
+        s['sigma'] = np.array(sigma)
+        s['solver'] = np.array(solver)
+        s['J'] = J
+        s['g'] = g
+        np.savez_compressed(os.path.join(HERE, tag + '.npz'), **s)
+        out[tag] = (float(np.real(lam)), int(it))
+    print(out)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
