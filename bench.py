@@ -1,0 +1,165 @@
+"""Benchmark: TT-ALS solves/s on one MI355X.
+
+Workload (one "step" = one pass of the hot path over one batch): a batch of
+B independent transverse-field Ising chains, d = 32 sites, n = 2, MPO rank
+3, couplings J_i, g_i ~ U[0.5, 1.5] (seed = instance index), random N(0,1)
+initial TT of rank 16 right-orthonormalised and normalised, evp.als lowest
+eigenpair ('eig', sigma = -100), 6 sweeps, solved in ONE engine launch.
+
+Printed (one JSON line): batched solves/s (value), ms per step, the plain
+drop-in path (one system per call) solves/s, eigenvalue agreement with the
+per-system FP64 result and with the free-fermion closed form, per-step
+latency p50/p99, the roofline of the engine kernel (algorithmic FP64 flops
+counted from shapes and Krylov iteration counts / launch time measured with
+events on the launch stream) and a CPU baseline: the numpy oracle timed on
+one sweep of one instance on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix peak
+PEAK_HBM_GBS = 8000.0
+
+
+def build_batch(B, d, rank, seed0=0):
+    from scikit_tt_amd import workloads as W
+    from scikit_tt_amd.tensor_train import TT
+    ops, x0s, exact, raw = [], [], [], []
+    for s in range(seed0, seed0 + B):
+        H, y0, J, g = W.tfim_chain(seed=s, d=d, rank=rank)
+        t = TT(y0).ortho_right()
+        t = (1.0 / t.norm()) * t
+        ops.append(TT(H))
+        x0s.append(t)
+        exact.append(W.tfim_ground_energy(J, g))
+        raw.append((H, y0))
+    return ops, x0s, np.array(exact), raw
+
+
+def cpu_baseline(raw_instance, sweeps):
+    """numpy oracle, one sweep of one instance on this host (extrapolated to `sweeps`)."""
+    from oracle import tt_ops, evp_als
+    H, y0 = raw_instance
+    y0 = tt_ops.ortho_right(y0)
+    y0 = tt_ops.scale(y0, 1.0 / tt_ops.norm(y0))
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([p.get('num_threads', 1) for p in threadpool_info() if p.get('user_api') == 'blas'] or [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count() or 1
+    t = time.perf_counter()
+    evp_als.als(H, y0, repeats=1, conv_eps=0, solver='eig', sigma=-100)
+    dt = time.perf_counter() - t
+    return {'value': 1.0 / (dt * sweeps), 'unit': 'solves/s', 'cores': int(threads), 'kind': 'port',
+            'sample': '1 of %d sweeps of one instance (numpy oracle, dense local eig), x%d extrapolated; '
+                      '%.2f s measured' % (sweeps, sweeps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=512)
+    ap.add_argument('--sites', type=int, default=32)
+    ap.add_argument('--rank', type=int, default=16)
+    ap.add_argument('--sweeps', type=int, default=6)
+    ap.add_argument('--plain', type=int, default=4, help='systems timed one per call on the plain path')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: every box here has one MI355X; the multi-GPU layout is '
+              'described in DESIGN.md (independent systems shard across GPUs without communication).',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from scikit_tt_amd import _lib
+    import scikit_tt_amd.solvers.evp as evp
+    assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    torch.cuda.set_device(0)
+    _lib.load()
+    print('[bench] building %d instances' % args.batch, file=sys.stderr, flush=True)
+    ops, x0s, exact, raw = build_batch(args.batch, args.sites, args.rank)
+    kw = dict(repeats=args.sweeps, conv_eps=0.0, solver='eig', sigma=-100.0)
+
+    stream = torch.cuda.current_stream()
+    for w in range(args.warmup):
+        evp.als_batch(ops, x0s, **kw)
+        print('[bench] warmup %d done' % w, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    lat, kernel_ms, flops, lams = [], [], 0.0, None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts = time.perf_counter()
+        e0.record(stream)
+        out, infos = evp.als_batch(ops, x0s, return_info=True, **kw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        lat.append(1e3 * (time.perf_counter() - ts))
+        kernel_ms.append(e0.elapsed_time(e1))
+        flops = sum(i.flops for i in infos)
+        lams = np.array([o[0] for o in out])
+        bad = [i.status for i in infos if i.status not in (0,)]
+        print('[bench] step %d: %.1f ms, status %s' % (k, lat[-1], 'ok' if not bad else bad), file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = args.batch * args.steps / elapsed
+
+    # plain drop-in path: one system per call
+    P = min(args.plain, args.batch)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    plain = [evp.als(ops[s], x0s[s], **kw)[0] for s in range(P)]
+    torch.cuda.synchronize()
+    plain_rate = P / (time.perf_counter() - tp)
+    err_vs_fp64 = float(np.max(np.abs(lams[:P] - np.array(plain))))
+    err_exact = lams - exact
+    print('[bench] plain path: %.2f solves/s' % plain_rate, file=sys.stderr, flush=True)
+
+    avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = flops / avg_kernel_s / 1e12
+    line = {
+        'metric': 'TT-ALS solves/s at rel.res <=1e-8 (eigenvalue error <=1e-10), batched vs plain; p50/p99 latency',
+        'value': round(value, 3),
+        'unit': 'solves/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(1e3 * elapsed / args.steps, 3),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'evp.als lowest eigenpair of %d TFIM chains (d=%d, n=2, MPO rank 3, TT rank %d, '
+                               '%d sweeps) in one launch' % (args.batch, args.sites, args.rank, args.sweeps),
+                   'batch': args.batch, 'sites': args.sites, 'rank': args.rank, 'sweeps': args.sweeps,
+                   'parallelism': 'one workgroup per system'},
+        'plain_fp64_solves_per_s': round(plain_rate, 3),
+        'speedup_batched_vs_plain': round(value / plain_rate, 2),
+        'eig_err_vs_fp64_plain_max': err_vs_fp64,
+        'eig_err_vs_closed_form_max': float(np.max(np.abs(err_exact))),
+        'latency_ms_p50': round(float(np.percentile(lat, 50)), 3),
+        'latency_ms_p99': round(float(np.percentile(lat, 99)), 3),
+        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None,
+                     'kernel': 'k_als<double>', 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
+    }
+    if not args.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(raw[0], args.sweeps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -63,6 +63,7 @@ typedef struct ttals_result {
   long long local_solves;  /* local problems solved */
   double lambda;           /* evp: eigenvalue returned */
   double worst_residual;   /* sle: worst final relative residual of a local Krylov solve */
+  double flops;            /* algorithmic flops of the solve (from shapes and iteration counts) */
 } ttals_result;
 
 /* ---- context ------------------------------------------------------------ */

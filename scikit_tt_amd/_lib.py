@@ -36,7 +36,8 @@ class Result(ctypes.Structure):
                 ('krylov_iters', ctypes.c_longlong),
                 ('local_solves', ctypes.c_longlong),
                 ('lambda_', ctypes.c_double),
-                ('worst_residual', ctypes.c_double)]
+                ('worst_residual', ctypes.c_double),
+                ('flops', ctypes.c_double)]
 
 
 # exported symbol -> (restype, argtypes)

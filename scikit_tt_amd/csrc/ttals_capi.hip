@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,11 +18,14 @@ using namespace ttals;
 // ---------------------------------------------------------------------------
 // kernels
 
+// One workgroup per system.  Dynamic LDS (`arena` doubles, at least the
+// GEMM staging tiles) holds either the staging tiles or the on-chip copy of
+// a local problem.
 template <class ACC>
-__global__ __launch_bounds__(NT) void k_als(const SysDesc* descs) {
-  __shared__ ACC lds[GEMM_LDS_DOUBLES];
+__global__ __launch_bounds__(NT) void k_als(const SysDesc* descs, long arena) {
+  extern __shared__ double dyn_lds[];
   __shared__ SolveLDS S;
-  als_solve<ACC>(descs[blockIdx.x], lds, S);
+  als_solve<ACC>(descs[blockIdx.x], reinterpret_cast<ACC*>(dyn_lds), arena, S);
 }
 
 struct PieceArgs {
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       micro_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.t1, lds);
       break;
     case P_SOLVE: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2};
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
       const int N = op.size();
       int its = 0, r = 0;
       int method = A.method;
@@ -81,8 +85,17 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         __syncthreads();
         r = wg_lu_solve(A.V, N, A.v, reinterpret_cast<int*>(A.V + (long)N * N), S.red) ? 0 : 1;
       } else {
-        if (method == 2) r = wg_cg<double>(op, A.f, A.v, A.V, A.tol, A.maxit, lds, S, &its);
-        if (method == 3 || r == 1) r = wg_gmres<double>(op, A.f, A.v, A.V, A.tol, A.maxit, KMAX, lds, S, &its);
+        Work W;
+        W.V = A.V;
+        W.f = const_cast<double*>(A.f);
+        W.x = A.v;
+        W.r = A.V;
+        W.p = A.V + N;
+        W.q = A.V + 2 * (long)N;
+        W.cur = A.V + (long)(KMAX + 1) * N;
+        W.w = A.V + (long)(KMAX + 2) * N;
+        if (method == 2) r = wg_cg<double>(op, W, A.tol, A.maxit, lds, S, &its);
+        if (method == 3 || r == 1) r = wg_gmres<double>(op, W, A.tol, A.maxit, KMAX, lds, S, &its);
       }
       if (threadIdx.x == 0) {
         A.res[0] = r;
@@ -91,10 +104,15 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       break;
     }
     case P_EIG: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2};
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
+      const int N = op.size();
+      Work W;
+      W.V = A.V;
+      W.cur = A.V + (long)(KMAX + 1) * N;
+      W.w = A.V + (long)(KMAX + 2) * N;
       int its = 0;
       double lam = 0.0;
-      const int r = wg_lanczos<double>(op, A.v, A.V, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      const int r = wg_lanczos<double>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
       if (threadIdx.x == 0) {
         A.res[0] = r;
         A.res[1] = its;
@@ -281,6 +299,7 @@ struct ttals_ctx {
   size_t hst_bytes = 0;
   double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
   long trace_cap = 0;
+  long arena = 6144;         // on-chip arena of k_als in doubles (48 KB)
   std::string err;
 };
 
@@ -346,6 +365,7 @@ struct SysPlan {
   std::vector<int> n, rx, ra, rb;
   long nmax = 0, tmax = 0, rmax = 0;
   size_t xsz_total = 0;
+  size_t nL = 0, nR = 0, nLb = 0, nRb = 0;
   // metadata (host staging offsets)
   size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_Loff, m_Roff, m_Lboff, m_Rboff, m_desc;
   // device workspace offsets (doubles region)
@@ -398,7 +418,11 @@ int plan_system(ttals_ctx* c, int kind, const ttals_tt& A, const ttals_tt& x0, c
   P.Roff.resize(d);
   P.Lboff.resize(d);
   P.Rboff.resize(d);
-  long lo = 0, bo = 0;
+  // Stacks are sized by the initial ranks (ranks never grow).  The left
+  // stack L_i lives on bond i: (r_i, R_i, r_i), Lb_i (p_i, r_i); the right
+  // stack R_i lives on bond i+1: (r_{i+1}, R_{i+1}, r_{i+1}), Rb_i
+  // (p_{i+1}, r_{i+1}).
+  long lo = 0, ro = 0, lbo = 0, rbo = 0;
   P.xsz_total = 0;
   for (int i = 0; i < d; ++i) {
     const long a = P.rx[i], bb = P.rx[i + 1], n = P.n[i];
@@ -407,15 +431,20 @@ int plan_system(ttals_ctx* c, int kind, const ttals_tt& A, const ttals_tt& x0, c
     const long mx = std::max(std::max((long)P.ra[i], (long)P.ra[i + 1]), std::max((long)P.rb[i], (long)P.rb[i + 1]));
     P.tmax = std::max(P.tmax, N * mx);
     P.rmax = std::max(P.rmax, std::max(a, bb));
-    P.Loff[i] = P.Roff[i] = lo;
+    P.Loff[i] = lo;
     lo += a * P.ra[i] * a;
-    P.Lboff[i] = P.Rboff[i] = bo;
-    bo += (long)P.rb[i] * a;
+    P.Roff[i] = ro;
+    ro += bb * P.ra[i + 1] * bb;
+    P.Lboff[i] = lbo;
+    lbo += (long)P.rb[i] * a;
+    P.Rboff[i] = rbo;
+    rbo += (long)P.rb[i + 1] * bb;
     P.xsz_total += (size_t)N;
   }
-  // stacks sized by the initial ranks (ranks never grow)
-  P.w_L = (size_t)lo;   // temporarily: element counts
-  P.w_Lb = (size_t)bo;
+  P.nL = (size_t)lo;  // element counts
+  P.nR = (size_t)ro;
+  P.nLb = (size_t)lbo;
+  P.nRb = (size_t)rbo;
   (void)kind;
   return 0;
 }
@@ -457,11 +486,10 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
     P.m_Roff = hl.take(sizeof(long) * d);
     P.m_Lboff = hl.take(sizeof(long) * d);
     P.m_Rboff = hl.take(sizeof(long) * d);
-    const size_t nst = P.w_L, nbst = P.w_Lb;
-    P.w_L = dl.take(8 * std::max<size_t>(nst, 1));
-    P.w_R = dl.take(8 * std::max<size_t>(nst, 1));
-    P.w_Lb = dl.take(8 * std::max<size_t>(nbst, 1));
-    P.w_Rb = dl.take(8 * std::max<size_t>(nbst, 1));
+    P.w_L = dl.take(8 * std::max<size_t>(P.nL, 1));
+    P.w_R = dl.take(8 * std::max<size_t>(P.nR, 1));
+    P.w_Lb = dl.take(8 * std::max<size_t>(P.nLb, 1));
+    P.w_Rb = dl.take(8 * std::max<size_t>(P.nRb, 1));
     P.w_f = dl.take(8 * P.nmax);
     P.w_g = dl.take(8 * P.nmax);
     P.w_t1 = dl.take(8 * P.tmax);
@@ -473,7 +501,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
     P.w_tau = dl.take(8 * P.rmax);
     P.w_rf = dl.take(8 * P.rmax * P.rmax);
     P.w_best = dl.take(kind == 1 ? 8 * P.xsz_total : 8);
-    P.w_out = dl.take(8 * 8);
+    P.w_out = dl.take(8 * 16);
   }
   const size_t meta_bytes = hl.bytes;
   // results region (ranks + scalars) in the staging buffer after the metadata
@@ -482,7 +510,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
   std::vector<size_t> r_rx(count), r_out(count);
   for (int s = 0; s < count; ++s) {
     r_rx[s] = rl.take(sizeof(int) * (plans[s].d + 1));
-    r_out[s] = rl.take(8 * 8);
+    r_out[s] = rl.take(8 * 16);
   }
   int rc;
   if ((rc = ensure_hst(c, rl.bytes))) return rc;
@@ -567,12 +595,14 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
     }
   }
   HIPCHK(c, hipMemcpyAsync(dmeta, c->hst, meta_bytes, hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_als<double>, dim3(count), dim3(NT), 0, stream, (const SysDesc*)dptr(m_descs));
+  const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
+  hipLaunchKernelGGL(k_als<double>, dim3(count), dim3(NT), (size_t)arena * 8, stream,
+                     (const SysDesc*)dptr(m_descs), arena);
   HIPCHK(c, hipGetLastError());
   for (int s = 0; s < count; ++s) {
     HIPCHK(c, hipMemcpyAsync(c->hst + r_rx[s], dptr(plans[s].m_rx), sizeof(int) * (plans[s].d + 1),
                              hipMemcpyDeviceToHost, stream));
-    HIPCHK(c, hipMemcpyAsync(c->hst + r_out[s], wptr(plans[s].w_out), 8 * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(c, hipMemcpyAsync(c->hst + r_out[s], wptr(plans[s].w_out), 8 * 16, hipMemcpyDeviceToHost, stream));
   }
   HIPCHK(c, hipStreamSynchronize(stream));
   int worst = 0;
@@ -589,6 +619,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
     R.local_solves = (long long)o[5];
     R.max_iters = (int)o[6];
     R.worst_residual = o[7];
+    R.flops = o[8];
     if (R.status == ST_OK && R.unconverged > 0) R.status = ST_NOCONV;
     if (R.status != ST_OK && (worst == 0 || (R.status != ST_NOCONV && worst == ST_NOCONV))) worst = R.status;
   }
@@ -597,7 +628,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
 
 // scratch for the piece entry points: 2*t + (KMAX+2)*N doubles
 int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double** V, double** res) {
-  const long nv = std::max<long>((KMAX + 2) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
+  const long nv = std::max<long>((KMAX + 4) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
   const size_t bytes = 8 * (size_t)(2 * t + nv + 16) + 4096;
   int rc = ensure_dws(c, bytes);
   if (rc) return rc;
@@ -636,6 +667,10 @@ int ttals_ctx_create(int device, ttals_ctx** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TTALS_E_ARG;
   ttals_ctx* c = new ttals_ctx();
   c->device = device;
+  if (const char* e = std::getenv("TTALS_ARENA_KB")) {
+    const long kb = std::atol(e);
+    if (kb >= 0 && kb <= 64) c->arena = kb * 1024 / 8;
+  }
   *out = c;
   return 0;
 }

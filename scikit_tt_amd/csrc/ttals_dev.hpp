@@ -195,6 +195,76 @@ __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C,
 }
 
 // ---------------------------------------------------------------------------
+// Direct GEMM for operands that already sit on-chip (LDS): no staging, each
+// thread owns 2x2 output micro-tiles, and the K index is walked with split
+// counters (no integer division in the inner loop).
+
+struct IWalk {
+  long hi, lo, off;
+  int div, l;
+  __device__ __forceinline__ void start(const IMap& m) {
+    div = m.div;
+    hi = m.hi;
+    lo = m.lo;
+    l = 0;
+    off = 0;
+  }
+  __device__ __forceinline__ void next() {
+    if (++l == div) {
+      l = 0;
+      off += hi - (long)(div - 1) * lo;
+    } else {
+      off += lo;
+    }
+  }
+};
+
+template <class T, class ACC>
+__device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C) {
+  const int tN = (N + 1) >> 1;
+  const int tiles = ((M + 1) >> 1) * tN;
+  for (int t = threadIdx.x; t < tiles; t += NT) {
+    const int ti = t / tN, tj = t - ti * tN;
+    const int i0 = 2 * ti, j0 = 2 * tj;
+    const bool i1 = i0 + 1 < M, j1 = j0 + 1 < N;
+    const long ar0 = imap(A.r, i0), ar1 = i1 ? imap(A.r, i0 + 1) : ar0;
+    const long bc0 = imap(B.c, j0), bc1 = j1 ? imap(B.c, j0 + 1) : bc0;
+    IWalk ka, kb;
+    ka.start(A.c);
+    kb.start(B.r);
+    ACC c00 = 0, c01 = 0, c10 = 0, c11 = 0;
+    for (int k = 0; k < K; ++k) {
+      const ACC a0 = (ACC)A.p[ar0 + ka.off], a1 = (ACC)A.p[ar1 + ka.off];
+      const ACC b0 = (ACC)B.p[kb.off + bc0], b1 = (ACC)B.p[kb.off + bc1];
+      c00 += a0 * b0;
+      c01 += a0 * b1;
+      c10 += a1 * b0;
+      c11 += a1 * b1;
+      ka.next();
+      kb.next();
+    }
+    C.at(i0, j0) = (T)c00;
+    if (j1) C.at(i0, j0 + 1) = (T)c01;
+    if (i1) {
+      C.at(i0 + 1, j0) = (T)c10;
+      if (j1) C.at(i0 + 1, j0 + 1) = (T)c11;
+    }
+  }
+  __syncthreads();
+}
+
+// unscaled 2-norm (for vectors of moderate magnitude in the hot loops)
+template <class T>
+__device__ __forceinline__ double wg_nrm2u(const T* x, int n, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const double t = (double)x[i];
+    s += t * t;
+  }
+  return sqrt(wg_sum(s, red));
+}
+
+// ---------------------------------------------------------------------------
 // Interface-stack contractions and the local matvec.
 //
 // Shapes: x core (a, n, b); operator core A (P, m, n, Q) with m == n;
@@ -222,6 +292,18 @@ __device__ void local_matvec(int a, int P, int n, int Q, int b, const T* L, cons
                   mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)), lds);
   // y[(a',m), b'] = T2[(a',m),(b,Q)] R[(b,Q), b']
   wg_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b), lds);
+}
+
+// Same contraction with every operand on-chip (ds_gemm stages).
+template <class T, class ACC>
+__device__ void local_matvec_ds(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R,
+                                const T* v, T* y, T* t1, T* t2) {
+  const int a2 = a;
+  ds_gemm<T, ACC>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
+                  mview<T>(t1, split(a2, n, b * P * n), split(b, 1, P * n)));
+  ds_gemm<T, ACC>(a * b, n * Q, P * n, crm<T>(t1, P * n), cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+                  mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)));
+  ds_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b));
 }
 
 // Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']

@@ -1,12 +1,18 @@
 // Local solvers of the TT-ALS engine: matrix-free CG / GMRES(m) for the
 // local linear system and restarted Lanczos for the extreme local
-// eigenpair.  One workgroup per local problem; vectors in global memory.
+// eigenpair.  One workgroup per local problem.
+//
+// Two memory modes share the same solver code: when the local problem fits
+// on-chip, the stacks, the operator core, the contraction intermediates and
+// the Krylov working vectors live in LDS (op.ds = true, contractions by
+// ds_gemm); otherwise everything is in global memory (L2-resident) and the
+// contractions stage tiles through LDS (wg_gemm).  The Krylov basis is
+// always in global memory.
 #pragma once
 #include "ttals_dev.hpp"
 
 namespace ttals {
 
-// The local operator M = (L x A x R) restricted to one core position.
 template <class T>
 struct LocalOp {
   int a, P, n, Q, b;
@@ -15,16 +21,19 @@ struct LocalOp {
   const T* R;
   T* t1;
   T* t2;
+  bool ds;  // operands on-chip
   __device__ int size() const { return a * n * b; }
 };
 
 template <class T, class ACC>
 __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y, ACC* lds) {
-  local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);
+  if (op.ds)
+    local_matvec_ds<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2);
+  else
+    local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);
 }
 
-// LDS scratch used by the solvers (besides the GEMM tiles).
-constexpr int KMAX = 48;  // Krylov basis size (GMRES restart / Lanczos)
+constexpr int KMAX = 40;  // Krylov basis size (GMRES restart / Lanczos)
 struct SolveLDS {
   double red[NWAVE];
   double h[KMAX + 2];
@@ -33,6 +42,19 @@ struct SolveLDS {
   double alpha[KMAX + 1], beta[KMAX + 1], s[KMAX + 1];
   double scal[4];
   int flag[4];
+};
+
+// Working vectors of a local solve.  V (global): (KMAX+1) x N basis;
+// cur, w (N each) and, for CG, r, p, q, x, f: on-chip when they fit.
+struct Work {
+  double* V;
+  double* cur;
+  double* w;
+  double* r;
+  double* p;
+  double* q;
+  double* x;
+  double* f;
 };
 
 // h[i] = <V_i, w> for i < cnt, one wave per dot product.
@@ -59,22 +81,21 @@ __device__ __forceinline__ void wg_multiaxpy(const double* V, long ldv, int cnt,
 }
 
 // Iterated classical Gram-Schmidt with the DGKS criterion: another pass is
-// made while a pass shrinks w by more than a factor 0.7 (at most 4 passes),
-// so that even a near-breakdown residual made of rounding noise leaves
+// made while a pass shrinks w by more than a factor 0.7 (2..4 passes), so
+// that even a near-breakdown residual made of rounding noise leaves
 // orthogonal to the basis.  Accumulates the coefficients in hsum; returns
 // ||w|| after orthogonalisation and the norm before it in *w0.
 __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, double* h, double* hsum,
                           double* red, double* w0) {
   for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] = 0.0;
-  double prev = wg_nrm2(w, N, red);
+  double prev = wg_nrm2u(w, N, red);
   *w0 = prev;
   double nrm = prev;
   for (int pass = 0; pass < 4; ++pass) {
     wg_multidot(V, ldv, cnt, w, N, h);
     wg_multiaxpy(V, ldv, cnt, h, w, N);
     for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] += h[i];
-    __syncthreads();
-    nrm = wg_nrm2(w, N, red);
+    nrm = wg_nrm2u(w, N, red);  // barrier inside: w and hsum complete
     if (pass >= 1 && nrm > 0.7 * prev) break;
     prev = nrm;
   }
@@ -82,16 +103,15 @@ __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, 
 }
 
 // ---------------------------------------------------------------------------
-// Conjugate gradients.  x holds the initial guess on entry.  Returns 0 on
+// Conjugate gradients on W.x (initial guess in W.x).  Returns 0 on
 // convergence, 1 if p^T M p <= 0 was met (not SPD: caller switches to
 // GMRES), 2 if maxit was reached.
 template <class ACC>
-__device__ int wg_cg(const LocalOp<double>& op, const double* f, double* x, double* work, double tol, int maxit,
-                     ACC* lds, SolveLDS& S, int* iters) {
+__device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int maxit, ACC* lds, SolveLDS& S,
+                     int* iters) {
   const int N = op.size();
-  double* r = work;
-  double* p = work + N;
-  double* q = work + 2 * N;
+  double *x = W.x, *r = W.r, *p = W.p, *q = W.q;
+  const double* f = W.f;
   const double fn = wg_nrm2(f, N, S.red);
   if (fn == 0.0) {
     wg_fill(x, 0.0, N);
@@ -128,11 +148,13 @@ __device__ int wg_cg(const LocalOp<double>& op, const double* f, double* x, doub
 }
 
 // ---------------------------------------------------------------------------
-// Restarted GMRES(m) with CGS2 orthogonalisation.  V: (m+1) x N scratch.
+// Restarted GMRES(m) with DGKS-reorthogonalised Arnoldi; solution in W.x.
 template <class ACC>
-__device__ int wg_gmres(const LocalOp<double>& op, const double* f, double* x, double* V, double tol, int maxit,
-                        int m, ACC* lds, SolveLDS& S, int* iters) {
+__device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, int maxit, int m, ACC* lds,
+                        SolveLDS& S, int* iters) {
   const int N = op.size();
+  double *V = W.V, *x = W.x, *cur = W.cur, *w = W.w;
+  const double* f = W.f;
   if (m > KMAX) m = KMAX;
   if (m > N) m = N;
   const double fn = wg_nrm2(f, N, S.red);
@@ -142,15 +164,18 @@ __device__ int wg_gmres(const LocalOp<double>& op, const double* f, double* x, d
     return 0;
   }
   for (;;) {
-    double* v0 = V;
-    apply_op<double, ACC>(op, x, v0, lds);
-    for (int e = threadIdx.x; e < N; e += NT) v0[e] = f[e] - v0[e];
+    apply_op<double, ACC>(op, x, w, lds);
+    for (int e = threadIdx.x; e < N; e += NT) w[e] = f[e] - w[e];
     __syncthreads();
-    const double beta = wg_nrm2(v0, N, S.red);
+    const double beta = wg_nrm2(w, N, S.red);
     if (beta <= tol * fn) return 0;
     if (*iters >= maxit) return 2;
     const double ib = 1.0 / beta;
-    for (int e = threadIdx.x; e < N; e += NT) v0[e] *= ib;
+    for (int e = threadIdx.x; e < N; e += NT) {
+      const double t = w[e] * ib;
+      cur[e] = t;
+      V[e] = t;
+    }
     if (threadIdx.x == 0) {
       S.g[0] = beta;
       for (int i = 1; i <= m; ++i) S.g[i] = 0.0;
@@ -159,14 +184,17 @@ __device__ int wg_gmres(const LocalOp<double>& op, const double* f, double* x, d
     int jend = 0;
     double res = beta;
     for (int j = 0; j < m; ++j) {
-      double* w = V + (long)(j + 1) * N;
-      apply_op<double, ACC>(op, V + (long)j * N, w, lds);
+      apply_op<double, ACC>(op, cur, w, lds);
       double w0;
       double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.red, &w0);
       if (!(hn > 1e-14 * w0)) hn = 0.0;  // (lucky) breakdown: Krylov space invariant
       if (hn > 0.0) {
         const double ih = 1.0 / hn;
-        for (int e = threadIdx.x; e < N; e += NT) w[e] *= ih;
+        for (int e = threadIdx.x; e < N; e += NT) {
+          const double t = w[e] * ih;
+          cur[e] = t;
+          V[(long)(j + 1) * N + e] = t;
+        }
       }
       if (threadIdx.x == 0) {
         double* col = S.H + j * (KMAX + 1);  // column j, rows 0..j+1
@@ -211,21 +239,19 @@ __device__ int wg_gmres(const LocalOp<double>& op, const double* f, double* x, d
     __syncthreads();
     if (res <= tol * fn) {
       // confirm with the true residual once
-      double* rv = V;
-      apply_op<double, ACC>(op, x, rv, lds);
-      for (int e = threadIdx.x; e < N; e += NT) rv[e] = f[e] - rv[e];
+      apply_op<double, ACC>(op, x, w, lds);
+      for (int e = threadIdx.x; e < N; e += NT) w[e] = f[e] - w[e];
       __syncthreads();
-      if (wg_nrm2(rv, N, S.red) <= 10.0 * tol * fn) return 0;
+      if (wg_nrm2(w, N, S.red) <= 10.0 * tol * fn) return 0;
     }
     if (*iters >= maxit) return 2;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Extreme eigenpair of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
+// Eigenvalues of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
 // Sturm-count multisection by one wave (64 shifts per round), then inverse
-// iteration for the eigenvector.  `which`: 0 = lowest, 1 = highest, or an
-// explicit eigenvalue index via which_idx >= 0.
+// iteration for the eigenvector.
 
 __device__ __forceinline__ int sturm_count(const double* al, const double* be, int m, double x) {
   int c = 0;
@@ -258,8 +284,7 @@ __device__ void tri_eigval(const double* al, const double* be, int m, int idx, S
       const double h = (hi - lo) / 65.0;
       const double xpt = lo + h * (lane + 1);
       const int c = sturm_count(al, be, m, xpt);
-      // largest lane with count <= idx
-      unsigned long long ball = __ballot(c <= idx);
+      const unsigned long long ball = __ballot(c <= idx);
       const int nle = __popcll(ball);  // counts are monotone in lane
       const double nlo = nle > 0 ? lo + h * nle : lo;
       const double nhi = nle < 64 ? lo + h * (nle + 1) : hi;
@@ -273,15 +298,14 @@ __device__ void tri_eigval(const double* al, const double* be, int m, int idx, S
 }
 
 // eigenvector of T for eigenvalue th by inverse iteration (thread 0).
-__device__ void tri_eigvec(const double* al, const double* be, int m, double th, double* s, double* tmpd,
-                           double* tmpu) {
+__device__ void tri_eigvec(const double* al, const double* be, int m, double th, double* s, double* tmpd) {
   if (threadIdx.x == 0) {
     double anorm = 0.0;
     for (int i = 0; i < m; ++i) anorm = fmax(anorm, fabs(al[i]) + (i < m - 1 ? fabs(be[i]) : 0.0));
     const double pert = 1e-15 * fmax(anorm, 1e-300);
     for (int i = 0; i < m; ++i) s[i] = 1.0 / sqrt((double)m) * (1.0 + 0.01 * (double)((i * 7) % 5));
     for (int it = 0; it < 3; ++it) {
-      // solve (T - th I) z = s with partial pivoting-free LDL (perturbed pivots)
+      // solve (T - th I) z = s with an LDL^T sweep (pivots nudged away from 0)
       double dprev = 0.0, uprev = 0.0;
       for (int i = 0; i < m; ++i) {
         double di = al[i] - th;
@@ -305,22 +329,22 @@ __device__ void tri_eigvec(const double* al, const double* be, int m, double th,
       nn = 1.0 / sqrt(nn);
       for (int i = 0; i < m; ++i) s[i] *= nn;
     }
-    (void)tmpu;
   }
   __syncthreads();
 }
 
-// Restarted Lanczos with full re-orthogonalisation (CGS2) for one extreme
+// Restarted Lanczos with full (DGKS) re-orthogonalisation for one extreme
 // eigenpair of the symmetric local operator.
 //   mode 0: lowest, 1: highest, 2: nearest sigma (lowest/highest chosen
 //   from the first Ritz spectrum; the choice is checked at the end).
-// v: start vector on entry, unit eigenvector on exit.  V: (KMAX+1) x N.
+// v (global): start vector on entry, unit eigenvector on exit.
 // Returns 0 converged, 2 not converged, 3 sigma lies inside the spectrum of
-// a problem too large for the Krylov space (unsupported).
+// a problem larger than the Krylov space (unsupported).
 template <class ACC>
-__device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int mode, double sigma, double tol,
+__device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, int mode, double sigma, double tol,
                           int maxit, int kmax, ACC* lds, SolveLDS& S, double* lambda, int* iters) {
   const int N = op.size();
+  double *V = W.V, *cur = W.cur, *w = W.w;
   if (kmax > KMAX) kmax = KMAX;
   double vn = wg_nrm2(v, N, S.red);
   if (!(vn > 0.0) || !isfinite(vn)) {
@@ -334,15 +358,18 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int m
   double th = 0.0;
   for (int restart = 0; restart < 1000; ++restart) {
     const double iv = 1.0 / vn;
-    for (int e = threadIdx.x; e < N; e += NT) V[e] = v[e] * iv;
+    for (int e = threadIdx.x; e < N; e += NT) {
+      const double t = v[e] * iv;
+      V[e] = t;
+      cur[e] = t;
+    }
     __syncthreads();
     int m = 0;
     bool conv = false;
     bool full = false;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
-      double* w = V + (long)(j + 1) * N;
-      apply_op<double, ACC>(op, V + (long)j * N, w, lds);
+      apply_op<double, ACC>(op, cur, w, lds);
       double w0;
       double bn = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.red, &w0);
       if (threadIdx.x == 0) {
@@ -366,16 +393,18 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int m
         __syncthreads();
         double r0;
         bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.red, &r0);
-        if (bn > 0.0) {
-          const double ib = 1.0 / bn;
-          for (int e = threadIdx.x; e < N; e += NT) w[e] *= ib;
+        const double ib = bn > 0.0 ? 1.0 / bn : 0.0;
+        for (int e = threadIdx.x; e < N; e += NT) {
+          const double t = w[e] * ib;
+          cur[e] = t;
+          V[(long)(j + 1) * N + e] = t;
         }
         __syncthreads();
         continue;
       }
       if (threadIdx.x == 0) S.beta[j] = brk ? 0.0 : bn;
       __syncthreads();
-      const bool check = full || brk || (m == kk) || (m >= 4 && (m % 4) == 0) || *iters >= maxit;
+      const bool check = full || brk || (m == kk) || (m >= 2 && (m % 4) == 2) || *iters >= maxit;
       if (check) {
         if (!decided) {
           tri_eigval(S.alpha, S.beta, m, 0, S);
@@ -409,13 +438,17 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int m
         }
         tri_eigval(S.alpha, S.beta, m, idx, S);
         th = S.scal[1];
-        tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs, S.sn);
+        tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
         const double res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
         if (res <= tol * fmax(fabs(th), 1e-3 * anorm)) { conv = true; break; }
         if (*iters >= maxit) break;
       }
       const double ib = 1.0 / bn;
-      for (int e = threadIdx.x; e < N; e += NT) w[e] *= ib;
+      for (int e = threadIdx.x; e < N; e += NT) {
+        const double t = w[e] * ib;
+        cur[e] = t;
+        V[(long)(j + 1) * N + e] = t;
+      }
       __syncthreads();
     }
     // Ritz vector
@@ -426,23 +459,16 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, double* V, int m
     }
     __syncthreads();
     vn = wg_nrm2(v, N, S.red);
-    if (conv) {
+    if (conv || *iters >= maxit) {
       const double iv2 = 1.0 / vn;
       for (int e = threadIdx.x; e < N; e += NT) v[e] *= iv2;
       __syncthreads();
-      status = 0;
-      if (mode == 2 && !full) {
+      status = conv ? 0 : 2;
+      if (conv && mode == 2 && !full) {
         // nearest-sigma with an extreme eigenvalue is only valid if sigma lies
         // outside the spectrum on that side
         if ((target == 0 && sigma > th) || (target == 1 && sigma < th)) status = 3;
       }
-      break;
-    }
-    if (*iters >= maxit) {
-      const double iv2 = 1.0 / vn;
-      for (int e = threadIdx.x; e < N; e += NT) v[e] *= iv2;
-      __syncthreads();
-      status = 2;
       break;
     }
   }

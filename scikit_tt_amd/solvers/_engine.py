@@ -12,7 +12,7 @@ class SolveInfo:
     """Per-system engine report: status, sweeps, Krylov iterations, eigenvalue."""
 
     __slots__ = ('status', 'sweeps', 'unconverged', 'max_iters', 'krylov_iters', 'local_solves', 'eigenvalue',
-                 'worst_residual')
+                 'worst_residual', 'flops')
 
     def __init__(self, r):
         self.status = int(r.status)
@@ -23,6 +23,7 @@ class SolveInfo:
         self.max_iters = int(r.max_iters)
         self.eigenvalue = float(r.lambda_)
         self.worst_residual = float(r.worst_residual)
+        self.flops = float(r.flops)
 
     def __repr__(self):
         return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in self.__slots__)

@@ -1,0 +1,347 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+Tolerances: pieces (stack contractions, matvecs, micro matrices) 1e-12
+relative; local Krylov solves to their stated tolerance; whole-solver
+iterates 1e-7 relative (the reference tests' own bar, tests/test_sle.py:17)
+and eigenvalues 1e-10 relative.  TT ranks must be identical.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import tt_ops, sle_als, evp_als
+from scikit_tt_amd import workloads as W
+from tests.conftest import load_golden
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def env():
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    from scikit_tt_amd import _lib
+    torch.cuda.set_device(0)
+    lib = _lib.load()
+    return lib, _lib.context(0), _lib
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def empty(*shape):
+    return torch.empty(shape, dtype=torch.float64, device='cuda')
+
+
+SHAPES = [(1, 1, 2, 3, 2), (3, 2, 4, 3, 5), (5, 2, 8, 2, 5), (20, 3, 8, 2, 17), (16, 3, 2, 3, 16), (2, 4, 16, 4, 48)]
+
+
+@pytest.mark.parametrize('a,P,n,Q,b', SHAPES)
+def test_stacks_matvec_micro(env, a, P, n, Q, b):
+    lib, ctx, _ = env
+    rng = np.random.default_rng(a * 100 + b)
+    L = rng.standard_normal((a, P, a))
+    R = rng.standard_normal((b, Q, b))
+    Ac = rng.standard_normal((P, n, n, Q))
+    x = rng.standard_normal((a, n, 1, b))
+    dL, dR, dA, dx = dev(L), dev(R), dev(Ac), dev(x)
+    out = empty(b, Q, b)
+    assert lib.ttals_stack_left_op(ctx, None, a, P, n, Q, b, ptr(dL), ptr(dA), ptr(dx), ptr(out)) == 0
+    ref = sle_als.stack_left_op(L, x, Ac)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    out = empty(a, P, a)
+    assert lib.ttals_stack_right_op(ctx, None, a, P, n, Q, b, ptr(dR), ptr(dA), ptr(dx), ptr(out)) == 0
+    ref = sle_als.stack_right_op(R, x, Ac)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # rhs stacks with p = P, q = Q ranks
+    Lb = rng.standard_normal((P, a))
+    Rb = rng.standard_normal((Q, b))
+    Bc = rng.standard_normal((P, n, 1, Q))
+    dLb, dRb, dB = dev(Lb), dev(Rb), dev(Bc)
+    out = empty(Q, b)
+    assert lib.ttals_stack_left_rhs(ctx, None, a, P, n, Q, b, ptr(dLb), ptr(dB), ptr(dx), ptr(out)) == 0
+    np.testing.assert_allclose(out.cpu().numpy(), sle_als.stack_left_rhs(Lb, x, Bc), rtol=1e-12, atol=1e-11)
+    out = empty(P, a)
+    assert lib.ttals_stack_right_rhs(ctx, None, a, P, n, Q, b, ptr(dRb), ptr(dB), ptr(dx), ptr(out)) == 0
+    np.testing.assert_allclose(out.cpu().numpy(), sle_als.stack_right_rhs(Rb, x, Bc), rtol=1e-12, atol=1e-11)
+    out = empty(a * n * b)
+    assert lib.ttals_micro_rhs(ctx, None, a, P, n, Q, b, ptr(dLb), ptr(dB), ptr(dRb), ptr(out)) == 0
+    np.testing.assert_allclose(out.cpu().numpy(), sle_als.micro_rhs(Lb, Bc, Rb).ravel(), rtol=1e-12, atol=1e-11)
+    # matvec and dense micro matrix
+    N = a * n * b
+    v = rng.standard_normal(N)
+    y = empty(N)
+    assert lib.ttals_micro_matvec(ctx, None, a, P, n, Q, b, ptr(dL), ptr(dA), ptr(dR), ptr(dev(v)), ptr(y)) == 0
+    ref = sle_als.micro_matvec(L, Ac, R, v)
+    np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    if N <= 600:
+        M = empty(N, N)
+        assert lib.ttals_micro_matrix(ctx, None, a, P, n, Q, b, ptr(dL), ptr(dA), ptr(dR), ptr(M)) == 0
+        Mr = sle_als.micro_matrix(L, Ac, R)
+        np.testing.assert_allclose(M.cpu().numpy(), Mr, rtol=1e-12, atol=1e-12 * np.abs(Mr).max())
+
+
+def spd_local(rng, a, P, n, Q, b, shift=5.0):
+    """Random SPD local operator: symmetric slices, PSD stacks, identity shift in slice 0."""
+    Ac = rng.standard_normal((P, n, n, Q))
+    Ac = Ac + Ac.transpose(0, 2, 1, 3)
+    Ac[0, :, :, 0] += shift * n * np.eye(n) * 4
+    def psd(r, k):
+        S = rng.standard_normal((r, k, r)) * 0.1
+        S = S + S.transpose(2, 1, 0)
+        S[:, 0, :] += np.eye(r) * 3
+        return S
+    return psd(a, P), Ac, psd(b, Q)
+
+
+@pytest.mark.parametrize('a,P,n,Q,b,method', [(2, 2, 3, 2, 4, 1), (4, 2, 6, 2, 4, 1), (5, 2, 8, 2, 5, 2),
+                                              (8, 3, 4, 3, 8, 2), (5, 2, 8, 2, 5, 3), (8, 3, 4, 3, 9, 3)])
+def test_local_solve(env, a, P, n, Q, b, method):
+    lib, ctx, _ = env
+    rng = np.random.default_rng(7 + method)
+    L, Ac, R = spd_local(rng, a, P, n, Q, b)
+    if method == 3:  # general (non-symmetric) operator for GMRES
+        Ac = Ac + 0.3 * rng.standard_normal(Ac.shape)
+    M = sle_als.micro_matrix(L, Ac, R)
+    N = M.shape[0]
+    f = rng.standard_normal(N)
+    v = torch.zeros(N, dtype=torch.float64, device='cuda')
+    its = ctypes.c_int()
+    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(dev(f)), ptr(v),
+                               method, 1 if method == 2 else 0, 1e-12, 4000, ctypes.byref(its))
+    assert rc == 0
+    x = v.cpu().numpy()
+    assert np.linalg.norm(M @ x - f) <= 1e-10 * np.linalg.norm(f)
+    np.testing.assert_allclose(x, np.linalg.solve(M, f), rtol=1e-8, atol=1e-9 * np.abs(x).max())
+
+
+def test_local_lu_singular_detected(env):
+    lib, ctx, _lib = env
+    a, P, n, Q, b = 2, 1, 3, 1, 2
+    Ac = np.zeros((P, n, n, Q))
+    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, ptr(dev(np.ones((a, P, a)))), ptr(dev(Ac)),
+                               ptr(dev(np.ones((b, Q, b)))), ptr(dev(np.ones(a * n * b))),
+                               ptr(torch.zeros(a * n * b, dtype=torch.float64, device='cuda')), 1, 0, 1e-12, 10, None)
+    assert rc == _lib.S_SINGULAR
+
+
+@pytest.mark.parametrize('N_shape,mode', [((2, 2, 3, 2, 4), 0), ((4, 2, 6, 2, 4), 1), ((8, 3, 4, 3, 8), 0),
+                                          ((8, 3, 4, 3, 8), 2), ((2, 3, 2, 3, 4), 2)])
+def test_local_eig(env, N_shape, mode):
+    lib, ctx, _ = env
+    a, P, n, Q, b = N_shape
+    rng = np.random.default_rng(11)
+    L, Ac, R = spd_local(rng, a, P, n, Q, b, shift=0.0)
+    M = sle_als.micro_matrix(L, Ac, R)
+    w = np.linalg.eigvalsh(M)
+    sigma = {0: 0.0, 1: 0.0, 2: w[0] - 100.0}[mode]
+    v = dev(rng.standard_normal(M.shape[0]))
+    lam, its = ctypes.c_double(), ctypes.c_int()
+    rc = lib.ttals_local_eig(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(v), mode, sigma,
+                             1e-12, 4000, ctypes.byref(lam), ctypes.byref(its))
+    assert rc == 0
+    target = w[-1] if mode == 1 else w[0]
+    assert abs(lam.value - target) <= 1e-10 * np.abs(w).max()
+    x = v.cpu().numpy()
+    assert abs(np.linalg.norm(x) - 1) < 1e-12
+    assert np.linalg.norm(M @ x - lam.value * x) <= 1e-8 * np.abs(w).max()
+
+
+def test_local_eig_nearest_interior_small(env):
+    """Local problem inside the Krylov space: the eigenvalue nearest an interior sigma (evp.py:429)."""
+    lib, ctx, _ = env
+    a, P, n, Q, b = 2, 2, 3, 2, 3
+    rng = np.random.default_rng(12)
+    L, Ac, R = spd_local(rng, a, P, n, Q, b, shift=0.0)
+    M = sle_als.micro_matrix(L, Ac, R)
+    w = np.linalg.eigvalsh(M)
+    sigma = 0.5 * (w[7] + w[8]) + 0.1 * (w[8] - w[7])
+    v = dev(rng.standard_normal(M.shape[0]))
+    lam = ctypes.c_double()
+    assert lib.ttals_local_eig(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(v), 2, sigma,
+                               1e-12, 4000, ctypes.byref(lam), None) == 0
+    assert abs(lam.value - w[np.argmin(np.abs(w - sigma))]) < 1e-9 * np.abs(w).max()
+
+
+@pytest.mark.parametrize('m,c', [(12, 5), (5, 12), (200, 48), (7, 7), (1, 9), (9, 1)])
+def test_qr(env, m, c):
+    lib, ctx, _ = env
+    Y = np.random.default_rng(m * c).standard_normal((m, c))
+    k = min(m, c)
+    q, r = empty(m, k), empty(k, c)
+    assert lib.ttals_qr(ctx, None, m, c, ptr(dev(Y)), ptr(q), ptr(r)) == 0
+    Q, Rr = q.cpu().numpy(), r.cpu().numpy()
+    np.testing.assert_allclose(Q @ Rr, Y, atol=1e-13 * np.abs(Y).max() * max(m, c))
+    np.testing.assert_allclose(Q.T @ Q, np.eye(k), atol=1e-13 * max(m, c))
+    assert np.allclose(np.tril(Rr, -1), 0.0)
+    # same factorisation as LAPACK up to the signs of R's rows
+    Qr, Rl = sla.qr(Y, mode='economic')
+    np.testing.assert_allclose(np.abs(Rr), np.abs(Rl), rtol=1e-10, atol=1e-12 * np.abs(Y).max())
+
+
+# --------------------------------------------------------------------------
+# TT algebra through the TT class
+
+
+def test_tt_algebra(env):
+    from scikit_tt_amd.tensor_train import TT
+    import scikit_tt_amd.tensor_train as tt
+    rng = np.random.default_rng(5)
+    A = W.random_tt(rng, [2, 3, 2, 3], [1, 2, 3, 2, 1], col_dims=[3, 2, 2, 3])
+    x = W.random_tt(rng, [3, 2, 2, 3], [1, 3, 4, 2, 1])
+    y = W.random_tt(rng, [3, 2, 2, 3], [1, 2, 2, 3, 1])
+    tA, tx, ty = TT(A), TT(x), TT(y)
+    np.testing.assert_allclose(tA.dot(tx).matricize(), tt_ops.matricize(tt_ops.dot(A, x)), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose((tA @ tx).matricize(), tt_ops.matricize(A) @ tt_ops.matricize(x), rtol=1e-11,
+                               atol=1e-11)
+    np.testing.assert_allclose((tx + ty).matricize(), tt_ops.matricize(tt_ops.add(x, y)), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose((tx - ty).matricize(), tt_ops.matricize(tt_ops.sub(x, y)), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose((2.5 * tx).matricize(), 2.5 * tt_ops.matricize(x), rtol=1e-13)
+    np.testing.assert_allclose((tx * -1.5).matricize(), -1.5 * tt_ops.matricize(x), rtol=1e-13)
+    for c, cr in zip((tx + ty).cores, tt_ops.add(x, y)):
+        np.testing.assert_array_equal(c, cr)  # the block layout itself
+    assert abs(tx.norm() - tt_ops.norm(x)) <= 1e-13 * tt_ops.norm(x)
+    assert abs(tA.norm() - tt_ops.norm(A)) <= 1e-13 * tt_ops.norm(A)
+    xr = tt_ops.ortho_right(x)
+    t = TT(x).ortho_right()
+    assert t.ranks == tt_ops.ranks_of(xr)
+    np.testing.assert_allclose(t.matricize(), tt_ops.matricize(x), rtol=1e-12, atol=1e-12)
+    for c in t.cores[1:]:
+        m = c.reshape(c.shape[0], -1)
+        np.testing.assert_allclose(m @ m.T, np.eye(m.shape[0]), atol=1e-13)
+    t = TT(x).ortho_left()
+    assert t.ranks == tt_ops.ranks_of(tt_ops.ortho_left(x))
+    np.testing.assert_allclose(t.matricize(), tt_ops.matricize(x), rtol=1e-12, atol=1e-12)
+    u = TT(W.random_tt(rng, [1, 1], [1, 2, 1]))
+    assert isinstance(u.dot(TT(W.random_tt(rng, [1, 1], [1, 2, 1]))), float)
+    with pytest.raises(ValueError):
+        tx.dot(tx)
+    with pytest.raises(TypeError):
+        tx + 1
+    with pytest.raises(ValueError):
+        TT([np.zeros((1, 2, 1, 2)), np.zeros((3, 2, 1, 1))])
+    with pytest.raises(TypeError):
+        TT('cores')
+    assert tt.eye([2, 3]).matricize().shape == (6, 6)
+    np.testing.assert_allclose(tt.ones([2, 3], [1, 1], ranks=2).matricize(), 4 * np.ones(6))
+
+
+# --------------------------------------------------------------------------
+# whole solvers
+
+
+def _tt(cores):
+    from scikit_tt_amd.tensor_train import TT
+    return TT([np.asarray(c) for c in cores])
+
+
+def test_sle_golden_laplace(env):
+    import scikit_tt_amd.solvers.sle as sle
+    g = load_golden('sle_laplace_d10')
+    A, x0, b = g['tt']('A'), g['tt']('x0'), g['tt']('b')
+    (x,), (info,) = sle.als_batch([_tt(A)], [_tt(x0)], [_tt(b)], repeats=4, return_info=True)
+    assert info.status == 0 and info.unconverged == 0, info
+    assert x.ranks == list(g['ranks'])
+    xr = g['tt']('x')
+    diff = tt_ops.norm(tt_ops.sub(x.cores[:], xr)) / tt_ops.norm(xr)
+    assert diff < 1e-7, diff
+    rel = (_tt(A).dot(x) - _tt(b)).norm() / _tt(b).norm()
+    assert abs(rel - float(g['relres'])) < 1e-3 * float(g['relres'])
+
+
+@pytest.mark.parametrize('solver', ['lu', 'solve'])
+def test_sle_golden_toeplitz(env, solver):
+    """The reference test's indefinite Toeplitz pattern (tests/test_sle.py:20-59), smaller."""
+    import scikit_tt_amd.solvers.sle as sle
+    g = load_golden('sle_toeplitz_d6')
+    x = sle.als(_tt(g['tt']('A')), _tt(g['tt']('x0')), _tt(g['tt']('b')), repeats=2, solver=solver)
+    assert x.ranks == list(g['ranks'])
+    np.testing.assert_allclose(x.matricize(), tt_ops.matricize(g['tt']('x')), rtol=1e-9, atol=1e-12)
+    ds = g['dense_solution']
+    assert np.linalg.norm(x.matricize() - ds) / np.linalg.norm(ds) < 1e-7
+
+
+@pytest.mark.parametrize('name', ['evp_tfim_d8_eig', 'evp_tfim_d8_eigh', 'evp_tfim_d12_conv'])
+def test_evp_golden(env, name):
+    import scikit_tt_amd.solvers.evp as evp
+    g = load_golden(name)
+    lam, y, it = evp.als(_tt(g['tt']('A')), _tt(g['tt']('x0')), repeats=int(g['repeats']),
+                         conv_eps=float(g['conv_eps']), solver=str(g['solver']), sigma=float(g['sigma']))
+    assert it == int(g['iterations'])
+    assert abs(lam - float(g['eigenvalue'])) <= 1e-10 * abs(float(g['eigenvalue']))
+    assert y.ranks == list(g['ranks'])
+    yr = g['tt']('x')
+    d1 = tt_ops.norm(tt_ops.sub(y.cores[:], yr))
+    d2 = tt_ops.norm(tt_ops.add(y.cores[:], yr))
+    assert min(d1, d2) < 1e-6
+
+
+def test_sle_laplace_small_vs_oracle(env):
+    import scikit_tt_amd.solvers.sle as sle
+    A, x0, b = W.laplace_small()
+    for reps in (1, 2):
+        x = sle.als(_tt(A), _tt(x0), _tt(b), repeats=reps)
+        xr = sle_als.als(A, x0, b, repeats=reps)
+        assert x.ranks == tt_ops.ranks_of(xr)
+        assert tt_ops.norm(tt_ops.sub(x.cores[:], xr)) / tt_ops.norm(xr) < 1e-7
+
+
+def test_evp_tfim_chain_full_size(env):
+    """TFIM chain d=32, rank 16, 6 sweeps: lowest eigenvalue vs the free-fermion closed form."""
+    import scikit_tt_amd.solvers.evp as evp
+    H, y0, J, g = W.tfim_chain(seed=0, d=32, rank=16)
+    y0 = tt_ops.ortho_right(y0)
+    y0 = tt_ops.scale(y0, 1.0 / tt_ops.norm(y0))
+    lam, y, it = evp.als(_tt(H), _tt(y0), repeats=6, conv_eps=0, solver='eig', sigma=-100)
+    E0 = W.tfim_ground_energy(J, g)
+    assert it == 6
+    assert lam >= E0 - 1e-9 * abs(E0)          # variational
+    assert lam - E0 < 1e-7 * abs(E0), (lam, E0)
+    assert abs(y.norm() - 1.0) < 1e-10
+    # the Rayleigh quotient of the returned (unit) TT equals the returned eigenvalue
+    rq = y.transpose().dot(_tt(H).dot(y))
+    assert abs(rq - lam) < 1e-10 * abs(lam)
+
+
+def test_batch_equals_alone(env):
+    """Each system's result is the same alone and in a mixed batch (one workgroup per system)."""
+    import scikit_tt_amd.solvers.sle as sle
+    systems = [W.shifted_batch_instance(s, d_range=(4, 8), n_choices=(2, 4), r_range=(2, 6)) for s in range(6)]
+    systems.append(W.laplace_small(d=6, n=5, rank=3))
+    As = [_tt(A) for A, _, _ in systems]
+    Xs = [_tt(x) for _, x, _ in systems]
+    Bs = [_tt(b) for _, _, b in systems]
+    together, infos = sle.als_batch(As, Xs, Bs, repeats=2, return_info=True)
+    for k in range(len(systems)):
+        (alone,), _ = sle.als_batch([As[k]], [Xs[k]], [Bs[k]], repeats=2, return_info=True)
+        assert alone.ranks == together[k].ranks
+        for c1, c2 in zip(alone.cores, together[k].cores):
+            np.testing.assert_array_equal(c1, c2)
+
+
+def test_sle_errors(env):
+    import scikit_tt_amd.solvers.sle as sle
+    import scikit_tt_amd.solvers.evp as evp
+    A, x0, b = W.laplace_small(d=4, n=3, rank=2)
+    with pytest.raises(NotImplementedError):
+        sle.als(_tt(A), _tt(x0), _tt(b), solver='gmres')
+    with pytest.raises(TypeError):
+        sle.als(A, _tt(x0), _tt(b))
+    with pytest.raises(ValueError):
+        sle.als(_tt(A), _tt(x0[:3] + [np.ones((2, 3, 1, 1))]), _tt(b[:3] + [np.ones((1, 2, 1, 1))]))
+    zero = [np.zeros_like(c) for c in A]
+    with pytest.raises(np.linalg.LinAlgError):
+        sle.als(_tt(zero), _tt(x0), _tt(b))
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), number_ev=2)
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), operator_gevp=_tt(A))
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), solver='eigs')
