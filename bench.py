@@ -63,6 +63,53 @@ def cpu_baseline(raw_instance, sweeps):
                       '%.2f s measured' % (sweeps, sweeps, dt)}
 
 
+def serving_phase(rate, seconds, seed=7):
+    """Poisson arrivals of mixed evp (TFIM, d 8..32, rank 4..16 log-uniform) and
+    sle (-Laplacian, d 6..16, n 4/8, rank 4..8) requests into the
+    continuous-batching server; returns latency percentiles and throughput."""
+    import torch
+    from scikit_tt_amd import workloads as W
+    from scikit_tt_amd.tensor_train import TT
+    from scikit_tt_amd.serving import Server
+    rng = np.random.default_rng(seed)
+    nreq = max(1, int(rate * seconds))
+    gaps = rng.exponential(1.0 / rate, nreq)
+    reqs = []
+    for k in range(nreq):
+        if rng.random() < 0.5:
+            d = int(rng.integers(8, 33))
+            r = int(np.exp(rng.uniform(np.log(4), np.log(16))))
+            H, y0, J, g = W.tfim_chain(seed=1000 + k, d=d, rank=r)
+            t = TT(y0).ortho_right()
+            reqs.append(('evp', TT(H), (1.0 / t.norm()) * t, None))
+        else:
+            d, n = int(rng.integers(6, 17)), int(rng.choice([4, 8]))
+            r = int(rng.integers(4, 9))
+            A, x0, b = W.laplace_small(seed=2000 + k, d=d, n=n, rank=r)
+            reqs.append(('sle', TT(A), TT(x0), TT(b)))
+    torch.cuda.synchronize()
+    srv = Server(slots=4, max_batch=256)
+    t0 = time.perf_counter()
+    next_t = t0
+    for (kind, A, x0, b), gap in zip(reqs, gaps):
+        next_t += gap
+        while time.perf_counter() < next_t:
+            srv.poll()
+        if kind == 'evp':
+            srv.submit_evp(A, x0, repeats=4, conv_eps=1e-10, solver='eig', sigma=-100.0)
+        else:
+            srv.submit_sle(A, x0, b, repeats=2)
+        srv.poll()
+    srv.drain(timeout=600)
+    elapsed = time.perf_counter() - t0
+    st = srv.latency_stats()
+    bad = sum(1 for t in srv.done if t.info.status not in (0,))
+    srv.close()
+    return {'rate_req_per_s': rate, 'requests': nreq, 'solves_per_s': round(nreq / elapsed, 2),
+            'p50_ms': round(st['p50_ms'], 3), 'p99_ms': round(st['p99_ms'], 3), 'launches': st['launches'],
+            'nonzero_status': bad}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -74,6 +121,8 @@ def main():
     ap.add_argument('--sweeps', type=int, default=6)
     ap.add_argument('--plain', type=int, default=4, help='systems timed one per call on the plain path')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--serve-rates', default='100,1000', help='Poisson arrival rates (req/s) of the serving phase')
+    ap.add_argument('--serve-seconds', type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: every box here has one MI355X; the multi-GPU layout is '
@@ -156,7 +205,13 @@ def main():
                      'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None,
                      'kernel': 'k_als<double>', 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
     }
+    if args.serve_rates:
+        line['serving'] = []
+        for rate in [float(r) for r in args.serve_rates.split(',') if r]:
+            print('[bench] serving at %.0f req/s' % rate, file=sys.stderr, flush=True)
+            line['serving'].append(serving_phase(rate, args.serve_seconds))
     if not args.no_cpu:
+        print('[bench] cpu baseline', file=sys.stderr, flush=True)
         line['cpu_baseline'] = cpu_baseline(raw[0], args.sweeps)
     print(json.dumps(line), flush=True)
 

@@ -64,6 +64,8 @@ typedef struct ttals_result {
   double lambda;           /* evp: eigenvalue returned */
   double worst_residual;   /* sle: worst final relative residual of a local Krylov solve */
   double flops;            /* algorithmic flops of the solve (from shapes and iteration counts) */
+  double cycles[7];        /* shader clock cycles: total, matvec, Gram-Schmidt, tridiagonal eigen,
+                              QR + warm start, stack updates, on-chip set-up (diagnostics) */
 } ttals_result;
 
 /* ---- context ------------------------------------------------------------ */
@@ -97,6 +99,21 @@ int ttals_sle_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt*
 int ttals_evp_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
                         const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
                         int maxit, ttals_result* res);
+
+/* ---- asynchronous form of the two solvers (serving) ----------------------
+ * ttals_*_submit enqueue the whole solve on `stream` and return at once;
+ * ttals_query returns 1 when the context's submission has finished (0 if
+ * still running); ttals_wait blocks, writes x_out ranks and `count` results.
+ * One submission may be pending per context; the x_out descriptors' rank
+ * arrays must stay valid until ttals_wait.  No counterpart in the reference
+ * (its solvers are synchronous Python). */
+int ttals_sle_submit(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                     const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit);
+int ttals_evp_submit(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                     const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
+                     int maxit);
+int ttals_query(ttals_ctx* ctx);
+int ttals_wait(ttals_ctx* ctx, ttals_result* res);
 
 /* ---- hot-path pieces (one workgroup; for unit parity) --------------------
  * Shapes: x core (a,n,b); operator core (P,n,n,Q); op stacks L (a,P,a),
@@ -156,11 +173,14 @@ int ttals_tt_norm(ttals_ctx* ctx, void* stream, const ttals_tt* t, double* out);
 
 /* scikit_tt/tensor_train.py:1092-1205 (dir 0, ortho_left over cores
  * start..end) and 1207-1310 (dir 1, ortho_right over cores start..end
- * descending) without truncation: Householder QR sweep, in place; the
- * cores keep their capacity and t->ranks receives the new ranks (the
- * reference's thin-SVD ranks min(r_i m_i n_i, r_{i+1}) / min(r_i,
- * m_i n_i r_{i+1})).  Synchronous. */
-int ttals_tt_ortho(ttals_ctx* ctx, void* stream, const ttals_tt* t, int dir, int start, int end);
+ * descending), in place: the cores keep their capacity and t->ranks receives
+ * the new ranks.  threshold == 0 and no max_ranks: Householder QR sweep (the
+ * reference's thin-SVD ranks, same tensor).  threshold > 0 or max_ranks
+ * (host int[order+1], <= 0 = unbounded): QR then one-sided Jacobi SVD of R,
+ * keeping s/s0 > threshold then at most max_ranks[bond] triplets
+ * (utils.py:145-156 rule).  Synchronous. */
+int ttals_tt_ortho(ttals_ctx* ctx, void* stream, const ttals_tt* t, int dir, int start, int end, double threshold,
+                   const int* max_ranks);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,8 @@ class Result(ctypes.Structure):
                 ('local_solves', ctypes.c_longlong),
                 ('lambda_', ctypes.c_double),
                 ('worst_residual', ctypes.c_double),
-                ('flops', ctypes.c_double)]
+                ('flops', ctypes.c_double),
+                ('cycles', ctypes.c_double * 7)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -51,6 +52,10 @@ SIGNATURES = {
     'ttals_last_error': (ctypes.c_char_p, [_vp]),
     'ttals_sle_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _Rp]),
     'ttals_evp_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i, _Rp]),
+    'ttals_sle_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i]),
+    'ttals_evp_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i]),
+    'ttals_query': (_i, [_vp]),
+    'ttals_wait': (_i, [_vp, _Rp]),
     'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_left_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
@@ -65,7 +70,7 @@ SIGNATURES = {
     'ttals_core_product': (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
     'ttals_core_embed': (_i, [_vp, _i, _i, _i, _vp, _d, _i, _i, _i, _i, _vp]),
     'ttals_tt_norm': (_i, [_vp, _vp, _TTp, c_double_p]),
-    'ttals_tt_ortho': (_i, [_vp, _vp, _TTp, _i, _i, _i]),
+    'ttals_tt_ortho': (_i, [_vp, _vp, _TTp, _i, _i, _i, _d, c_int_p]),
     'ttals_set_trace': (_i, [_vp, _vp, ctypes.c_longlong]),
 }
 
@@ -104,6 +109,16 @@ def context(device):
             c = h
             _ctx[device] = c
     return c
+
+
+def new_context(device):
+    """A dedicated engine context (own workspace; one pending submission at a time)."""
+    lib = load()
+    h = ctypes.c_void_p()
+    rc = lib.ttals_ctx_create(int(device), ctypes.byref(h))
+    if rc != 0:
+        raise RuntimeError('ttals_ctx_create(%d) failed with %d' % (device, rc))
+    return h
 
 
 def check(rc, ctx=None, what='ttals'):

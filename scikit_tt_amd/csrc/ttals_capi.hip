@@ -22,7 +22,7 @@ using namespace ttals;
 // GEMM staging tiles) holds either the staging tiles or the on-chip copy of
 // a local problem.
 template <class ACC>
-__global__ __launch_bounds__(NT) void k_als(const SysDesc* descs, long arena) {
+__global__ __launch_bounds__(NT, 2) void k_als(const SysDesc* descs, long arena) {
   extern __shared__ double dyn_lds[];
   __shared__ SolveLDS S;
   als_solve<ACC>(descs[blockIdx.x], reinterpret_cast<ACC*>(dyn_lds), arena, S);
@@ -110,9 +110,10 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       W.V = A.V;
       W.cur = A.V + (long)(KMAX + 1) * N;
       W.w = A.V + (long)(KMAX + 2) * N;
+      W.r = A.V + (long)(KMAX + 3) * N;
       int its = 0;
       double lam = 0.0;
-      const int r = wg_lanczos<double>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      const int r = wg_lanczos<double, false>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
       if (threadIdx.x == 0) {
         A.res[0] = r;
         A.res[1] = its;
@@ -216,11 +217,19 @@ __global__ __launch_bounds__(NT) void k_tt_norm(NormArgs A) {
   if (threadIdx.x == 0) A.out[0] = nrm;
 }
 
-// Left/right orthonormalisation sweep of a TT by Householder QR (one
-// workgroup).  Cores are updated in place (capacity of the input ranks);
-// ranks (device int[d+1]) are updated.  dims: mn[i] = m_i * n_i.
+// Left/right orthonormalisation sweep of a TT (one workgroup).  Cores are
+// updated in place (capacity of the input ranks); ranks (device int[d+1]) are
+// updated.  dims: mn[i] = m_i * n_i.
+//   no truncation: Householder QR of the tall unfolding, Q kept, R pushed on;
+//   truncation (threshold > 0 or a finite max rank): the same QR, then a
+//   one-sided Jacobi SVD of R, singular triplets with s/s0 > threshold (then
+//   at most max_rank[bond]) kept, S V^T (or U S) pushed on
+//   (scikit_tt/tensor_train.py:1162-1189, 1268-1299).
 struct OrthoArgs {
   int d, dir, start, end;  // dir 0: left (i = start..end), 1: right (i = start..end descending)
+  int trunc;
+  double thr;
+  const int* maxr;  // [d+1], <= 0 = unbounded
   const int* mn;
   int* r;
   double* const* c;
@@ -229,7 +238,62 @@ struct OrthoArgs {
   double* Rc;  // >= rmax^2
   double* T;   // >= max core size
   double* tau; // >= rmax
+  double* Bm;  // >= rmax^2
+  double* Vm;  // >= rmax^2
+  double* sig; // >= rmax
+  int* perm;   // >= rmax
 };
+
+// T (mt x ct) tall unfolding in A.W; on return the kept left factor
+// (mt x kk, row-major) is in A.Qm and the pushed factor in A.Bm:
+// B[:, perm] (ct x kk, row-major, ld kk).  Returns kk.
+__device__ int ortho_factor(const OrthoArgs& A, int mt, int ct, int cap, double* lds, SolveLDS& S) {
+  wg_householder(A.W, mt, ct, A.tau, S.red);
+  wg_form_q(A.W, mt, ct, A.tau, A.T);  // Q (mt x k) in T
+  const int k = mt < ct ? mt : ct;
+  if (!A.trunc) {
+    for (long e = threadIdx.x; e < (long)k * ct; e += NT) {
+      const int ii = (int)(e / ct), jj = (int)(e % ct);
+      A.Rc[e] = (jj >= ii) ? A.W[(long)ii * ct + jj] : 0.0;
+    }
+    wg_copy(A.T, A.Qm, mt * k);
+    // pushed factor R^T (ct x k)
+    for (long e = threadIdx.x; e < (long)ct * k; e += NT) {
+      const int ii = (int)(e / k), jj = (int)(e % k);
+      A.Bm[e] = A.Rc[(long)jj * ct + ii];
+    }
+    __syncthreads();
+    return k;
+  }
+  // B = R^T (ct x k); Jacobi: B = U_B S, V
+  for (long e = threadIdx.x; e < (long)ct * k; e += NT) {
+    const int ii = (int)(e / k), jj = (int)(e % k);
+    A.Rc[e] = (ii >= jj) ? A.W[(long)jj * ct + ii] : 0.0;
+  }
+  __syncthreads();
+  wg_jacobi_svd(A.Rc, ct, k, A.Vm, A.sig, A.perm, S.red);
+  // number kept: s/s0 > thr, then the cap
+  int kk = 0;
+  const double s0 = A.sig[A.perm[0]];
+  for (int j = 0; j < k; ++j)
+    if (A.thr == 0.0 || A.sig[A.perm[j]] / s0 > A.thr) ++kk;
+  if (cap > 0 && kk > cap) kk = cap;
+  if (kk < 1) kk = 1;
+  // left factor (Q V)[:, perm[:kk]] -> Qm (mt x kk)
+  wg_gemm<double, double>(mt, k, k, crm<double>(A.T, k), crm<double>(A.Vm, k), mrm<double>(A.W, k), lds);
+  for (long e = threadIdx.x; e < (long)mt * kk; e += NT) {
+    const int ii = (int)(e / kk), jj = (int)(e % kk);
+    A.Qm[e] = A.W[(long)ii * k + A.perm[jj]];
+  }
+  // pushed factor B[:, perm[:kk]] (ct x kk)
+  for (long e = threadIdx.x; e < (long)ct * kk; e += NT) {
+    const int ii = (int)(e / kk), jj = (int)(e % kk);
+    A.Bm[e] = A.Rc[(long)ii * k + A.perm[jj]];
+  }
+  __syncthreads();
+  return kk;
+}
+
 __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
   __shared__ double lds[GEMM_LDS_DOUBLES];
   __shared__ SolveLDS S;
@@ -239,19 +303,13 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
       const int m = a * q, c = b;
       wg_copy(A.c[i], A.W, m * c);
       __syncthreads();
-      wg_householder(A.W, m, c, A.tau, S.red);
-      wg_form_q(A.W, m, c, A.tau, A.Qm);
-      const int k = m < c ? m : c;
-      for (long e = threadIdx.x; e < (long)k * c; e += NT) {
-        const int ii = (int)(e / c), jj = (int)(e % c);
-        A.Rc[e] = (jj >= ii) ? A.W[(long)ii * c + jj] : 0.0;
-      }
+      const int k = ortho_factor(A, m, c, A.maxr[i + 1], lds, S);
       wg_copy(A.Qm, A.c[i], m * k);
       __syncthreads();
-      // core_{i+1} (b x rest) <- R (k x b) * core_{i+1}
+      // core_{i+1} (b x rest) <- Bm^T (k x b) * core_{i+1}
       const int rest = A.mn[i + 1] * A.r[i + 2];
-      wg_gemm<double, double>(k, rest, b, crm<double>(A.Rc, c), crm<double>(A.c[i + 1], rest), mrm<double>(A.T, rest),
-                              lds);
+      wg_gemm<double, double>(k, rest, b, ctr<double>(A.Bm, k), crm<double>(A.c[i + 1], rest),
+                              mrm<double>(A.T, rest), lds);
       wg_copy(A.T, A.c[i + 1], k * rest);
       if (threadIdx.x == 0) A.r[i + 1] = k;
       __syncthreads();
@@ -265,21 +323,15 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
         A.W[e] = A.c[i][(long)jj * m + ii];
       }
       __syncthreads();
-      wg_householder(A.W, m, c, A.tau, S.red);
-      wg_form_q(A.W, m, c, A.tau, A.Qm);
-      const int k = m < c ? m : c;
-      for (long e = threadIdx.x; e < (long)k * c; e += NT) {
-        const int ii = (int)(e / c), jj = (int)(e % c);
-        A.Rc[e] = (jj >= ii) ? A.W[(long)ii * c + jj] : 0.0;
-      }
-      // core_i (k x m) = Q^T
+      const int k = ortho_factor(A, m, c, A.maxr[i], lds, S);
+      // core_i (k x m) = Qm^T
       for (long e = threadIdx.x; e < (long)k * m; e += NT) {
         const int rr = (int)(e / m), cc = (int)(e % m);
         A.c[i][e] = A.Qm[(long)cc * k + rr];
       }
       __syncthreads();
       const int rows = A.r[i - 1] * A.mn[i - 1];
-      wg_gemm<double, double>(rows, k, a, crm<double>(A.c[i - 1], a), ctr<double>(A.Rc, c), mrm<double>(A.T, k),
+      wg_gemm<double, double>(rows, k, a, crm<double>(A.c[i - 1], a), crm<double>(A.Bm, k), mrm<double>(A.T, k),
                               lds);
       wg_copy(A.T, A.c[i - 1], rows * k);
       if (threadIdx.x == 0) A.r[i] = k;
@@ -291,12 +343,21 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
 // ---------------------------------------------------------------------------
 // host side
 
+struct Pending {
+  size_t r_rx, r_out;  // staging offsets of the ranks and scalars of one system
+  int d;
+  int* ranks;          // caller's output rank array
+};
+
 struct ttals_ctx {
   int device = 0;
   void* dws = nullptr;
   size_t dws_bytes = 0;
   char* hst = nullptr;       // pinned staging
   size_t hst_bytes = 0;
+  hipEvent_t done = nullptr;  // completion of the last submission
+  bool inflight = false;
+  std::vector<Pending> pending;
   double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
   long trace_cap = 0;
   long arena = 6144;         // on-chip arena of k_als in doubles (48 KB)
@@ -322,6 +383,7 @@ int hip_fail(ttals_ctx* c, hipError_t e, const char* where) {
 
 int ensure_dws(ttals_ctx* c, size_t bytes) {
   if (bytes <= c->dws_bytes) return 0;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
   if (c->dws) (void)hipFree(c->dws);
   c->dws = nullptr;
   c->dws_bytes = 0;
@@ -337,6 +399,7 @@ int ensure_dws(ttals_ctx* c, size_t bytes) {
 
 int ensure_hst(ttals_ctx* c, size_t bytes) {
   if (bytes <= c->hst_bytes) return 0;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "staging busy: a submission is pending on this context");
   if (c->hst) (void)hipHostFree(c->hst);
   c->hst = nullptr;
   c->hst_bytes = 0;
@@ -454,10 +517,11 @@ T* hptr(ttals_ctx* c, size_t off) { return reinterpret_cast<T*>(c->hst + off); }
 
 // Builds descriptors for `count` systems, runs k_als (one workgroup per
 // system), copies ranks and scalars back and synchronises the stream.
-int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals_tt* A, const ttals_tt* x0,
-              const ttals_tt* b, const ttals_tt* xo, int repeats, int solver, double tol, int maxit, double sigma,
-              double conv_eps, ttals_result* res) {
-  if (count < 1 || !A || !x0 || !xo || !res) return fail(c, TTALS_E_ARG, "bad batch arguments");
+int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals_tt* A, const ttals_tt* x0,
+                 const ttals_tt* b, const ttals_tt* xo, int repeats, int solver, double tol, int maxit, double sigma,
+                 double conv_eps) {
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is still pending on this context (call ttals_wait)");
+  if (count < 1 || !A || !x0 || !xo) return fail(c, TTALS_E_ARG, "bad batch arguments");
   if (kind == 0 && !b) return fail(c, TTALS_E_ARG, "missing right-hand side");
   if (repeats < 0) return fail(c, TTALS_E_ARG, "repeats must be >= 0");
   if (!(tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
@@ -604,12 +668,25 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
                              hipMemcpyDeviceToHost, stream));
     HIPCHK(c, hipMemcpyAsync(c->hst + r_out[s], wptr(plans[s].w_out), 8 * 16, hipMemcpyDeviceToHost, stream));
   }
-  HIPCHK(c, hipStreamSynchronize(stream));
+  if (!c->done) HIPCHK(c, hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(c->done, stream));
+  c->pending.clear();
+  for (int s = 0; s < count; ++s) c->pending.push_back(Pending{r_rx[s], r_out[s], plans[s].d, xo[s].ranks});
+  c->inflight = true;
+  return 0;
+}
+
+// Waits for the context's submission and unpacks ranks and results.
+int finish_batch(ttals_ctx* c, ttals_result* res) {
+  if (!c->inflight) return fail(c, TTALS_E_ARG, "no submission pending on this context");
+  c->inflight = false;
+  HIPCHK(c, hipEventSynchronize(c->done));
   int worst = 0;
-  for (int s = 0; s < count; ++s) {
-    const int* rr = hptr<int>(c, r_rx[s]);
-    for (int i = 0; i <= plans[s].d; ++i) xo[s].ranks[i] = rr[i];
-    const double* o = hptr<double>(c, r_out[s]);
+  for (size_t s = 0; s < c->pending.size(); ++s) {
+    const Pending& pd = c->pending[s];
+    const int* rr = hptr<int>(c, pd.r_rx);
+    for (int i = 0; i <= pd.d; ++i) pd.ranks[i] = rr[i];
+    const double* o = hptr<double>(c, pd.r_out);
     ttals_result& R = res[s];
     R.lambda = o[0];
     R.sweeps = (int)o[1];
@@ -620,6 +697,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
     R.max_iters = (int)o[6];
     R.worst_residual = o[7];
     R.flops = o[8];
+    for (int k = 0; k < 7; ++k) R.cycles[k] = o[9 + k];
     if (R.status == ST_OK && R.unconverged > 0) R.status = ST_NOCONV;
     if (R.status != ST_OK && (worst == 0 || (R.status != ST_NOCONV && worst == ST_NOCONV))) worst = R.status;
   }
@@ -628,6 +706,7 @@ int run_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals
 
 // scratch for the piece entry points: 2*t + (KMAX+2)*N doubles
 int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double** V, double** res) {
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
   const long nv = std::max<long>((KMAX + 4) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
   const size_t bytes = 8 * (size_t)(2 * t + nv + 16) + 4096;
   int rc = ensure_dws(c, bytes);
@@ -678,6 +757,10 @@ int ttals_ctx_create(int device, ttals_ctx** out) {
 void ttals_ctx_destroy(ttals_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
+  if (c->done) {
+    (void)hipEventSynchronize(c->done);
+    (void)hipEventDestroy(c->done);
+  }
   if (c->dws) (void)hipFree(c->dws);
   if (c->hst) (void)hipHostFree(c->hst);
   delete c;
@@ -685,21 +768,50 @@ void ttals_ctx_destroy(ttals_ctx* c) {
 
 const char* ttals_last_error(ttals_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+int ttals_sle_submit(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0, const ttals_tt* b,
+                     const ttals_tt* x_out, int repeats, int solver, double tol, int maxit) {
+  if (!c) return TTALS_E_ARG;
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "sle solver must be 'solve' or 'lu'");
+  return submit_batch(c, (hipStream_t)stream, 0, count, A, x0, b, x_out, repeats, solver, tol, maxit, 0.0, 0.0);
+}
+
+int ttals_evp_submit(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                     const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
+                     int maxit) {
+  if (!c) return TTALS_E_ARG;
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "evp solver must be 'eig' or 'eigh'");
+  return submit_batch(c, (hipStream_t)stream, 1, count, A, x0, nullptr, x_out, repeats, solver, tol, maxit, sigma,
+                      conv_eps);
+}
+
+int ttals_query(ttals_ctx* c) {
+  if (!c) return TTALS_E_ARG;
+  if (!c->inflight) return 1;
+  const hipError_t e = hipEventQuery(c->done);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return hip_fail(c, e, "hipEventQuery");
+}
+
+int ttals_wait(ttals_ctx* c, ttals_result* res) {
+  if (!c || !res) return TTALS_E_ARG;
+  return finish_batch(c, res);
+}
+
 int ttals_sle_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
                         const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
                         ttals_result* res) {
-  if (!c) return TTALS_E_ARG;
-  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "sle solver must be 'solve' or 'lu'");
-  return run_batch(c, (hipStream_t)stream, 0, count, A, x0, b, x_out, repeats, solver, tol, maxit, 0.0, 0.0, res);
+  if (!res) return c ? fail(c, TTALS_E_ARG, "null result array") : TTALS_E_ARG;
+  int rc = ttals_sle_submit(c, stream, count, A, x0, b, x_out, repeats, solver, tol, maxit);
+  return rc ? rc : finish_batch(c, res);
 }
 
 int ttals_evp_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
                         const ttals_tt* x_out, int repeats, double conv_eps, int solver, double sigma, double tol,
                         int maxit, ttals_result* res) {
-  if (!c) return TTALS_E_ARG;
-  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "evp solver must be 'eig' or 'eigh'");
-  return run_batch(c, (hipStream_t)stream, 1, count, A, x0, nullptr, x_out, repeats, solver, tol, maxit, sigma,
-                   conv_eps, res);
+  if (!res) return c ? fail(c, TTALS_E_ARG, "null result array") : TTALS_E_ARG;
+  int rc = ttals_evp_submit(c, stream, count, A, x0, x_out, repeats, conv_eps, solver, sigma, tol, maxit);
+  return rc ? rc : finish_batch(c, res);
 }
 
 #define PIECE_PROLOGUE(tsize, Nsize)                                  \
@@ -862,6 +974,7 @@ int ttals_core_embed(void* stream, int r0, int mn, int r1, const double* src, do
 
 int ttals_tt_norm(ttals_ctx* c, void* stream, const ttals_tt* t, double* out) {
   if (!c || !t || !out) return TTALS_E_ARG;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
   int rc = check_tt(c, *t, false, "tt");
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
@@ -922,42 +1035,57 @@ int ttals_set_trace(ttals_ctx* c, double* buf, long long cap) {
   return 0;
 }
 
-int ttals_tt_ortho(ttals_ctx* c, void* stream, const ttals_tt* t, int dir, int start, int end) {
+int ttals_tt_ortho(ttals_ctx* c, void* stream, const ttals_tt* t, int dir, int start, int end, double threshold,
+                   const int* max_ranks) {
   if (!c || !t) return TTALS_E_ARG;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
   int rc = check_tt(c, *t, false, "tt");
   if (rc) return rc;
   const int d = t->order;
   if (dir == 0 && (start < 0 || end > d - 2 || start > end + 1)) return fail(c, TTALS_E_ARG, "bad index range");
   if (dir == 1 && (start > d - 1 || end < 1 || start < end - 1)) return fail(c, TTALS_E_ARG, "bad index range");
   if (dir != 0 && dir != 1) return fail(c, TTALS_E_ARG, "bad direction");
+  if (!(threshold >= 0.0)) return fail(c, TTALS_E_ARG, "threshold must be >= 0");
   HIPCHK(c, hipSetDevice(c->device));
   hipStream_t st = (hipStream_t)stream;
   long cmax = 1, rmax = 1;
+  bool trunc = threshold > 0.0;
   for (int i = 0; i < d; ++i) {
     cmax = std::max(cmax, (long)t->ranks[i] * t->row_dims[i] * t->col_dims[i] * t->ranks[i + 1]);
     rmax = std::max(rmax, (long)std::max(t->ranks[i], t->ranks[i + 1]));
   }
+  if (max_ranks)
+    for (int i = 0; i <= d; ++i) trunc = trunc || max_ranks[i] > 0;
   Layout L;
   const size_t oW = L.take(8 * cmax), oQ = L.take(8 * cmax), oR = L.take(8 * rmax * rmax), oT = L.take(8 * cmax),
-               otau = L.take(8 * rmax);
-  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), optr = L.take(sizeof(void*) * d);
+               otau = L.take(8 * rmax), oB = L.take(8 * rmax * rmax), oV = L.take(8 * rmax * rmax),
+               osig = L.take(8 * rmax), operm = L.take(sizeof(int) * rmax);
+  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), omx = L.take(sizeof(int) * (d + 1)),
+               optr = L.take(sizeof(void*) * d);
   if ((rc = ensure_dws(c, L.bytes))) return rc;
   if ((rc = ensure_hst(c, L.bytes))) return rc;
   char* db = reinterpret_cast<char*>(c->dws);
   int* hmn = reinterpret_cast<int*>(c->hst + omn);
   int* hrr = reinterpret_cast<int*>(c->hst + orr);
+  int* hmx = reinterpret_cast<int*>(c->hst + omx);
   double** hp = reinterpret_cast<double**>(c->hst + optr);
   for (int i = 0; i < d; ++i) {
     hmn[i] = t->row_dims[i] * t->col_dims[i];
     hp[i] = t->cores[i];
   }
-  for (int i = 0; i <= d; ++i) hrr[i] = t->ranks[i];
+  for (int i = 0; i <= d; ++i) {
+    hrr[i] = t->ranks[i];
+    hmx[i] = max_ranks ? max_ranks[i] : 0;
+  }
   HIPCHK(c, hipMemcpyAsync(db + omn, c->hst + omn, L.bytes - omn, hipMemcpyHostToDevice, st));
   OrthoArgs oa;
   oa.d = d;
   oa.dir = dir;
   oa.start = start;
   oa.end = end;
+  oa.trunc = trunc ? 1 : 0;
+  oa.thr = threshold;
+  oa.maxr = reinterpret_cast<const int*>(db + omx);
   oa.mn = reinterpret_cast<const int*>(db + omn);
   oa.r = reinterpret_cast<int*>(db + orr);
   oa.c = reinterpret_cast<double* const*>(db + optr);
@@ -966,6 +1094,10 @@ int ttals_tt_ortho(ttals_ctx* c, void* stream, const ttals_tt* t, int dir, int s
   oa.Rc = reinterpret_cast<double*>(db + oR);
   oa.T = reinterpret_cast<double*>(db + oT);
   oa.tau = reinterpret_cast<double*>(db + otau);
+  oa.Bm = reinterpret_cast<double*>(db + oB);
+  oa.Vm = reinterpret_cast<double*>(db + oV);
+  oa.sig = reinterpret_cast<double*>(db + osig);
+  oa.perm = reinterpret_cast<int*>(db + operm);
   hipLaunchKernelGGL(k_tt_ortho, dim3(1), dim3(NT), 0, st, oa);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(c->hst + orr, db + orr, sizeof(int) * (d + 1), hipMemcpyDeviceToHost, st));

@@ -55,6 +55,27 @@ template <class T>
 __device__ __forceinline__ CView<T> ctr(const T* p, int ld) { return CView<T>{p, plain(1), plain(ld)}; }
 
 // ---------------------------------------------------------------------------
+// Memory-space accessors.  Mem<true> reads/writes through an LDS-qualified
+// pointer (the address must lie in the workgroup's LDS) so the compiler
+// emits ds_read/ds_write instead of flat accesses; Mem<false> is a plain
+// (global or generic) access.
+#define TT_LDS __attribute__((address_space(3)))
+template <bool OnChip>
+struct Mem {
+  template <class T>
+  static __device__ __forceinline__ T ld(const T* p, long i) { return p[i]; }
+  template <class T>
+  static __device__ __forceinline__ void st(T* p, long i, T v) { p[i] = v; }
+};
+template <>
+struct Mem<true> {
+  template <class T>
+  static __device__ __forceinline__ T ld(const T* p, long i) { return ((const TT_LDS T*)p)[i]; }
+  template <class T>
+  static __device__ __forceinline__ void st(T* p, long i, T v) { ((TT_LDS T*)p)[i] = v; }
+};
+
+// ---------------------------------------------------------------------------
 // reductions (fixed order: deterministic for a given shape)
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -219,7 +240,7 @@ struct IWalk {
   }
 };
 
-template <class T, class ACC>
+template <class T, class ACC, bool OA = false, bool OB = false, bool OC = false>
 __device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C) {
   const int tN = (N + 1) >> 1;
   const int tiles = ((M + 1) >> 1) * tN;
@@ -234,8 +255,8 @@ __device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C)
     kb.start(B.r);
     ACC c00 = 0, c01 = 0, c10 = 0, c11 = 0;
     for (int k = 0; k < K; ++k) {
-      const ACC a0 = (ACC)A.p[ar0 + ka.off], a1 = (ACC)A.p[ar1 + ka.off];
-      const ACC b0 = (ACC)B.p[kb.off + bc0], b1 = (ACC)B.p[kb.off + bc1];
+      const ACC a0 = (ACC)Mem<OA>::ld(A.p, ar0 + ka.off), a1 = (ACC)Mem<OA>::ld(A.p, ar1 + ka.off);
+      const ACC b0 = (ACC)Mem<OB>::ld(B.p, kb.off + bc0), b1 = (ACC)Mem<OB>::ld(B.p, kb.off + bc1);
       c00 += a0 * b0;
       c01 += a0 * b1;
       c10 += a1 * b0;
@@ -243,11 +264,13 @@ __device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C)
       ka.next();
       kb.next();
     }
-    C.at(i0, j0) = (T)c00;
-    if (j1) C.at(i0, j0 + 1) = (T)c01;
+    const long cr0 = imap(C.r, i0), cc0 = imap(C.c, j0);
+    Mem<OC>::st(C.p, cr0 + cc0, (T)c00);
+    if (j1) Mem<OC>::st(C.p, cr0 + imap(C.c, j0 + 1), (T)c01);
     if (i1) {
-      C.at(i0 + 1, j0) = (T)c10;
-      if (j1) C.at(i0 + 1, j0 + 1) = (T)c11;
+      const long cr1 = imap(C.r, i0 + 1);
+      Mem<OC>::st(C.p, cr1 + cc0, (T)c10);
+      if (j1) Mem<OC>::st(C.p, cr1 + imap(C.c, j0 + 1), (T)c11);
     }
   }
   __syncthreads();
@@ -295,15 +318,17 @@ __device__ void local_matvec(int a, int P, int n, int Q, int b, const T* L, cons
 }
 
 // Same contraction with every operand on-chip (ds_gemm stages).
+// (all operands, intermediates, v and y in LDS)
 template <class T, class ACC>
 __device__ void local_matvec_ds(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R,
                                 const T* v, T* y, T* t1, T* t2) {
   const int a2 = a;
-  ds_gemm<T, ACC>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
-                  mview<T>(t1, split(a2, n, b * P * n), split(b, 1, P * n)));
-  ds_gemm<T, ACC>(a * b, n * Q, P * n, crm<T>(t1, P * n), cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
-                  mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)));
-  ds_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b));
+  ds_gemm<T, ACC, true, true, true>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
+                                    mview<T>(t1, split(a2, n, b * P * n), split(b, 1, P * n)));
+  ds_gemm<T, ACC, true, true, true>(a * b, n * Q, P * n, crm<T>(t1, P * n),
+                                    cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+                                    mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)));
+  ds_gemm<T, ACC, true, true, true>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b));
 }
 
 // Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']
@@ -460,6 +485,84 @@ __device__ void wg_form_q(const double* W, int m, int c, const double* tau, doub
     }
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// One-sided (Hestenes) Jacobi SVD of a tall row-major c x k matrix B
+// (c >= k), in place: on exit B = U * diag(sig) (columns orthogonal), V (k x
+// k, row-major) accumulates the rotations, so B_in = U diag(sig) V^T.
+// Round-robin pairing (k/2 disjoint column pairs per round, one wave per
+// pair); sweeps until every pair satisfies |b_i.b_j| <= eps sqrt(|b_i|^2
+// |b_j|^2) or 60 sweeps.  Columns are then sorted by decreasing sigma and
+// the column norms returned in sig.  perm: int scratch of k.
+__device__ void wg_jacobi_svd(double* B, int c, int k, double* V, double* sig, int* perm, double* red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ int s_rot;
+  for (long e = threadIdx.x; e < (long)k * k; e += NT) V[e] = ((e / k) == (e % k)) ? 1.0 : 0.0;
+  __syncthreads();
+  const int kk = (k & 1) ? k + 1 : k;  // padded for round-robin (index k = dummy)
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    if (threadIdx.x == 0) s_rot = 0;
+    __syncthreads();
+    for (int round = 0; round < kk - 1; ++round) {
+      for (int pr = wv; pr < kk / 2; pr += NWAVE) {
+        // round-robin: position 0 fixed, the others rotate
+        int p0 = pr == 0 ? 0 : 1 + (pr - 1 + round) % (kk - 1);
+        int p1 = 1 + (kk - 2 - pr + round) % (kk - 1);
+        int i = p0 < p1 ? p0 : p1, j = p0 < p1 ? p1 : p0;
+        if (j >= k) continue;  // dummy partner
+        double aii = 0.0, ajj = 0.0, aij = 0.0;
+        for (int r = lane; r < c; r += 64) {
+          const double bi = B[(long)r * k + i], bj = B[(long)r * k + j];
+          aii += bi * bi;
+          ajj += bj * bj;
+          aij += bi * bj;
+        }
+        aii = wave_sum(aii);
+        ajj = wave_sum(ajj);
+        aij = wave_sum(aij);
+        if (fabs(aij) <= 1e-15 * sqrt(aii * ajj) || aij == 0.0) continue;
+        const double zeta = (ajj - aii) / (2.0 * aij);
+        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+        for (int r = lane; r < c; r += 64) {
+          const double bi = B[(long)r * k + i], bj = B[(long)r * k + j];
+          B[(long)r * k + i] = cs * bi - sn * bj;
+          B[(long)r * k + j] = sn * bi + cs * bj;
+        }
+        for (int r = lane; r < k; r += 64) {
+          const double vi = V[(long)r * k + i], vj = V[(long)r * k + j];
+          V[(long)r * k + i] = cs * vi - sn * vj;
+          V[(long)r * k + j] = sn * vi + cs * vj;
+        }
+        if (lane == 0) s_rot = 1;
+      }
+      __syncthreads();
+    }
+    if (s_rot == 0) break;
+    __syncthreads();
+  }
+  // column norms, then sort by decreasing norm (selection by thread 0)
+  for (int j = wv; j < k; j += NWAVE) {
+    double s2 = 0.0;
+    for (int r = lane; r < c; r += 64) s2 += B[(long)r * k + j] * B[(long)r * k + j];
+    s2 = wave_sum(s2);
+    if (lane == 0) sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < k; ++j) perm[j] = j;
+    for (int j = 0; j < k; ++j) {
+      int best = j;
+      for (int q = j + 1; q < k; ++q)
+        if (sig[perm[q]] > sig[perm[best]]) best = q;
+      const int t = perm[j];
+      perm[j] = perm[best];
+      perm[best] = t;
+    }
+  }
+  __syncthreads();
+  (void)red;
 }
 
 // ---------------------------------------------------------------------------

@@ -59,11 +59,29 @@ __device__ __forceinline__ void set_one(double* p) {
   if (threadIdx.x == 0) p[0] = 1.0;
 }
 
+// Buffers of one QR: W (m x c), Q (m x k), R (k x c), tau (k).
+struct QRBuf {
+  double* W;
+  double* Q;
+  double* R;
+  double* tau;
+};
+
+// On-chip QR buffers carved from the arena when m x c fits, else the global
+// scratch of the system.
+__device__ __forceinline__ QRBuf qr_buffers(const SysDesc& D, int m, int c, double* arena, long cap) {
+  const long k = m < c ? m : c;
+  const long need = (long)m * c + (long)m * k + k * c + k;
+  if (need <= cap) return QRBuf{arena, arena + (long)m * c, arena + (long)m * c + (long)m * k,
+                                 arena + (long)m * c + (long)m * k + k * c};
+  return QRBuf{D.qw, D.qq, D.rf, D.tau};
+}
+
 // v is (m x c) row-major (forward) or (c x m) row-major transposed
-// (backward).  Computes the Householder QR of the (m x c) view, writes the
-// explicit Q (m x k) into qq, the clean upper-triangular R (k x c) into rf.
-__device__ int qr_factor(const double* v, int m, int c, bool transposed, const SysDesc& D, SolveLDS& S) {
-  double* W = D.qw;
+// (backward).  Householder QR of the (m x c) view: explicit Q (m x k) into
+// B.Q, the clean upper-triangular R (k x c) into B.R.  Returns k.
+__device__ int qr_factor(const double* v, int m, int c, bool transposed, const QRBuf& B, SolveLDS& S) {
+  double* W = B.W;
   const long mc = (long)m * c;
   if (!transposed) {
     for (long e = threadIdx.x; e < mc; e += NT) W[e] = v[e];
@@ -75,12 +93,12 @@ __device__ int qr_factor(const double* v, int m, int c, bool transposed, const S
     }
   }
   __syncthreads();
-  wg_householder(W, m, c, D.tau, S.red);
-  wg_form_q(W, m, c, D.tau, D.qq);
+  wg_householder(W, m, c, B.tau, S.red);
+  wg_form_q(W, m, c, B.tau, B.Q);
   const int k = m < c ? m : c;
   for (long e = threadIdx.x; e < (long)k * c; e += NT) {
     const int i = (int)(e / c), j = (int)(e % c);
-    D.rf[e] = (j >= i) ? W[(long)i * c + j] : 0.0;
+    B.R[e] = (j >= i) ? W[(long)i * c + j] : 0.0;
   }
   __syncthreads();
   return k;
@@ -103,35 +121,115 @@ __device__ __forceinline__ double fl_qr(double m, double c) {
 
 // --- stacks ----------------------------------------------------------------
 
+// Interface-stack updates.  When the operands (previous stack, core,
+// operator core, rhs core), the two intermediates and the output fit the
+// arena, they are staged in LDS and contracted with ds_gemm; otherwise the
+// global-memory path (wg_gemm) runs.
+
 template <class ACC>
-__device__ double build_left(int i, const SysDesc& D, ACC* lds) {
+__device__ double build_left(int i, const SysDesc& D, ACC* lds, long cap) {
   // L_i from L_{i-1} and core i-1
   const int j = i - 1;
-  const int a = D.rx[j], n = D.n[j], b = D.rx[i];
-  const double fl = fl_op(a, D.ra[j], n, D.ra[i], b) + (D.kind == 0 ? fl_rhs(a, D.rb[j], n, D.rb[i], b) : 0.0);
-  stack_left_op<double, ACC>(a, D.ra[j], n, D.ra[i], b, D.Ls + D.Loff[j], D.ac[j], D.xc[j], D.Ls + D.Loff[i], D.t1,
-                             D.t2, lds);
-  if (D.kind == 0)
-    stack_left_rhs<double, ACC>(a, D.rb[j], n, D.rb[i], b, D.Lbs + D.Lboff[j], D.bc[j], D.xc[j],
-                                D.Lbs + D.Lboff[i], D.t1, lds);
+  const int a = D.rx[j], n = D.n[j], b = D.rx[i], P = D.ra[j], Q = D.ra[i];
+  const bool rhs = (D.kind == 0);
+  const int p = D.rb[j], q = D.rb[i];
+  const double fl = fl_op(a, P, n, Q, b) + (rhs ? fl_rhs(a, p, n, q, b) : 0.0);
+  const long mx = (long)(P > Q ? P : Q) > (long)(p > q ? p : q) ? (P > Q ? P : Q) : (p > q ? p : q);
+  const long need = (long)a * P * a + (long)a * n * b + (long)P * n * n * Q + 2L * a * n * b * mx + (long)b * Q * b +
+                    (rhs ? (long)p * a + (long)p * n * q + (long)q * b : 0);
+  if (need <= cap) {
+    double* Lc = reinterpret_cast<double*>(lds);
+    double* xc = Lc + (long)a * P * a;
+    double* Ac = xc + (long)a * n * b;
+    double* t1 = Ac + (long)P * n * n * Q;
+    double* t2 = t1 + (long)a * n * b * mx;
+    double* Lo = t2 + (long)a * n * b * mx;
+    double* Lb = Lo + (long)b * Q * b;
+    double* Bc = Lb + (long)p * a;
+    double* Lbo = Bc + (long)p * n * q;
+    wg_copy(D.Ls + D.Loff[j], Lc, a * P * a);
+    wg_copy(D.xc[j], xc, a * n * b);
+    wg_copy(D.ac[j], Ac, P * n * n * Q);
+    if (rhs) {
+      wg_copy(D.Lbs + D.Lboff[j], Lb, p * a);
+      wg_copy(D.bc[j], Bc, p * n * q);
+    }
+    __syncthreads();
+    // same three stages as stack_left_op, LDS to LDS
+    ds_gemm<double, double, true, true, true>(P * a, n * b, a, cview<double>(Lc, plain(1), plain(P * a)), crm<double>(xc, n * b),
+                            mview<double>(t1, split(a, n, b * P * n), split(b, 1, P * n)));
+    ds_gemm<double, double, true, true, true>(a * b, n * Q, P * n, crm<double>(t1, P * n),
+                            cview<double>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+                            mview<double>(t2, split(b, n, Q * a * n), split(Q, 1, a * n)));
+    ds_gemm<double, double, true, true, true>(b * Q, b, a * n, crm<double>(t2, a * n), crm<double>(xc, b), mrm<double>(Lo, b));
+    if (rhs) {
+      ds_gemm<double, double, true, true, true>(a, n * q, p, ctr<double>(Lb, a), crm<double>(Bc, n * q),
+                              mview<double>(t1, plain(n), split(q, 1, a * n)));
+      ds_gemm<double, double, true, true, true>(q, b, a * n, crm<double>(t1, a * n), crm<double>(xc, b), mrm<double>(Lbo, b));
+      wg_copy(Lbo, D.Lbs + D.Lboff[i], q * b);
+    }
+    wg_copy(Lo, D.Ls + D.Loff[i], b * Q * b);
+    __syncthreads();
+    return fl;
+  }
+  stack_left_op<double, ACC>(a, P, n, Q, b, D.Ls + D.Loff[j], D.ac[j], D.xc[j], D.Ls + D.Loff[i], D.t1, D.t2, lds);
+  if (rhs)
+    stack_left_rhs<double, ACC>(a, p, n, q, b, D.Lbs + D.Lboff[j], D.bc[j], D.xc[j], D.Lbs + D.Lboff[i], D.t1, lds);
   return fl;
 }
 
 template <class ACC>
-__device__ double build_right(int i, const SysDesc& D, ACC* lds) {
+__device__ double build_right(int i, const SysDesc& D, ACC* lds, long cap) {
   // R_i from R_{i+1} and core i+1
   const int j = i + 1;
-  const int a = D.rx[j], n = D.n[j], b = D.rx[j + 1];
-  const double fl = fl_op(a, D.ra[j], n, D.ra[j + 1], b) + (D.kind == 0 ? fl_rhs(a, D.rb[j], n, D.rb[j + 1], b) : 0.0);
-  stack_right_op<double, ACC>(a, D.ra[j], n, D.ra[j + 1], b, D.Rs + D.Roff[j], D.ac[j], D.xc[j],
-                              D.Rs + D.Roff[i], D.t1, D.t2, lds);
-  if (D.kind == 0)
-    stack_right_rhs<double, ACC>(a, D.rb[j], n, D.rb[j + 1], b, D.Rbs + D.Rboff[j], D.bc[j], D.xc[j],
-                                 D.Rbs + D.Rboff[i], D.t1, lds);
+  const int a = D.rx[j], n = D.n[j], b = D.rx[j + 1], P = D.ra[j], Q = D.ra[j + 1];
+  const bool rhs = (D.kind == 0);
+  const int p = D.rb[j], q = D.rb[j + 1];
+  const double fl = fl_op(a, P, n, Q, b) + (rhs ? fl_rhs(a, p, n, q, b) : 0.0);
+  const long mx = (long)(P > Q ? P : Q) > (long)(p > q ? p : q) ? (P > Q ? P : Q) : (p > q ? p : q);
+  const long need = (long)b * Q * b + (long)a * n * b + (long)P * n * n * Q + 2L * a * n * b * mx + (long)a * P * a +
+                    (rhs ? (long)q * b + (long)p * n * q + (long)p * a : 0);
+  if (need <= cap) {
+    double* Rc = reinterpret_cast<double*>(lds);
+    double* xc = Rc + (long)b * Q * b;
+    double* Ac = xc + (long)a * n * b;
+    double* t1 = Ac + (long)P * n * n * Q;
+    double* t2 = t1 + (long)a * n * b * mx;
+    double* Ro = t2 + (long)a * n * b * mx;
+    double* Rb = Ro + (long)a * P * a;
+    double* Bc = Rb + (long)q * b;
+    double* Rbo = Bc + (long)p * n * q;
+    wg_copy(D.Rs + D.Roff[j], Rc, b * Q * b);
+    wg_copy(D.xc[j], xc, a * n * b);
+    wg_copy(D.ac[j], Ac, P * n * n * Q);
+    if (rhs) {
+      wg_copy(D.Rbs + D.Rboff[j], Rb, q * b);
+      wg_copy(D.bc[j], Bc, p * n * q);
+    }
+    __syncthreads();
+    // same three stages as stack_right_op, LDS to LDS
+    ds_gemm<double, double, true, true, true>(a * n, b * Q, b, crm<double>(xc, b), ctr<double>(Rc, b),
+                            mview<double>(t1, split(n, b * n * Q, Q), split(Q, n * Q, 1)));
+    ds_gemm<double, double, true, true, true>(a * b, P * n, n * Q, crm<double>(t1, n * Q),
+                            cview<double>(Ac, split(Q, n * Q, 1), split(n, n * n * Q, Q)),
+                            mview<double>(t2, split(b, 1, P * a), split(n, a, b * P * a)));
+    ds_gemm<double, double, true, true, true>(a, P * a, n * b, crm<double>(xc, n * b), crm<double>(t2, P * a), mrm<double>(Ro, P * a));
+    if (rhs) {
+      ds_gemm<double, double, true, true, true>(a * n, q, b, crm<double>(xc, b), ctr<double>(Rb, b),
+                              mview<double>(t1, split(n, 1, q * a), plain(a)));
+      ds_gemm<double, double, true, true, true>(p, a, n * q, crm<double>(Bc, n * q), crm<double>(t1, a), mrm<double>(Rbo, a));
+      wg_copy(Rbo, D.Rbs + D.Rboff[i], p * a);
+    }
+    wg_copy(Ro, D.Rs + D.Roff[i], a * P * a);
+    __syncthreads();
+    return fl;
+  }
+  stack_right_op<double, ACC>(a, P, n, Q, b, D.Rs + D.Roff[j], D.ac[j], D.xc[j], D.Rs + D.Roff[i], D.t1, D.t2, lds);
+  if (rhs)
+    stack_right_rhs<double, ACC>(a, p, n, q, b, D.Rbs + D.Rboff[j], D.bc[j], D.xc[j], D.Rbs + D.Rboff[i], D.t1,
+                                 lds);
   return fl;
 }
-
-// --- one local step ----------------------------------------------------------
 
 struct StepStats {
   long kit = 0;        // Krylov iterations (matvecs)
@@ -148,7 +246,7 @@ __device__ __forceinline__ long arena_need(const SysDesc& D, int a, int P, int n
   const long N = (long)a * n * b;
   const long mx = P > Q ? P : Q;
   long need = (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + 2 * N * mx;
-  need += (D.kind == 0) ? 5 * N : 2 * N;
+  need += (D.kind == 0) ? 5 * N : 3 * N;
   return need;
 }
 
@@ -173,6 +271,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
                            D.t1, lds);
   }
   // working vectors: on-chip arena when the local problem fits, else global
+  const long long tsetup = ph_now();
   Work W;
   W.V = D.V;
   const bool onchip = !dense && arena_need(D, a, P, n, Q, b) <= arena;
@@ -202,17 +301,21 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     } else {
       W.cur = vec;
       W.w = vec + N;
+      W.r = vec + 2 * N;
     }
     __syncthreads();
   } else {
     W.f = D.f;
     W.x = D.g;
-    W.r = D.V;
+    // sle: CG vectors in the (then unused) basis area; evp: Lanczos' previous
+    // vector in the rhs buffer, which evp does not use
+    W.r = (D.kind == 0) ? D.V : D.f;
     W.p = D.V + N;
     W.q = D.V + 2 * (long)N;
     W.cur = D.qw;
     W.w = D.qq;
   }
+  ph_add(S, PH_SETUP, tsetup);
   if (D.kind == 0) {
     if (dense) {
       micro_dense<double>(a, P, n, Q, b, op.L, op.A, op.R, D.dense, N);
@@ -243,7 +346,10 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
-    const int r = wg_lanczos<ACC>(op, D.g, W, mode, D.sigma, D.tol, D.maxit, D.gm_restart, lds, S, lam, &its);
+    const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, D.tol, D.maxit, D.gm_restart, lds, S, lam,
+                                                 &its)
+                         : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, D.tol, D.maxit, D.gm_restart, lds, S,
+                                                  lam, &its);
     // matvecs + two Gram-Schmidt passes against on average half the basis
     st.flops += its * fmv + (double)its * 8.0 * N * 0.5 * (its < D.gm_restart ? its : D.gm_restart);
     if (r == 3) return ST_UNSUPPORTED;
@@ -265,53 +371,56 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     const double nv = wg_nrm2(D.g, N, S.red);
     if (!isfinite(nv)) return ST_NONFINITE;
   }
+  const long long tqr = ph_now();
   if (forward) {
     // QR of (a*n, b): core i = Q, warm start for i+1 = R * core_{i+1}
-    const int k = qr_factor(D.g, a * n, b, false, D, S);
-    wg_copy(D.qq, D.xc[i], a * n * k);
+    const QRBuf B = qr_buffers(D, a * n, b, reinterpret_cast<double*>(lds), arena);
+    const int k = qr_factor(D.g, a * n, b, false, B, S);
+    wg_copy(B.Q, D.xc[i], a * n * k);
     const int n1 = D.n[i + 1], c1 = D.rx[i + 2];
     st.flops += fl_qr(a * n, b) + 2.0 * k * n1 * c1 * b;
-    wg_gemm<double, ACC>(k, n1 * c1, b, crm<double>(D.rf, b), crm<double>(D.xc[i + 1], n1 * c1),
-                         mrm<double>(D.g, n1 * c1), lds);
+    ds_gemm<double, ACC>(k, n1 * c1, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], n1 * c1),
+                         mrm<double>(D.g, n1 * c1));
     if (threadIdx.x == 0) D.rx[i + 1] = k;
     __syncthreads();
   } else if (i > 0) {
     // QR of the transpose of (a, n*b): core i = Q^T, warm start for i-1 =
     // core_{i-1} * R^T
-    const int k = qr_factor(D.g, n * b, a, true, D, S);
+    const QRBuf B = qr_buffers(D, n * b, a, reinterpret_cast<double*>(lds), arena);
+    const int k = qr_factor(D.g, n * b, a, true, B, S);
     st.flops += fl_qr(n * b, a) + 2.0 * D.rx[i - 1] * D.n[i - 1] * k * a;
     // core i (k, n*b) = Q^T, Q is (n*b x k)
     for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
       const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
-      D.xc[i][e] = D.qq[(long)c * k + r];
+      D.xc[i][e] = B.Q[(long)c * k + r];
     }
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
     // g[(c0 n0), k] = core_{i-1}[(c0 n0), a] * R[k, a]^T
-    wg_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(D.rf, a), mrm<double>(D.g, k),
-                         lds);
+    ds_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
     if (threadIdx.x == 0) D.rx[i] = k;
     __syncthreads();
   } else {
     wg_copy(D.g, D.xc[0], N);
     __syncthreads();
   }
+  ph_add(S, PH_QR, tqr);
   return ST_OK;
 }
 
 // Right-orthonormalisation of the initial guess (sle only): for i = d-1..1,
 // core i = Q^T of the QR of its transpose, R^T pushed into core i-1.
 template <class ACC>
-__device__ void ortho_right_x(const SysDesc& D, ACC* lds, SolveLDS& S) {
+__device__ void ortho_right_x(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   for (int i = D.d - 1; i >= 1; --i) {
     const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
-    const int k = qr_factor(D.xc[i], n * b, a, true, D, S);
+    const QRBuf B = qr_buffers(D, n * b, a, reinterpret_cast<double*>(lds), arena);
+    const int k = qr_factor(D.xc[i], n * b, a, true, B, S);
     for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
       const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
-      D.xc[i][e] = D.qq[(long)c * k + r];
+      D.xc[i][e] = B.Q[(long)c * k + r];
     }
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
-    wg_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(D.rf, a), mrm<double>(D.g, k),
-                         lds);
+    ds_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
     wg_copy(D.g, D.xc[i - 1], c0 * n0 * k);
     if (threadIdx.x == 0) D.rx[i] = k;
     __syncthreads();
@@ -339,6 +448,9 @@ __device__ bool operator_symmetric(const SysDesc& D, SolveLDS& S) {
 template <class ACC>
 __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   const int d = D.d;
+  if (threadIdx.x < 8) S.prof[threadIdx.x] = 0;
+  __syncthreads();
+  const long long tstart = ph_now();
   StepStats st;
   int status = ST_OK;
   double lam = 0.0;
@@ -352,7 +464,7 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   bool sym = D.sym > 0;
   if (D.sym < 0) sym = operator_symmetric(D, S);
 
-  if (D.kind == 0 && D.ortho_x0) ortho_right_x<ACC>(D, lds, S);
+  if (D.kind == 0 && D.ortho_x0) ortho_right_x<ACC>(D, lds, arena, S);
 
   // boundary stacks
   set_one(D.Ls + D.Loff[0]);
@@ -362,7 +474,7 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     set_one(D.Rbs + D.Rboff[d - 1]);
   }
   __syncthreads();
-  for (int i = d - 2; i >= 0; --i) st.flops += build_right<ACC>(i, D, lds);
+  for (int i = d - 2; i >= 0; --i) st.flops += build_right<ACC>(i, D, lds, arena);
 
   bool converged = false;
   while (sweeps < D.repeats && !converged && status == ST_OK) {
@@ -370,14 +482,18 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     wg_copy(D.xc[0], D.g, D.rx[0] * D.n[0] * D.rx[1]);
     __syncthreads();
     for (int i = 0; i < d && status == ST_OK; ++i) {
-      if (i > 0) st.flops += build_left<ACC>(i, D, lds);
+      const long long ts = ph_now();
+      if (i > 0) st.flops += build_left<ACC>(i, D, lds, arena);
+      ph_add(S, PH_STACK, ts);
       if (i < d - 1) {
         status = local_step<ACC>(i, true, D, sym, lds, arena, S, &lam, st);
         ++nsolves;
       }
     }
     for (int i = d - 1; i >= 0 && status == ST_OK; --i) {
-      if (i < d - 1) st.flops += build_right<ACC>(i, D, lds);
+      const long long ts = ph_now();
+      if (i < d - 1) st.flops += build_right<ACC>(i, D, lds, arena);
+      ph_add(S, PH_STACK, ts);
       status = local_step<ACC>(i, false, D, sym, lds, arena, S, &lam, st);
       ++nsolves;
     }
@@ -416,6 +532,8 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     D.out[6] = (double)st.maxit_one;
     D.out[7] = st.worst;
     D.out[8] = st.flops;
+    D.out[9] = (double)(ph_now() - tstart);
+    for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
   }
 }
 

@@ -35,14 +35,23 @@ __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y,
 
 constexpr int KMAX = 40;  // Krylov basis size (GMRES restart / Lanczos)
 struct SolveLDS {
-  double red[NWAVE];
+  double red[4 * NWAVE];
   double h[KMAX + 2];
   double H[(KMAX + 1) * KMAX];
   double cs[KMAX + 1], sn[KMAX + 1], g[KMAX + 2];
   double alpha[KMAX + 1], beta[KMAX + 1], s[KMAX + 1];
   double scal[4];
+  double part[(KMAX + 2) * NWAVE];  // per-wave partial dot products
+  long long prof[8];                // cycle counters per phase (thread 0)
   int flag[4];
 };
+
+// phase cycle counters (thread 0 accumulates; read back as diagnostics)
+enum Phase { PH_MATVEC = 0, PH_ORTH, PH_TRI, PH_QR, PH_STACK, PH_SETUP, PH_OTHER };
+__device__ __forceinline__ long long ph_now() { return (long long)clock64(); }
+__device__ __forceinline__ void ph_add(SolveLDS& S, int ph, long long t0) {
+  if (threadIdx.x == 0) S.prof[ph] += ph_now() - t0;
+}
 
 // Working vectors of a local solve.  V (global): (KMAX+1) x N basis;
 // cur, w (N each) and, for CG, r, p, q, x, f: on-chip when they fit.
@@ -57,15 +66,42 @@ struct Work {
   double* f;
 };
 
-// h[i] = <V_i, w> for i < cnt, one wave per dot product.
-__device__ void wg_multidot(const double* V, long ldv, int cnt, const double* w, int N, double* h) {
+// h[i] = <V_i, w> for i < cnt (and h[cnt] = <w, w> if with_norm).  Each
+// wave owns a contiguous slice of the elements and accumulates 8 basis
+// vectors per pass over it (8 independent load streams per lane, so the L2
+// latency of the basis overlaps); per-wave partial sums meet in LDS.  Fixed
+// summation order: deterministic.
+__device__ void wg_multidot(const double* V, long ldv, int cnt, const double* w, int N, double* h, double* part,
+                            bool with_norm) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int i = wv; i < cnt; i += NWAVE) {
-    double s = 0.0;
-    const double* vi = V + (long)i * ldv;
-    for (int e = lane; e < N; e += 64) s += vi[e] * w[e];
-    s = wave_sum(s);
-    if (lane == 0) h[i] = s;
+  const int chunk = (N + NWAVE - 1) / NWAVE;
+  const int e0 = wv * chunk, e1 = (e0 + chunk < N) ? e0 + chunk : N;
+  const int tot = cnt + (with_norm ? 1 : 0);
+  for (int i0 = 0; i0 < tot; i0 += 8) {
+    double acc[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) acc[g] = 0.0;
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const double we = w[e];
+#pragma unroll
+      for (int g = 0; g < 8; ++g) {
+        const int i = i0 + g;
+        if (i < cnt) acc[g] += V[(long)i * ldv + e] * we;
+        else if (i == cnt && with_norm) acc[g] += we * we;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      const double sg = wave_sum(acc[g]);
+      if (lane == 0 && i0 + g < tot) part[(i0 + g) * NWAVE + wv] = sg;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += NT) {
+    double sg = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) sg += part[i * NWAVE + q];
+    h[i] = sg;
   }
   __syncthreads();
 }
@@ -74,29 +110,31 @@ __device__ void wg_multidot(const double* V, long ldv, int cnt, const double* w,
 __device__ __forceinline__ void wg_multiaxpy(const double* V, long ldv, int cnt, const double* h, double* w,
                                              int N) {
   for (int e = threadIdx.x; e < N; e += NT) {
-    double s = 0.0;
-    for (int i = 0; i < cnt; ++i) s += h[i] * V[(long)i * ldv + e];
-    w[e] -= s;
+    double sg = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < cnt; ++i) sg += h[i] * V[(long)i * ldv + e];
+    w[e] -= sg;
   }
 }
 
-// Iterated classical Gram-Schmidt with the DGKS criterion: another pass is
-// made while a pass shrinks w by more than a factor 0.7 (2..4 passes), so
-// that even a near-breakdown residual made of rounding noise leaves
+// Iterated classical Gram-Schmidt with the DGKS criterion: a second (third,
+// fourth) pass is made only while a pass shrinks w by more than a factor
+// 0.7, so that even a near-breakdown residual made of rounding noise leaves
 // orthogonal to the basis.  Accumulates the coefficients in hsum; returns
 // ||w|| after orthogonalisation and the norm before it in *w0.
 __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, double* h, double* hsum,
-                          double* red, double* w0) {
-  for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] = 0.0;
-  double prev = wg_nrm2u(w, N, red);
-  *w0 = prev;
-  double nrm = prev;
+                          double* part, double* red, double* w0) {
+  double prev = 0.0, nrm = 0.0;
   for (int pass = 0; pass < 4; ++pass) {
-    wg_multidot(V, ldv, cnt, w, N, h);
+    wg_multidot(V, ldv, cnt, w, N, h, part, pass == 0);
+    if (pass == 0) {
+      prev = sqrt(fmax(h[cnt], 0.0));
+      *w0 = prev;
+    }
     wg_multiaxpy(V, ldv, cnt, h, w, N);
-    for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] += h[i];
+    for (int i = threadIdx.x; i < cnt; i += NT) hsum[i] = (pass == 0 ? 0.0 : hsum[i]) + h[i];
     nrm = wg_nrm2u(w, N, red);  // barrier inside: w and hsum complete
-    if (pass >= 1 && nrm > 0.7 * prev) break;
+    if (nrm > 0.7 * prev) break;
     prev = nrm;
   }
   return nrm;
@@ -186,7 +224,7 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     for (int j = 0; j < m; ++j) {
       apply_op<double, ACC>(op, cur, w, lds);
       double w0;
-      double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.red, &w0);
+      double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.part, S.red, &w0);
       if (!(hn > 1e-14 * w0)) hn = 0.0;  // (lucky) breakdown: Krylov space invariant
       if (hn > 0.0) {
         const double ih = 1.0 / hn;
@@ -248,18 +286,46 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
   }
 }
 
+// Three workgroup sums with one pair of barriers.
+__device__ __forceinline__ void wg_sum3(double& a, double& b, double& c, double* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[wv] = a;
+    red[NWAVE + wv] = b;
+    red[2 * NWAVE + wv] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+#pragma unroll
+  for (int q = 0; q < NWAVE; ++q) {
+    a += red[q];
+    b += red[NWAVE + q];
+    c += red[2 * NWAVE + q];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Eigenvalues of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
 // Sturm-count multisection by one wave (64 shifts per round), then inverse
 // iteration for the eigenvector.
 
+// Sturm count #{eigenvalues < x} from the LDL^T pivots; the division by the
+// previous pivot uses the hardware reciprocal refined by two Newton steps
+// (full double precision, no IEEE division sequence).
 __device__ __forceinline__ int sturm_count(const double* al, const double* be, int m, double x) {
   int c = 0;
   double q = al[0] - x;
   if (q < 0.0) ++c;
   for (int i = 1; i < m; ++i) {
     const double qq = (q == 0.0) ? 1e-300 : q;
-    q = (al[i] - x) - be[i - 1] * be[i - 1] / qq;
+    double r = __builtin_amdgcn_rcp(qq);
+    r = r * (2.0 - qq * r);
+    r = r * (2.0 - qq * r);
+    q = (al[i] - x) - be[i - 1] * be[i - 1] * r;
     if (q < 0.0) ++c;
   }
   return c;
@@ -333,19 +399,29 @@ __device__ void tri_eigvec(const double* al, const double* be, int m, double th,
   __syncthreads();
 }
 
-// Restarted Lanczos with full (DGKS) re-orthogonalisation for one extreme
-// eigenpair of the symmetric local operator.
+// Restarted Lanczos for one extreme eigenpair of the symmetric local
+// operator.
 //   mode 0: lowest, 1: highest, 2: nearest sigma (lowest/highest chosen
 //   from the first Ritz spectrum; the choice is checked at the end).
-// v (global): start vector on entry, unit eigenvector on exit.
-// Returns 0 converged, 2 not converged, 3 sigma lies inside the spectrum of
-// a problem larger than the Krylov space (unsupported).
-template <class ACC>
+// v (global): start vector on entry, unit eigenvector on exit.  The three
+// working vectors (previous, current, next) rotate through W.r/W.cur/W.w
+// (on-chip when OC).  Each step re-orthogonalises locally against the two
+// previous vectors (two fused reductions); problems that fit the Krylov
+// space (N <= KMAX, where the nearest-sigma rule needs the exact spectrum)
+// and steps with severe cancellation (|w| < 1e-3 |A v|) use full DGKS
+// Gram-Schmidt against the basis V (global).  Breakdowns (an invariant
+// subspace, e.g. a symmetry sector) continue with a fresh orthogonal
+// direction.  Returns 0 converged, 2 not converged, 3 sigma inside the
+// spectrum of a problem larger than the Krylov space (unsupported).
+template <class ACC, bool OC>
 __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, int mode, double sigma, double tol,
                           int maxit, int kmax, ACC* lds, SolveLDS& S, double* lambda, int* iters) {
+  using M = Mem<OC>;
   const int N = op.size();
-  double *V = W.V, *cur = W.cur, *w = W.w;
+  double* V = W.V;
+  double *prv = W.r, *cur = W.cur, *w = W.w;
   if (kmax > KMAX) kmax = KMAX;
+  const bool full_orth = (N <= KMAX);
   double vn = wg_nrm2(v, N, S.red);
   if (!(vn > 0.0) || !isfinite(vn)) {
     for (int e = threadIdx.x; e < N; e += NT) v[e] = 1.0 + 0.1 * (double)((e * 2654435761u) % 97) / 97.0;
@@ -361,50 +437,109 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
     for (int e = threadIdx.x; e < N; e += NT) {
       const double t = v[e] * iv;
       V[e] = t;
-      cur[e] = t;
+      M::st(cur, e, t);
+      M::st(prv, e, 0.0);
     }
     __syncthreads();
     int m = 0;
     bool conv = false;
     bool full = false;
+    double beta_prev = 0.0;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
+      long long t0 = ph_now();
       apply_op<double, ACC>(op, cur, w, lds);
-      double w0;
-      double bn = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.red, &w0);
-      if (threadIdx.x == 0) {
-        S.alpha[j] = S.H[j];
-        S.beta[j] = 0.0;
+      ph_add(S, PH_MATVEC, t0);
+      t0 = ph_now();
+      double alpha, w0, bn;
+      bool dgks = full_orth;
+      if (!full_orth) {
+        // pass 1: alpha = <w, cur>, |w|^2
+        double a1 = 0.0, ww = 0.0, dummy = 0.0;
+        for (int e = threadIdx.x; e < N; e += NT) {
+          const double we = M::ld(w, e);
+          a1 += we * M::ld(cur, e);
+          ww += we * we;
+        }
+        wg_sum3(a1, ww, dummy, S.red);
+        w0 = sqrt(ww);
+        // pass 2 fused with the update: c1 = <w', cur>, c0 = <w', prv>, |w'|^2
+        double c1 = 0.0, c0 = 0.0, w2 = 0.0;
+        for (int e = threadIdx.x; e < N; e += NT) {
+          const double we = M::ld(w, e) - a1 * M::ld(cur, e) - beta_prev * M::ld(prv, e);
+          M::st(w, e, we);
+          c1 += we * M::ld(cur, e);
+          c0 += we * M::ld(prv, e);
+          w2 += we * we;
+        }
+        wg_sum3(c1, c0, w2, S.red);
+        for (int e = threadIdx.x; e < N; e += NT)
+          M::st(w, e, M::ld(w, e) - c1 * M::ld(cur, e) - c0 * M::ld(prv, e));
+        alpha = a1 + c1;
+        const double b2 = w2 - c1 * c1 - c0 * c0;
+        bn = sqrt(fmax(b2, 0.0));
+        if (!(bn > 1e-3 * w0)) {
+          dgks = true;  // severe cancellation: restore orthogonality to the whole basis
+          __syncthreads();
+        }
+        if (!dgks && threadIdx.x == 0) {
+          S.alpha[j] = alpha;
+          S.beta[j] = 0.0;
+        }
+      }
+      if (dgks) {
+        // w = A cur - (alpha cur + beta prv) (local pass already applied when !full_orth)
+        double h0;
+        const double nb = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.part, S.red, &h0);
+        if (full_orth) {
+          w0 = h0;
+          alpha = S.H[j];
+        } else {
+          alpha += S.H[j];
+        }
+        bn = nb;
+        if (threadIdx.x == 0) {
+          S.alpha[j] = alpha;
+          S.beta[j] = 0.0;
+        }
       }
       __syncthreads();
+      ph_add(S, PH_ORTH, t0);
       ++*iters;
       m = j + 1;
       double anorm = 0.0;
       for (int i = 0; i <= j; ++i)
         anorm = fmax(anorm, fabs(S.alpha[i]) + (i > 0 ? fabs(S.beta[i - 1]) : 0.0));
-      // breakdown: the residual is at the rounding level of M v_j
+      // breakdown: the residual is at the rounding level of A v_j
       const bool brk = !(bn > 1e-12 * fmax(w0, 1e-300));
       full = (m == N);
       if (brk && !full && m < kk) {
         // invariant subspace: continue with a fresh direction orthogonal to
         // the basis (T stays tridiagonal with a zero coupling)
         for (int e = threadIdx.x; e < N; e += NT)
-          w[e] = (double)((((unsigned)e + 1u) * 2654435761u + ((unsigned)j + 1u) * 40503u) % 1009u) / 1009.0 - 0.5;
+          M::st(w, e, (double)((((unsigned)e + 1u) * 2654435761u + ((unsigned)j + 1u) * 40503u) % 1009u) / 1009.0 -
+                          0.5);
         __syncthreads();
         double r0;
-        bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.red, &r0);
+        bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.part, S.red, &r0);
         const double ib = bn > 0.0 ? 1.0 / bn : 0.0;
+        double* t = prv;
+        prv = cur;
+        cur = w;
+        w = t;
         for (int e = threadIdx.x; e < N; e += NT) {
-          const double t = w[e] * ib;
-          cur[e] = t;
-          V[(long)(j + 1) * N + e] = t;
+          const double x = M::ld(cur, e) * ib;
+          M::st(cur, e, x);
+          V[(long)(j + 1) * N + e] = x;
         }
+        beta_prev = 0.0;
         __syncthreads();
         continue;
       }
       if (threadIdx.x == 0) S.beta[j] = brk ? 0.0 : bn;
       __syncthreads();
       const bool check = full || brk || (m == kk) || (m >= 2 && (m % 4) == 2) || *iters >= maxit;
+      const long long tc = ph_now();
       if (check) {
         if (!decided) {
           tri_eigval(S.alpha, S.beta, m, 0, S);
@@ -440,15 +575,21 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         th = S.scal[1];
         tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
         const double res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
+        ph_add(S, PH_TRI, tc);
         if (res <= tol * fmax(fabs(th), 1e-3 * anorm)) { conv = true; break; }
         if (*iters >= maxit) break;
       }
       const double ib = 1.0 / bn;
+      double* t = prv;
+      prv = cur;
+      cur = w;
+      w = t;
       for (int e = threadIdx.x; e < N; e += NT) {
-        const double t = w[e] * ib;
-        cur[e] = t;
-        V[(long)(j + 1) * N + e] = t;
+        const double x = M::ld(cur, e) * ib;
+        M::st(cur, e, x);
+        V[(long)(j + 1) * N + e] = x;
       }
+      beta_prev = bn;
       __syncthreads();
     }
     // Ritz vector

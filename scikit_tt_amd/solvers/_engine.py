@@ -12,7 +12,7 @@ class SolveInfo:
     """Per-system engine report: status, sweeps, Krylov iterations, eigenvalue."""
 
     __slots__ = ('status', 'sweeps', 'unconverged', 'max_iters', 'krylov_iters', 'local_solves', 'eigenvalue',
-                 'worst_residual', 'flops')
+                 'worst_residual', 'flops', 'cycles')
 
     def __init__(self, r):
         self.status = int(r.status)
@@ -24,6 +24,8 @@ class SolveInfo:
         self.eigenvalue = float(r.lambda_)
         self.worst_residual = float(r.worst_residual)
         self.flops = float(r.flops)
+        names = ('total', 'matvec', 'gram_schmidt', 'tridiag', 'qr', 'stacks', 'setup')
+        self.cycles = {k: float(v) for k, v in zip(names, r.cycles)}
 
     def __repr__(self):
         return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in self.__slots__)

@@ -247,18 +247,17 @@ class TT:
     def ortho_right(self, start_index=None, end_index=1, threshold=0, max_rank=np.inf):
         """Right-orthonormalisation (tensor_train.py:1207-1310), in place.
 
-        Without truncation (threshold 0, max_rank inf) the reference keeps
-        all min(r_i, m_i n_i r_{i+1}) singular triplets, i.e. the same ranks
-        and the same tensor as this Householder-QR sweep (only the gauge of
-        the orthonormal cores differs).
+        Without truncation the device runs a Householder-QR sweep: the
+        reference keeps all min(r_i, m_i n_i r_{i+1}) thin-SVD triplets, i.e.
+        the same ranks and tensor (only the orthonormal gauge differs).  With
+        ``threshold > 0`` or a finite ``max_rank`` it adds a Jacobi SVD of
+        the R factor and keeps s/s0 > threshold, then at most max_rank.
         """
         if start_index is None:
             start_index = self.order - 1
-        _check_ortho_args(start_index, end_index, threshold, max_rank)
-        if threshold != 0 or max_rank != np.inf:
-            raise NotImplementedError('truncated ortho_right (threshold/max_rank) is not implemented')
+        caps = _check_ortho_args(self.order, start_index, end_index, threshold, max_rank)
         if start_index >= end_index:
-            self._ortho_sweep(1, int(start_index), int(end_index))
+            self._ortho_sweep(1, int(start_index), int(end_index), float(threshold), caps)
         return self
 
     def ortho_left(self, start_index=0, end_index=None, threshold=0.0, max_rank=np.inf, progress=False,
@@ -266,18 +265,17 @@ class TT:
         """Left-orthonormalisation (tensor_train.py:1092-1205), in place."""
         if end_index is None:
             end_index = self.order - 2
-        _check_ortho_args(start_index, end_index, threshold, max_rank)
-        if threshold != 0 or max_rank != np.inf:
-            raise NotImplementedError('truncated ortho_left (threshold/max_rank) is not implemented')
+        caps = _check_ortho_args(self.order, start_index, end_index, threshold, max_rank)
         if end_index >= start_index:
-            self._ortho_sweep(0, int(start_index), int(end_index))
+            self._ortho_sweep(0, int(start_index), int(end_index), float(threshold), caps)
         return self
 
-    def _ortho_sweep(self, direction, start, end):
+    def _ortho_sweep(self, direction, start, end, threshold, caps):
         ctx = _lib.context(self.device.index)
         pack = self._desc()
+        cap_arr = None if caps is None else (ctypes.c_int * (self.order + 1))(*caps)
         _lib.check(_lib.load().ttals_tt_ortho(ctx, _stream(self.device), ctypes.byref(pack.desc), direction, start,
-                                              end), ctx, 'tt_ortho')
+                                              end, threshold, cap_arr), ctx, 'tt_ortho')
         ranks = pack.final_ranks()
         for i in range(self.order):
             c = self.device_cores[i]
@@ -313,14 +311,23 @@ class TT:
 # helpers
 
 
-def _check_ortho_args(start_index, end_index, threshold, max_rank):
+def _check_ortho_args(order, start_index, end_index, threshold, max_rank):
+    """Argument checks of ortho_left/right (tensor_train.py:1137-1159); returns the
+    per-bond rank caps for the device (0 = unbounded) or None."""
     if not isinstance(start_index, (int, np.integer)) or not isinstance(end_index, (int, np.integer)):
         raise TypeError('Start and end indices must be integers.')
     if not isinstance(threshold, (int, float, np.floating, np.integer)) or threshold < 0:
         raise ValueError('Threshold must be greater or equal 0.')
-    if not ((isinstance(max_rank, (int, np.integer)) and max_rank > 0) or max_rank == np.inf or
-            isinstance(max_rank, list)):
+    if (isinstance(max_rank, (int, np.integer)) and max_rank > 0) or max_rank == np.inf:
+        caps = [1] + [max_rank] * (order - 1) + [1]
+    elif isinstance(max_rank, list) and len(max_rank) == order + 1 and all(
+            (isinstance(r, (int, np.integer)) and r > 0) or r == np.inf for r in max_rank):
+        caps = list(max_rank)
+    else:
         raise ValueError('Maximum rank(s) must be positive integers.')
+    if all(c == np.inf for c in caps[1:-1]):
+        return None
+    return [0 if c == np.inf else int(c) for c in caps]
 
 
 def _embed_sum(x, y, sign):

@@ -78,7 +78,8 @@ def test_stacks_matvec_micro(env, a, P, n, Q, b):
     N = a * n * b
     v = rng.standard_normal(N)
     y = empty(N)
-    assert lib.ttals_micro_matvec(ctx, None, a, P, n, Q, b, ptr(dL), ptr(dA), ptr(dR), ptr(dev(v)), ptr(y)) == 0
+    dv = dev(v)
+    assert lib.ttals_micro_matvec(ctx, None, a, P, n, Q, b, ptr(dL), ptr(dA), ptr(dR), ptr(dv), ptr(y)) == 0
     ref = sle_als.micro_matvec(L, Ac, R, v)
     np.testing.assert_allclose(y.cpu().numpy(), ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
     if N <= 600:
@@ -114,7 +115,8 @@ def test_local_solve(env, a, P, n, Q, b, method):
     f = rng.standard_normal(N)
     v = torch.zeros(N, dtype=torch.float64, device='cuda')
     its = ctypes.c_int()
-    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(dev(f)), ptr(v),
+    keep = [dev(L), dev(Ac), dev(R), dev(f)]  # device copies must outlive the call
+    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], ptr(v),
                                method, 1 if method == 2 else 0, 1e-12, 4000, ctypes.byref(its))
     assert rc == 0
     x = v.cpu().numpy()
@@ -126,9 +128,9 @@ def test_local_lu_singular_detected(env):
     lib, ctx, _lib = env
     a, P, n, Q, b = 2, 1, 3, 1, 2
     Ac = np.zeros((P, n, n, Q))
-    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, ptr(dev(np.ones((a, P, a)))), ptr(dev(Ac)),
-                               ptr(dev(np.ones((b, Q, b)))), ptr(dev(np.ones(a * n * b))),
-                               ptr(torch.zeros(a * n * b, dtype=torch.float64, device='cuda')), 1, 0, 1e-12, 10, None)
+    keep = [dev(np.ones((a, P, a))), dev(Ac), dev(np.ones((b, Q, b))), dev(np.ones(a * n * b)),
+            torch.zeros(a * n * b, dtype=torch.float64, device='cuda')]
+    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], 1, 0, 1e-12, 10, None)
     assert rc == _lib.S_SINGULAR
 
 
@@ -144,7 +146,8 @@ def test_local_eig(env, N_shape, mode):
     sigma = {0: 0.0, 1: 0.0, 2: w[0] - 100.0}[mode]
     v = dev(rng.standard_normal(M.shape[0]))
     lam, its = ctypes.c_double(), ctypes.c_int()
-    rc = lib.ttals_local_eig(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(v), mode, sigma,
+    keep = [dev(L), dev(Ac), dev(R)]
+    rc = lib.ttals_local_eig(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], ptr(v), mode, sigma,
                              1e-12, 4000, ctypes.byref(lam), ctypes.byref(its))
     assert rc == 0
     target = w[-1] if mode == 1 else w[0]
@@ -165,7 +168,8 @@ def test_local_eig_nearest_interior_small(env):
     sigma = 0.5 * (w[7] + w[8]) + 0.1 * (w[8] - w[7])
     v = dev(rng.standard_normal(M.shape[0]))
     lam = ctypes.c_double()
-    assert lib.ttals_local_eig(ctx, None, a, P, n, Q, b, ptr(dev(L)), ptr(dev(Ac)), ptr(dev(R)), ptr(v), 2, sigma,
+    keep = [dev(L), dev(Ac), dev(R)]
+    assert lib.ttals_local_eig(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], ptr(v), 2, sigma,
                                1e-12, 4000, ctypes.byref(lam), None) == 0
     assert abs(lam.value - w[np.argmin(np.abs(w - sigma))]) < 1e-9 * np.abs(w).max()
 
@@ -176,7 +180,8 @@ def test_qr(env, m, c):
     Y = np.random.default_rng(m * c).standard_normal((m, c))
     k = min(m, c)
     q, r = empty(m, k), empty(k, c)
-    assert lib.ttals_qr(ctx, None, m, c, ptr(dev(Y)), ptr(q), ptr(r)) == 0
+    dY = dev(Y)
+    assert lib.ttals_qr(ctx, None, m, c, ptr(dY), ptr(q), ptr(r)) == 0
     Q, Rr = q.cpu().numpy(), r.cpu().numpy()
     np.testing.assert_allclose(Q @ Rr, Y, atol=1e-13 * np.abs(Y).max() * max(m, c))
     np.testing.assert_allclose(Q.T @ Q, np.eye(k), atol=1e-13 * max(m, c))
@@ -230,7 +235,8 @@ def test_tt_algebra(env):
     with pytest.raises(TypeError):
         TT('cores')
     assert tt.eye([2, 3]).matricize().shape == (6, 6)
-    np.testing.assert_allclose(tt.ones([2, 3], [1, 1], ranks=2).matricize(), 4 * np.ones(6))
+    # ranks [1, 2, 1]: every entry is the sum over the 2 rank paths of 1 * 1
+    np.testing.assert_allclose(tt.ones([2, 3], [1, 1], ranks=2).matricize(), 2 * np.ones(6))
 
 
 # --------------------------------------------------------------------------
@@ -345,3 +351,47 @@ def test_sle_errors(env):
         evp.als(_tt(A), _tt(x0), operator_gevp=_tt(A))
     with pytest.raises(NotImplementedError):
         evp.als(_tt(A), _tt(x0), solver='eigs')
+
+
+def _gapped_tt(rng, dims, rank, decay):
+    """A TT whose bond spectra have clear gaps: core_i = Q_i diag(decay) ..."""
+    cores = []
+    r0 = 1
+    for i, n in enumerate(dims):
+        r1 = 1 if i == len(dims) - 1 else rank
+        c = rng.standard_normal((r0, n, 1, r1))
+        if i < len(dims) - 1:
+            c = c * np.asarray(decay)[None, None, None, :r1]
+        cores.append(c)
+        r0 = r1
+    return cores
+
+
+@pytest.mark.parametrize('thr,maxr', [(1e-6, np.inf), (1e-3, np.inf), (1e-9, 3), (0, 2), (1e-6, [1, 4, 3, 5, 2, 1])])
+def test_truncated_ortho_ranks_and_tensor(env, thr, maxr):
+    """ortho with threshold/max_rank: identical ranks to the reference rule, same truncated tensor."""
+    from scikit_tt_amd.tensor_train import TT
+    rng = np.random.default_rng(21)
+    decay = [1.0, 0.3, 1e-2, 1e-5, 1e-8, 1e-12]
+    x = _gapped_tt(rng, [3, 4, 3, 4, 3], 6, decay)
+    ref = tt_ops.ortho(x, threshold=thr, max_rank=maxr)
+    t = TT(x).ortho(threshold=thr, max_rank=maxr)
+    assert t.ranks == tt_ops.ranks_of(ref)
+    np.testing.assert_allclose(t.matricize(), tt_ops.matricize(ref), rtol=1e-9,
+                               atol=1e-9 * np.abs(tt_ops.matricize(ref)).max())
+    for c in t.cores[1:]:
+        m = c.reshape(c.shape[0], -1)
+        np.testing.assert_allclose(m @ m.T, np.eye(m.shape[0]), atol=1e-12)
+    # the constructor's truncation path (tensor_train.py:190-192)
+    t2 = TT(x, threshold=thr, max_rank=maxr)
+    assert t2.ranks == t.ranks
+
+
+def test_ortho_left_truncated(env):
+    from scikit_tt_amd.tensor_train import TT
+    rng = np.random.default_rng(22)
+    x = _gapped_tt(rng, [2, 5, 3, 4], 5, [1.0, 1e-1, 1e-4, 1e-9, 1e-14])
+    ref = tt_ops.ortho_left(x, threshold=1e-7)
+    t = TT(x).ortho_left(threshold=1e-7)
+    assert t.ranks == tt_ops.ranks_of(ref)
+    np.testing.assert_allclose(t.matricize(), tt_ops.matricize(ref), rtol=1e-8, atol=1e-10)
