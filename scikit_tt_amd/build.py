@@ -1,5 +1,6 @@
 """Build libttals.so in-tree for gfx950 (``python -m scikit_tt_amd.build``)."""
 import os
+import re
 import subprocess
 import sys
 
@@ -12,9 +13,24 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 
+def _resource_usage(text):
+    """Per-kernel {field: value} from hipcc's kernel-resource-usage remarks."""
+    usage, name = {}, None
+    for line in text.splitlines():
+        m = re.search(r'remark:\s+Function Name: (\S+)', line)
+        if m:
+            name = m.group(1)
+            usage[name] = {}
+            continue
+        m = re.search(r'remark:\s+([A-Za-z ]+?)(?: \[[^]]*\])?: (\d+)', line)
+        if m and name:
+            usage[name][m.group(1)] = int(m.group(2))
+    return usage
+
+
 def build(verbose=False):
     tmp = OUT + '.tmp'
-    cmd = [HIPCC] + FLAGS + ['-o', tmp, SRC]
+    cmd = [HIPCC] + FLAGS + ['-Rpass-analysis=kernel-resource-usage', '-o', tmp, SRC]
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -22,6 +38,11 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError('hipcc failed building libttals.so')
     os.replace(tmp, OUT)
+    # the batched engine is sized for two resident systems per CU
+    for name, u in _resource_usage(r.stderr).items():
+        if name.startswith('_Z5k_als') and u.get('Occupancy', 2) < 2:
+            sys.stderr.write('warning: %s occupancy %d wave/SIMD (VGPRs %s, AGPRs %s): one system per CU\n'
+                             % (name, u['Occupancy'], u.get('VGPRs'), u.get('AGPRs')))
     return OUT
 
 

@@ -22,7 +22,10 @@ using namespace ttals;
 // GEMM staging tiles) holds either the staging tiles or the on-chip copy of
 // a local problem.
 template <class ACC>
-__global__ __launch_bounds__(NT, 2) void k_als(const SysDesc* descs, long arena) {
+#ifndef TTALS_ALS_WAVES_PER_EU
+#define TTALS_ALS_WAVES_PER_EU 2  // two resident systems per CU
+#endif
+__global__ __launch_bounds__(NT, TTALS_ALS_WAVES_PER_EU) void k_als(const SysDesc* descs, long arena) {
   extern __shared__ double dyn_lds[];
   __shared__ SolveLDS S;
   als_solve<ACC>(descs[blockIdx.x], reinterpret_cast<ACC*>(dyn_lds), arena, S);
@@ -94,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         W.q = A.V + 2 * (long)N;
         W.cur = A.V + (long)(KMAX + 1) * N;
         W.w = A.V + (long)(KMAX + 2) * N;
-        if (method == 2) r = wg_cg<double>(op, W, A.tol, A.maxit, lds, S, &its);
+        if (method == 2) r = wg_cg<double, false>(op, W, A.tol, A.maxit, lds, S, &its);
         if (method == 3 || r == 1) r = wg_gmres<double>(op, W, A.tol, A.maxit, KMAX, lds, S, &its);
       }
       if (threadIdx.x == 0) {
@@ -360,7 +363,9 @@ struct ttals_ctx {
   std::vector<Pending> pending;
   double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
   long trace_cap = 0;
-  long arena = 6144;         // on-chip arena of k_als in doubles (48 KB)
+  // on-chip arena of k_als in doubles: 62 KB, which with the ~17 KB of static
+  // LDS keeps two systems resident per CU (160 KB of LDS)
+  long arena = 7936;
   std::string err;
 };
 

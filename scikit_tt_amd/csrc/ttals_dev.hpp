@@ -11,6 +11,9 @@
 namespace ttals {
 
 constexpr int NT = 256;            // threads per workgroup: 4 x wave64
+#ifndef TTALS_MFMA_MATVEC
+#define TTALS_MFMA_MATVEC 1        // on-chip local matvec on the FP64 matrix cores
+#endif
 constexpr int NWAVE = NT / 64;
 
 // ---------------------------------------------------------------------------
@@ -101,6 +104,28 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
 #pragma unroll
   for (int i = 0; i < NWAVE; ++i) s += red[i];
   return s;
+}
+
+// Three workgroup sums with one pair of barriers.
+__device__ __forceinline__ void wg_sum3(double& a, double& b, double& c, double* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[wv] = a;
+    red[NWAVE + wv] = b;
+    red[2 * NWAVE + wv] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+#pragma unroll
+  for (int q = 0; q < NWAVE; ++q) {
+    a += red[q];
+    b += red[NWAVE + q];
+    c += red[2 * NWAVE + q];
+  }
 }
 
 template <class T>
@@ -276,6 +301,80 @@ __device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C)
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// FP64 matrix-core GEMM with every operand in LDS.
+//
+// v_mfma_f64_16x16x4f64 operand layout (wave64): lane l supplies
+// A[i0 + l%16][k + l/16] and B[k + l/16][j0 + l%16]; element e of its
+// 4-wide accumulator is C[i0 + l/16 + 4*e][j0 + l%16] (checked on gfx950).  Each lane loads
+// one A and one B value per 16 FMAs, against four loads per four FMAs for a
+// 2x2 register tile: the local matvec is LDS-bound, so this is what counts.
+
+typedef double dx4_t __attribute__((ext_vector_type(4)));
+
+// offsets of k, k+4, k+8, ... along an IMap, advanced without division
+struct KStep4 {
+  int off, r, div, dr, step, wrap;
+  __device__ __forceinline__ void start(const IMap& m, int k) {
+    div = m.div;
+    const int q = k / div;
+    r = k - q * div;
+    off = q * m.hi + r * m.lo;
+    const int dq = 4 / div;
+    dr = 4 - dq * div;
+    step = dq * m.hi + dr * m.lo;
+    wrap = (int)((long)m.hi - (long)div * m.lo);
+  }
+  __device__ __forceinline__ void next() {
+    r += dr;
+    off += step;
+    if (r >= div) {
+      r -= div;
+      off += wrap;
+    }
+  }
+};
+
+// C[M,N] = A[M,K] B[K,N]; 16x16 output tiles round-robin over the waves.
+// Indices past M, N or K contribute zeros and are not stored.
+__device__ void mf_gemm_lds(int M, int N, int K, CView<double> A, CView<double> B, MView<double> C) {
+  const TT_LDS double* pa = (const TT_LDS double*)A.p;
+  const TT_LDS double* pb = (const TT_LDS double*)B.p;
+  TT_LDS double* pc = (TT_LDS double*)C.p;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tn = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tn;
+  for (int t = w; t < tiles; t += NWAVE) {
+    const int ti = t / tn, tj = t - ti * tn;
+    const int i = (ti << 4) + li, j = (tj << 4) + li;
+    const bool iv = i < M, jv = j < N;
+    const int ar = iv ? (int)imap(A.r, i) : 0;
+    const int bc = jv ? (int)imap(B.c, j) : 0;
+    KStep4 ka, kb;
+    ka.start(A.c, lk);
+    kb.start(B.r, lk);
+    dx4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const bool kv = k0 + lk < K;
+      double av = 0.0, bv = 0.0;
+      if (iv && kv) av = pa[ar + ka.off];
+      if (jv && kv) bv = pb[kb.off + bc];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      ka.next();
+      kb.next();
+    }
+    if (jv) {
+      const int cc = (int)imap(C.c, j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = (ti << 4) + lk + 4 * e;
+        if (row < M) pc[(int)imap(C.r, row) + cc] = acc[e];
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // unscaled 2-norm (for vectors of moderate magnitude in the hot loops)
 template <class T>
 __device__ __forceinline__ double wg_nrm2u(const T* x, int n, double* red) {
@@ -317,18 +416,41 @@ __device__ void local_matvec(int a, int P, int n, int Q, int b, const T* L, cons
   wg_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b), lds);
 }
 
-// Same contraction with every operand on-chip (ds_gemm stages).
-// (all operands, intermediates, v and y in LDS)
+// Same contraction with every operand on-chip (ds_gemm stages).  The two
+// intermediates get odd row strides (P*n | 1, b*Q | 1 doubles) so rows read
+// by neighbouring threads fall into different LDS banks; their sizes are
+// a*b*(P*n+1) and a*n*(b*Q+1) (ds_t1_size / ds_t2_size).
+__host__ __device__ __forceinline__ int odd_stride(int s) { return s | 1; }
+__host__ __device__ __forceinline__ long ds_t1_size(int a, int P, int n, int b) {
+  return (long)a * b * odd_stride(P * n);
+}
+__host__ __device__ __forceinline__ long ds_t2_size(int a, int n, int Q, int b) {
+  return (long)a * n * odd_stride(b * Q);
+}
 template <class T, class ACC>
 __device__ void local_matvec_ds(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R,
                                 const T* v, T* y, T* t1, T* t2) {
   const int a2 = a;
+  const int s1 = odd_stride(P * n), s2 = odd_stride(b * Q);
+#if TTALS_MFMA_MATVEC
+  static_assert(sizeof(T) == 8, "matrix-core matvec is FP64");
+  // T1[a', b | P, n] (row stride s1)
+  mf_gemm_lds(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
+              mview<T>(t1, split(a2, n, b * s1), split(b, 1, s1)));
+  // T2[a', m | b, Q] (row stride s2)
+  mf_gemm_lds(a * b, n * Q, P * n, crm<T>(t1, s1), cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
+              mview<T>(t2, split(b, n * s2, Q), split(Q, s2, 1)));
+  mf_gemm_lds(a * n, b, b * Q, crm<T>(t2, s2), crm<T>(R, b), mrm<T>(y, b));
+#else
+  // T1[a', b | P, n] (row stride s1)
   ds_gemm<T, ACC, true, true, true>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
-                                    mview<T>(t1, split(a2, n, b * P * n), split(b, 1, P * n)));
-  ds_gemm<T, ACC, true, true, true>(a * b, n * Q, P * n, crm<T>(t1, P * n),
+                                    mview<T>(t1, split(a2, n, b * s1), split(b, 1, s1)));
+  // T2[a', m | b, Q] (row stride s2)
+  ds_gemm<T, ACC, true, true, true>(a * b, n * Q, P * n, crm<T>(t1, s1),
                                     cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
-                                    mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)));
-  ds_gemm<T, ACC, true, true, true>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b));
+                                    mview<T>(t2, split(b, n * s2, Q), split(Q, s2, 1)));
+  ds_gemm<T, ACC, true, true, true>(a * n, b, b * Q, crm<T>(t2, s2), crm<T>(R, b), mrm<T>(y, b));
+#endif
 }
 
 // Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']

@@ -244,8 +244,8 @@ struct StepStats {
 // Doubles of on-chip arena a local problem needs.
 __device__ __forceinline__ long arena_need(const SysDesc& D, int a, int P, int n, int Q, int b) {
   const long N = (long)a * n * b;
-  const long mx = P > Q ? P : Q;
-  long need = (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + 2 * N * mx;
+  long need = (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + ds_t1_size(a, P, n, b) +
+              ds_t2_size(a, n, Q, b);
   need += (D.kind == 0) ? 5 * N : 3 * N;
   return need;
 }
@@ -277,13 +277,12 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   const bool onchip = !dense && arena_need(D, a, P, n, Q, b) <= arena;
   if (onchip) {
     double* ar = reinterpret_cast<double*>(lds);
-    const long mx = P > Q ? P : Q;
     double* Lc = ar;
     double* Rc = Lc + (long)a * P * a;
     double* Ac = Rc + (long)b * Q * b;
     double* t1 = Ac + (long)P * n * n * Q;
-    double* t2 = t1 + (long)N * mx;
-    double* vec = t2 + (long)N * mx;
+    double* t2 = t1 + ds_t1_size(a, P, n, b);
+    double* vec = t2 + ds_t2_size(a, n, Q, b);
     wg_copy(op.L, Lc, a * P * a);
     wg_copy(op.R, Rc, b * Q * b);
     wg_copy(op.A, Ac, P * n * n * Q);
@@ -326,7 +325,8 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       st.flops += 3.0 * (double)N * N * P * Q + (2.0 / 3.0) * (double)N * N * N + 2.0 * (double)N * N;
     } else {
       int r = 1;
-      if (sym) r = wg_cg<ACC>(op, W, D.tol, D.maxit, lds, S, &its);
+      if (sym) r = onchip ? wg_cg<ACC, true>(op, W, D.tol, D.maxit, lds, S, &its)
+                          : wg_cg<ACC, false>(op, W, D.tol, D.maxit, lds, S, &its);
       if (r == 1) r = wg_gmres<ACC>(op, W, D.tol, D.maxit, D.gm_restart, lds, S, &its);
       if (r != 0) ++st.nunconv;
       // final true relative residual (diagnostic)

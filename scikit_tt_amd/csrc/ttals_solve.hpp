@@ -141,45 +141,71 @@ __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, 
 }
 
 // ---------------------------------------------------------------------------
-// Conjugate gradients on W.x (initial guess in W.x).  Returns 0 on
-// convergence, 1 if p^T M p <= 0 was met (not SPD: caller switches to
-// GMRES), 2 if maxit was reached.
-template <class ACC>
+// Conjugate gradients on W.x (initial guess in W.x); vectors x, r, p, q, f
+// on-chip when OC.  Returns 0 on convergence, 1 if p^T M p <= 0 was met (not
+// SPD: caller switches to GMRES), 2 if maxit was reached.
+template <class ACC, bool OC>
 __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int maxit, ACC* lds, SolveLDS& S,
                      int* iters) {
+  using M = Mem<OC>;
   const int N = op.size();
   double *x = W.x, *r = W.r, *p = W.p, *q = W.q;
   const double* f = W.f;
-  const double fn = wg_nrm2(f, N, S.red);
+  double fn = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int e = threadIdx.x; e < N; e += NT) {
+    const double fe = M::ld(f, e);
+    fn += fe * fe;
+  }
+  wg_sum3(fn, d1, d2, S.red);
+  fn = sqrt(fn);
   if (fn == 0.0) {
-    wg_fill(x, 0.0, N);
+    for (int e = threadIdx.x; e < N; e += NT) M::st(x, e, 0.0);
     __syncthreads();
     return 0;
   }
   for (int restart = 0; restart < 3; ++restart) {
     apply_op<double, ACC>(op, x, q, lds);
-    for (int e = threadIdx.x; e < N; e += NT) r[e] = f[e] - q[e], p[e] = r[e];
-    double rr = wg_dot(r, r, N, S.red);
+    double rr = 0.0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+      const double re = M::ld(f, e) - M::ld(q, e);
+      M::st(r, e, re);
+      M::st(p, e, re);
+      rr += re * re;
+    }
+    wg_sum3(rr, d1, d2, S.red);
     if (sqrt(rr) <= tol * fn) return 0;
     while (*iters < maxit) {
       apply_op<double, ACC>(op, p, q, lds);
-      const double pq = wg_dot(p, q, N, S.red);
+      double pq = 0.0;
+      for (int e = threadIdx.x; e < N; e += NT) pq += M::ld(p, e) * M::ld(q, e);
+      wg_sum3(pq, d1, d2, S.red);
       if (!(pq > 0.0)) return 1;
       const double al = rr / pq;
-      for (int e = threadIdx.x; e < N; e += NT) x[e] += al * p[e], r[e] -= al * q[e];
-      const double rn = wg_dot(r, r, N, S.red);
+      double rn = 0.0;
+      for (int e = threadIdx.x; e < N; e += NT) {
+        M::st(x, e, M::ld(x, e) + al * M::ld(p, e));
+        const double re = M::ld(r, e) - al * M::ld(q, e);
+        M::st(r, e, re);
+        rn += re * re;
+      }
+      wg_sum3(rn, d1, d2, S.red);
       ++*iters;
       if (sqrt(rn) <= tol * fn) break;
       const double be = rn / rr;
       rr = rn;
-      for (int e = threadIdx.x; e < N; e += NT) p[e] = r[e] + be * p[e];
+      for (int e = threadIdx.x; e < N; e += NT) M::st(p, e, M::ld(r, e) + be * M::ld(p, e));
       __syncthreads();
     }
     // true residual check (the recurrence drifts)
     apply_op<double, ACC>(op, x, q, lds);
-    for (int e = threadIdx.x; e < N; e += NT) r[e] = f[e] - q[e];
-    const double tr = wg_nrm2(r, N, S.red);
-    if (tr <= 10.0 * tol * fn) return 0;
+    double tr = 0.0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+      const double re = M::ld(f, e) - M::ld(q, e);
+      M::st(r, e, re);
+      tr += re * re;
+    }
+    wg_sum3(tr, d1, d2, S.red);
+    if (sqrt(tr) <= 10.0 * tol * fn) return 0;
     if (*iters >= maxit) return 2;
   }
   return 2;
@@ -286,28 +312,6 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
   }
 }
 
-// Three workgroup sums with one pair of barriers.
-__device__ __forceinline__ void wg_sum3(double& a, double& b, double& c, double* red) {
-  a = wave_sum(a);
-  b = wave_sum(b);
-  c = wave_sum(c);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) {
-    red[wv] = a;
-    red[NWAVE + wv] = b;
-    red[2 * NWAVE + wv] = c;
-  }
-  __syncthreads();
-  a = b = c = 0.0;
-#pragma unroll
-  for (int q = 0; q < NWAVE; ++q) {
-    a += red[q];
-    b += red[NWAVE + q];
-    c += red[2 * NWAVE + q];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Eigenvalues of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
 // Sturm-count multisection by one wave (64 shifts per round), then inverse
@@ -333,7 +337,10 @@ __device__ __forceinline__ int sturm_count(const double* al, const double* be, i
 
 // eigenvalue with index idx (0 = smallest) of T; executed by wave 0, result
 // broadcast through S.scal[1].
-__device__ void tri_eigval(const double* al, const double* be, int m, int idx, SolveLDS& S) {
+// (kept out of line: it runs only at convergence checks, and inlining it
+// pushes wg_lanczos past the register budget of two workgroups per CU)
+__device__ __attribute__((noinline)) void tri_eigval(const double* al, const double* be, int m, int idx,
+                                                     SolveLDS& S) {
   if ((threadIdx.x >> 6) == 0) {
     const int lane = threadIdx.x & 63;
     double lo = 1e300, hi = -1e300;
@@ -376,7 +383,10 @@ __device__ void tri_eigvec(const double* al, const double* be, int m, double th,
       for (int i = 0; i < m; ++i) {
         double di = al[i] - th;
         if (i > 0) {
-          const double l = be[i - 1] / dprev;
+          double r = __builtin_amdgcn_rcp(dprev);
+          r = r * (2.0 - dprev * r);
+          r = r * (2.0 - dprev * r);
+          const double l = be[i - 1] * r;
           di -= l * be[i - 1];
           s[i] -= l * uprev;
         }
@@ -445,6 +455,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
     bool conv = false;
     bool full = false;
     double beta_prev = 0.0;
+    int next_check = 2;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
       long long t0 = ph_now();
@@ -538,15 +549,30 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       }
       if (threadIdx.x == 0) S.beta[j] = brk ? 0.0 : bn;
       __syncthreads();
-      const bool check = full || brk || (m == kk) || (m >= 2 && (m % 4) == 2) || *iters >= maxit;
+      // convergence checks at m = 2, then every 4 steps, spaced out while the
+      // residual estimate is still far from the tolerance
+      const bool check = full || brk || (m == kk) || m == next_check || *iters >= maxit;
       const long long tc = ph_now();
       if (check) {
         if (!decided) {
-          tri_eigval(S.alpha, S.beta, m, 0, S);
-          const double tlo = S.scal[1];
-          tri_eigval(S.alpha, S.beta, m, m - 1, S);
-          const double thi = S.scal[1];
-          target = (sigma <= 0.5 * (tlo + thi)) ? 0 : 1;
+          // sigma outside the Gershgorin interval of T decides at once
+          double glo = 1e300, ghi = -1e300;
+          for (int i = 0; i < m; ++i) {
+            const double rr = (i > 0 ? fabs(S.beta[i - 1]) : 0.0) + (i < m - 1 ? fabs(S.beta[i]) : 0.0);
+            glo = fmin(glo, S.alpha[i] - rr);
+            ghi = fmax(ghi, S.alpha[i] + rr);
+          }
+          if (sigma <= glo) {
+            target = 0;
+          } else if (sigma >= ghi) {
+            target = 1;
+          } else {
+            tri_eigval(S.alpha, S.beta, m, 0, S);
+            const double tlo = S.scal[1];
+            tri_eigval(S.alpha, S.beta, m, m - 1, S);
+            const double thi = S.scal[1];
+            target = (sigma <= 0.5 * (tlo + thi)) ? 0 : 1;
+          }
           decided = true;
         }
         int idx = target == 0 ? 0 : m - 1;
@@ -576,7 +602,9 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
         const double res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
         ph_add(S, PH_TRI, tc);
-        if (res <= tol * fmax(fabs(th), 1e-3 * anorm)) { conv = true; break; }
+        const double goal = tol * fmax(fabs(th), 1e-3 * anorm);
+        if (res <= goal) { conv = true; break; }
+        next_check = m + ((res > 1e6 * goal && m >= 10) ? 8 : 4);
         if (*iters >= maxit) break;
       }
       const double ib = 1.0 / bn;
