@@ -34,6 +34,7 @@ __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y,
 }
 
 constexpr int KMAX = 40;  // Krylov basis size (GMRES restart / Lanczos)
+static_assert(KMAX < 64, "the tridiagonal eigensolver keeps one row of T per lane");
 struct SolveLDS {
   double red[4 * NWAVE];
   double h[KMAX + 2];
@@ -44,6 +45,7 @@ struct SolveLDS {
   double part[(KMAX + 2) * NWAVE];  // per-wave partial dot products
   long long prof[8];                // cycle counters per phase (thread 0)
   int flag[4];
+  int cnt[2 * NWAVE];               // per-wave ballot counts (double-buffered)
 };
 
 // phase cycle counters (thread 0 accumulates; read back as diagnostics)
@@ -335,11 +337,174 @@ __device__ __forceinline__ int sturm_count(const double* al, const double* be, i
   return c;
 }
 
+// The tridiagonal routines run only at convergence checks; they are kept out
+// of line so that inlining them does not push wg_lanczos past the register
+// budget of two workgroups per CU.
+#define TT_COLD __attribute__((noinline))
+
+__device__ __forceinline__ double rcp_nr(double x) {  // 1/x: hardware estimate + two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
+
+// value of v in lane l (l uniform across the wave)
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Sturm count #{eigenvalues of T < x} with T held one row per lane (a_l =
+// alpha, b2_l = beta^2 of the coupling to the next row), read back with
+// v_readlane so the recurrence never waits on memory.  Signs of p_k =
+// det(T_k - x I) from the three-term recurrence (one dependent FMA a step),
+// rescaled by 2^-600 / 2^600 when |p_k| leaves [2^-600, 2^600]; an exact
+// zero takes the sign of its predecessor.
+__device__ __forceinline__ int sturm_count_rows(double a_l, double b2_l, int m, double x) {
+  double p0 = 1.0, p1 = rdlane(a_l, 0) - x;
+  bool neg = p1 < 0.0;
+  int c = neg ? 1 : 0;
+  for (int k = 1; k < m; ++k) {
+    double p2 = fma(rdlane(a_l, k) - x, p1, -rdlane(b2_l, k - 1) * p0);
+    const double ap = fabs(p2);
+    if (ap > 0x1p+600) {
+      p2 *= 0x1p-600;
+      p1 *= 0x1p-600;
+    } else if (ap < 0x1p-600 && fabs(p1) < 0x1p-600) {
+      p2 *= 0x1p+600;
+      p1 *= 0x1p+600;
+    }
+    const bool ng = (p2 < 0.0) || (p2 == 0.0 && neg);
+    c += (ng != neg) ? 1 : 0;
+    neg = ng;
+    p0 = p1;
+    p1 = p2;
+  }
+  return c;
+}
+
+// Extreme eigenpair of T (m <= 64; idx = 0 smallest or m-1 largest).
+//  * eigenvalue: multisection over all NT threads (NT+1 subintervals a
+//    round) from the Gershgorin interval down to the rounding level;
+//    invariant count(lo) <= idx < count(hi).  Left in S.scal[1].
+//  * unit eigenvector in s, by a twisted factorization: top-down LDL^T
+//    pivots q+ (wave 0) and bottom-up UDU^T pivots q- (wave 1) run
+//    concurrently; the twist index r minimises
+//    |gamma_k| = |q+_k + q-_k - (alpha_k - th)|; then s_r = 1 and
+//    s_k = -(beta_k / q+_k) s_{k+1} above r, s_k = -(beta_{k-1} / q-_k) s_{k-1}
+//    below r.  Stable where one-sided recurrences are not: a converged Ritz
+//    vector decays toward the bottom of T.
+// tw: LDS scratch of 128 doubles.  Returns false (s untouched) if the
+// eigenvector overflowed.
+__device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in, int idx_in, double* s, double* tw,
+                                    SolveLDS& S) {
+  const int m = __builtin_amdgcn_readfirstlane(m_in);
+  const int idx = __builtin_amdgcn_readfirstlane(idx_in);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const bool row = lane < m;
+  const double a_l = row ? al[lane] : 0.0;
+  const double b_l = (lane < m - 1) ? be[lane] : 0.0;
+  const double bm_l = (row && lane > 0) ? be[lane - 1] : 0.0;
+  const double b2_l = b_l * b_l;
+  const double rad = fabs(b_l) + fabs(bm_l);
+  double lo = wave_min(row ? a_l - rad : 1e300);
+  double hi = wave_max(row ? a_l + rad : -1e300);
+  const double pad = 1e-12 * fmax(hi - lo, 1e-300) + 1e-300;
+  lo -= pad;
+  hi += pad;
+  for (int round = 0; round < 64; ++round) {
+    if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+    const double h = (hi - lo) / (double)(NT + 1);
+    if (!(lo + h > lo) || !(lo + h * NT < hi)) break;  // spacing below the rounding of lo / hi
+    const double x = lo + h * (double)(threadIdx.x + 1);
+    const int c = sturm_count_rows(a_l, b2_l, m, x);
+    const unsigned long long ball = __ballot(c <= idx);
+    int* cnt = S.cnt + (round & 1) * NWAVE;
+    if (lane == 0) cnt[wv] = __popcll(ball);
+    __syncthreads();
+    int nle = 0;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) nle += cnt[q];
+    // counts are monotone in x: the first nle points lie left of the eigenvalue
+    const double nlo = nle > 0 ? lo + h * (double)nle : lo;
+    const double nhi = nle < NT ? lo + h * (double)(nle + 1) : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  const double th = 0.5 * (lo + hi);
+  double q_l = 0.0, r_l = 0.0;  // this lane's pivot and ratio
+  if (wv == 0) {
+    double rprev = 0.0;
+    for (int k = 0; k < m; ++k) {
+      double q = (rdlane(a_l, k) - th) - (k > 0 ? rdlane(b_l, k - 1) * rprev : 0.0);
+      if (q == 0.0) q = 1e-300;
+      const double rk = (k < m - 1) ? rdlane(b_l, k) * rcp_nr(q) : 0.0;
+      if (lane == k) {
+        q_l = q;
+        r_l = rk;
+      }
+      rprev = rk;
+    }
+  } else if (wv == 1) {
+    double rnext = 0.0;
+    for (int k = m - 1; k >= 0; --k) {
+      double q = (rdlane(a_l, k) - th) - (k < m - 1 ? rdlane(b_l, k) * rnext : 0.0);
+      if (q == 0.0) q = 1e-300;
+      const double rk = (k > 0) ? rdlane(b_l, k - 1) * rcp_nr(q) : 0.0;
+      if (lane == k) {
+        q_l = q;
+        r_l = rk;
+      }
+      rnext = rk;
+    }
+    tw[lane] = q_l;
+    tw[64 + lane] = r_l;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    const double qm_l = tw[lane], rm_l = tw[64 + lane];
+    const double g = row ? fabs(q_l + qm_l - (a_l - th)) : 1e308;
+    const double gmin = wave_min(g);
+    const unsigned long long hit = __ballot(g == gmin);
+    const int r = hit ? (int)__builtin_ctzll(hit) : 0;
+    double s_l = (lane == r) ? 1.0 : 0.0;
+    double z = 1.0;
+    for (int k = r - 1; k >= 0; --k) {
+      z = -rdlane(r_l, k) * z;
+      if (lane == k) s_l = z;
+    }
+    z = 1.0;
+    for (int k = r + 1; k < m; ++k) {
+      z = -rdlane(rm_l, k) * z;
+      if (lane == k) s_l = z;
+    }
+    const double nn = wave_sum(s_l * s_l);
+    const bool ok = isfinite(nn) && nn > 0.0;
+    if (ok && row) s[lane] = s_l * (1.0 / sqrt(nn));
+    if (lane == 0) {
+      S.flag[1] = ok ? 1 : 0;
+      S.scal[1] = th;
+    }
+  }
+  __syncthreads();
+  return S.flag[1] != 0;
+}
+
 // eigenvalue with index idx (0 = smallest) of T; executed by wave 0, result
 // broadcast through S.scal[1].
-// (kept out of line: it runs only at convergence checks, and inlining it
-// pushes wg_lanczos past the register budget of two workgroups per CU)
-__device__ __attribute__((noinline)) void tri_eigval(const double* al, const double* be, int m, int idx,
+__device__ TT_COLD void tri_eigval(const double* al, const double* be, int m, int idx,
                                                      SolveLDS& S) {
   if ((threadIdx.x >> 6) == 0) {
     const int lane = threadIdx.x & 63;
@@ -371,7 +536,7 @@ __device__ __attribute__((noinline)) void tri_eigval(const double* al, const dou
 }
 
 // eigenvector of T for eigenvalue th by inverse iteration (thread 0).
-__device__ void tri_eigvec(const double* al, const double* be, int m, double th, double* s, double* tmpd) {
+__device__ TT_COLD void tri_eigvec(const double* al, const double* be, int m, double th, double* s, double* tmpd) {
   if (threadIdx.x == 0) {
     double anorm = 0.0;
     for (int i = 0; i < m; ++i) anorm = fmax(anorm, fabs(al[i]) + (i < m - 1 ? fabs(be[i]) : 0.0));
@@ -383,22 +548,22 @@ __device__ void tri_eigvec(const double* al, const double* be, int m, double th,
       for (int i = 0; i < m; ++i) {
         double di = al[i] - th;
         if (i > 0) {
-          double r = __builtin_amdgcn_rcp(dprev);
-          r = r * (2.0 - dprev * r);
-          r = r * (2.0 - dprev * r);
-          const double l = be[i - 1] * r;
+          const double l = be[i - 1] * tmpd[i - 1];
           di -= l * be[i - 1];
           s[i] -= l * uprev;
         }
         if (fabs(di) < pert) di = (di < 0.0 ? -pert : pert);
-        tmpd[i] = di;
+        double rd = __builtin_amdgcn_rcp(di);
+        rd = rd * (2.0 - di * rd);
+        rd = rd * (2.0 - di * rd);
+        tmpd[i] = rd;
         dprev = di;
         uprev = s[i];
       }
       for (int i = m - 1; i >= 0; --i) {
         double t = s[i];
         if (i < m - 1) t -= be[i] * s[i + 1];
-        s[i] = t / tmpd[i];
+        s[i] = t * tmpd[i];
       }
       double nn = 0.0;
       for (int i = 0; i < m; ++i) nn += s[i] * s[i];
@@ -456,6 +621,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
     bool full = false;
     double beta_prev = 0.0;
     int next_check = 2;
+    bool have_vec = false;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
       long long t0 = ph_now();
@@ -597,10 +763,25 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           }
           idx = bidx;
         }
-        tri_eigval(S.alpha, S.beta, m, idx, S);
-        th = S.scal[1];
-        tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
-        const double res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
+        double res;
+        if (full && mode == 2) {
+          // interior eigenvalue: full precision and inverse iteration
+          tri_eigval(S.alpha, S.beta, m, idx, S);
+          th = S.scal[1];
+          tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
+          have_vec = true;
+          res = 0.0;
+        } else {
+          // extreme eigenvalue: multisection to full precision, eigenvector
+          // by twisted factorization, residual estimate bn |s_m|
+          have_vec = tri_extreme(S.alpha, S.beta, m, idx, S.s, S.H + (KMAX + 1), S);
+          th = S.scal[1];
+          if (!have_vec) {
+            tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
+            have_vec = true;
+          }
+          res = (brk || full) ? 0.0 : bn * fabs(S.s[m - 1]);
+        }
         ph_add(S, PH_TRI, tc);
         const double goal = tol * fmax(fabs(th), 1e-3 * anorm);
         if (res <= goal) { conv = true; break; }
@@ -619,6 +800,11 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       }
       beta_prev = bn;
       __syncthreads();
+    }
+    if (!have_vec) {
+      const long long tv = ph_now();
+      tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
+      ph_add(S, PH_TRI, tv);
     }
     // Ritz vector
     for (int e = threadIdx.x; e < N; e += NT) {
