@@ -15,6 +15,7 @@ events on the launch stream) and a CPU baseline: the numpy oracle timed on
 one sweep of one instance on this host.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -30,22 +31,23 @@ PEAK_HBM_GBS = 8000.0
 
 
 def build_batch(B, d, rank, seed0=0):
+    """B TFIM instances; operators and right-orthonormal unit-norm initial
+    guesses uploaded with one host-to-device copy."""
     from scikit_tt_amd import workloads as W
-    from scikit_tt_amd.tensor_train import TT
-    ops, x0s, exact, raw = [], [], [], []
+    from scikit_tt_amd.tensor_train import from_host_batch
+    hops, hx0, exact, raw = [], [], [], []
     for s in range(seed0, seed0 + B):
         H, y0, J, g = W.tfim_chain(seed=s, d=d, rank=rank)
-        t = TT(y0).ortho_right()
-        t = (1.0 / t.norm()) * t
-        ops.append(TT(H))
-        x0s.append(t)
+        hops.append(H)
+        hx0.append(W.right_orthonormal(y0))
         exact.append(W.tfim_ground_energy(J, g))
         raw.append((H, y0))
-    return ops, x0s, np.array(exact), raw
+    tts = from_host_batch(hops + hx0)
+    return tts[:B], tts[B:], np.array(exact), raw
 
 
-def cpu_baseline(raw_instance, sweeps):
-    """numpy oracle, one sweep of one instance on this host (extrapolated to `sweeps`)."""
+def cpu_baseline(raw_instance, sweeps, sample_sweeps=2):
+    """numpy oracle: `sample_sweeps` sweeps of one instance on this host, extrapolated to `sweeps`."""
     from oracle import tt_ops, evp_als
     H, y0 = raw_instance
     y0 = tt_ops.ortho_right(y0)
@@ -56,11 +58,27 @@ def cpu_baseline(raw_instance, sweeps):
     except Exception:  # pragma: no cover
         threads = os.cpu_count() or 1
     t = time.perf_counter()
-    evp_als.als(H, y0, repeats=1, conv_eps=0, solver='eig', sigma=-100)
+    evp_als.als(H, y0, repeats=sample_sweeps, conv_eps=0, solver='eig', sigma=-100)
     dt = time.perf_counter() - t
-    return {'value': 1.0 / (dt * sweeps), 'unit': 'solves/s', 'cores': int(threads), 'kind': 'port',
-            'sample': '1 of %d sweeps of one instance (numpy oracle, dense local eig), x%d extrapolated; '
-                      '%.2f s measured' % (sweeps, sweeps, dt)}
+    per_solve = dt * sweeps / sample_sweeps
+    return {'value': 1.0 / per_solve, 'unit': 'solves/s', 'cores': int(threads), 'kind': 'port',
+            'sample': '%d of %d sweeps of one instance (numpy oracle, dense local eig), extrapolated x%.1f; '
+                      '%.2f s measured' % (sample_sweeps, sweeps, sweeps / sample_sweeps, dt)}
+
+
+def measured_traffic(batch, sites, rank, sweeps):
+    """HBM bytes per k_als launch for this workload from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    known-size kernel), or None if no summary covers it."""
+    import re
+    path = os.path.join(ROOT, 'profiles', 'k_als_tfim_d%d_r%d_b%d_pmc.md' % (sites, rank, batch))
+    if not os.path.exists(path):
+        return None, None
+    text = open(path).read()
+    if ('%d sweeps' % sweeps) not in text.splitlines()[0]:
+        return None, None
+    m = re.search(r'^traffic_bytes_per_launch: (\d+)', text, re.M)
+    return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
 def serving_phase(rate, seconds, seed=7):
@@ -69,7 +87,7 @@ def serving_phase(rate, seconds, seed=7):
     continuous-batching server; returns latency percentiles and throughput."""
     import torch
     from scikit_tt_amd import workloads as W
-    from scikit_tt_amd.tensor_train import TT
+    from scikit_tt_amd.tensor_train import from_host_batch
     from scikit_tt_amd.serving import Server
     rng = np.random.default_rng(seed)
     nreq = max(1, int(rate * seconds))
@@ -80,13 +98,12 @@ def serving_phase(rate, seconds, seed=7):
             d = int(rng.integers(8, 33))
             r = int(np.exp(rng.uniform(np.log(4), np.log(16))))
             H, y0, J, g = W.tfim_chain(seed=1000 + k, d=d, rank=r)
-            t = TT(y0).ortho_right()
-            reqs.append(('evp', TT(H), (1.0 / t.norm()) * t, None))
+            reqs.append(('evp',) + tuple(from_host_batch([H, W.right_orthonormal(y0)])) + (None,))
         else:
             d, n = int(rng.integers(6, 17)), int(rng.choice([4, 8]))
             r = int(rng.integers(4, 9))
             A, x0, b = W.laplace_small(seed=2000 + k, d=d, n=n, rank=r)
-            reqs.append(('sle', TT(A), TT(x0), TT(b)))
+            reqs.append(('sle',) + tuple(from_host_batch([A, x0, b])))
     torch.cuda.synchronize()
     srv = Server(slots=4, max_batch=256)
     t0 = time.perf_counter()
@@ -123,6 +140,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--serve-rates', default='100,1000', help='Poisson arrival rates (req/s) of the serving phase')
     ap.add_argument('--serve-seconds', type=float, default=1.0)
+    ap.add_argument('--copy-bytes', type=int, default=0,
+                    help='run one plain device copy of this many bytes before the timed steps (a known-size '
+                         'reference for hardware byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: every box here has one MI355X; the multi-GPU layout is '
@@ -141,11 +161,20 @@ def main():
     kw = dict(repeats=args.sweeps, conv_eps=0.0, solver='eig', sigma=-100.0)
 
     stream = torch.cuda.current_stream()
+    if args.copy_bytes > 0:
+        # an ordinary elementwise kernel (reads and writes copy_bytes each), not a blit copy
+        src = torch.ones(args.copy_bytes // 8, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     for w in range(args.warmup):
         evp.als_batch(ops, x0s, **kw)
         print('[bench] warmup %d done' % w, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    lat, kernel_ms, flops, lams = [], [], 0.0, None
+    lat, kernel_ms, call_ms, flops, lams = [], [], [], 0.0, None
+    ctx = _lib.context(0)
+    ms = ctypes.c_double()
     t0 = time.perf_counter()
     for k in range(args.steps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -155,7 +184,10 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
         lat.append(1e3 * (time.perf_counter() - ts))
-        kernel_ms.append(e0.elapsed_time(e1))
+        call_ms.append(e0.elapsed_time(e1))
+        # the engine kernel alone: events recorded by the library right around its launch
+        _lib.check(_lib.load().ttals_last_launch_ms(ctx, ctypes.byref(ms)), ctx, 'last_launch_ms')
+        kernel_ms.append(ms.value)
         flops = sum(i.flops for i in infos)
         lams = np.array([o[0] for o in out])
         bad = [i.status for i in infos if i.status not in (0,)]
@@ -177,6 +209,7 @@ def main():
     print('[bench] plain path: %.2f solves/s' % plain_rate, file=sys.stderr, flush=True)
 
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
+    traffic, traffic_src = measured_traffic(args.batch, args.sites, args.rank, args.sweeps)
     achieved = flops / avg_kernel_s / 1e12
     line = {
         'metric': 'TT-ALS solves/s at rel.res <=1e-8 (eigenvalue error <=1e-10), batched vs plain; p50/p99 latency',
@@ -202,8 +235,9 @@ def main():
         'latency_ms_p50': round(float(np.percentile(lat, 50)), 3),
         'latency_ms_p99': round(float(np.percentile(lat, 99)), 3),
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': None,
-                     'kernel': 'k_als<double>', 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
+                     'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': traffic, 'traffic_source': traffic_src,
+                     'kernel': 'k_als<double>', 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3,
+                     'call_ms': float(np.mean(call_ms))},
     }
     if args.serve_rates:
         line['serving'] = []

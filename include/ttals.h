@@ -115,6 +115,13 @@ int ttals_evp_submit(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A,
 int ttals_query(ttals_ctx* ctx);
 int ttals_wait(ttals_ctx* ctx, ttals_result* res);
 
+/* Device time (ms) of the engine kernel of the context's last finished
+ * submission, from HIP events recorded on its stream immediately before and
+ * after the launch (host-side preparation excluded).  TTALS_E_ARG while a
+ * submission is pending or before the first one.  No counterpart in the
+ * reference: a measurement hook for benchmarks. */
+int ttals_last_launch_ms(ttals_ctx* ctx, double* ms);
+
 /* ---- hot-path pieces (one workgroup; for unit parity) --------------------
  * Shapes: x core (a,n,b); operator core (P,n,n,Q); op stacks L (a,P,a),
  * R (b,Q,b); rhs core (p,n,q); rhs stacks Lb (p,a), Rb (q,b). */

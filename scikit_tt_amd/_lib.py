@@ -56,6 +56,7 @@ SIGNATURES = {
     'ttals_evp_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i]),
     'ttals_query': (_i, [_vp]),
     'ttals_wait': (_i, [_vp, _Rp]),
+    'ttals_last_launch_ms': (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_left_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),

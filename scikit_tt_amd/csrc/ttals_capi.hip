@@ -28,7 +28,20 @@ template <class ACC>
 __global__ __launch_bounds__(NT, TTALS_ALS_WAVES_PER_EU) void k_als(const SysDesc* descs, long arena) {
   extern __shared__ double dyn_lds[];
   __shared__ SolveLDS S;
-  als_solve<ACC>(descs[blockIdx.x], reinterpret_cast<ACC*>(dyn_lds), arena, S);
+  const SysDesc& D = descs[blockIdx.x];
+  // x <- x0: the engine works in place on the output cores
+  for (int i = 0; i < D.d; ++i) {
+    const double* src = D.x0c[i];
+    double* dst = D.xc[i];
+    if (src == dst) continue;
+    const long len = (long)D.rx[i] * D.n[i] * D.rx[i + 1];
+    for (long e = threadIdx.x; e < len; e += NT) dst[e] = src[e];
+  }
+  __syncthreads();
+  als_solve<ACC>(D, reinterpret_cast<ACC*>(dyn_lds), arena, S);
+  __syncthreads();
+  // one compact row per system, so the host fetches every result with one copy
+  for (int k = threadIdx.x; k < 17 + D.d; k += NT) D.res[k] = k < 16 ? D.out[k] : (double)D.rx[k - 16];
 }
 
 struct PieceArgs {
@@ -181,7 +194,8 @@ struct NormArgs {
   int d;
   const int* mn;
   const int* r;
-  double* const* c;   // working copies
+  const double* const* src;  // the caller's cores (read only)
+  double* const* c;   // working cores: c[i-1] is written when core i is absorbed
   double* W;          // >= max_i mn_i * r_{i+1} * r_i
   double* Rc;         // >= r_i^2
   double* T;          // >= max_i r_{i-1} * mn_{i-1} * r_i
@@ -195,10 +209,11 @@ __global__ __launch_bounds__(NT) void k_tt_norm(NormArgs A) {
   for (int i = A.d - 1; i >= 1; --i) {
     const int a = A.r[i], q = A.mn[i], b = right;
     const int m = q * b, c = a;
-    // W (m x c) = core_i^T, core_i is (a x q*b)
+    // W (m x c) = core_i^T, core_i is (a x q*b); the last core is still the caller's
+    const double* ci = (i == A.d - 1) ? A.src[i] : A.c[i];
     for (long e = threadIdx.x; e < (long)m * c; e += NT) {
       const int ii = (int)(e / c), jj = (int)(e % c);
-      A.W[e] = A.c[i][(long)jj * m + ii];
+      A.W[e] = ci[(long)jj * m + ii];
     }
     __syncthreads();
     wg_householder(A.W, m, c, A.tau, S.red);
@@ -210,13 +225,14 @@ __global__ __launch_bounds__(NT) void k_tt_norm(NormArgs A) {
     __syncthreads();
     // core_{i-1} (rows x a) <- core_{i-1} * R^T  (rows x k)
     const int rows = A.r[i - 1] * A.mn[i - 1];
-    wg_gemm<double, double>(rows, k, a, crm<double>(A.c[i - 1], a), ctr<double>(A.Rc, c), mrm<double>(A.T, k), lds);
+    wg_gemm<double, double>(rows, k, a, crm<double>(A.src[i - 1], a), ctr<double>(A.Rc, c), mrm<double>(A.T, k),
+                            lds);
     wg_copy(A.T, A.c[i - 1], rows * k);
     __syncthreads();
     right = k;
   }
   const long n0 = (long)A.r[0] * A.mn[0] * right;
-  const double nrm = wg_nrm2(A.c[0], (int)n0, S.red);
+  const double nrm = wg_nrm2(A.d == 1 ? A.src[0] : A.c[0], (int)n0, S.red);
   if (threadIdx.x == 0) A.out[0] = nrm;
 }
 
@@ -347,7 +363,7 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
 // host side
 
 struct Pending {
-  size_t r_rx, r_out;  // staging offsets of the ranks and scalars of one system
+  size_t r_res;        // staging offset of the system's result row (16 scalars, then d+1 ranks)
   int d;
   int* ranks;          // caller's output rank array
 };
@@ -359,6 +375,8 @@ struct ttals_ctx {
   char* hst = nullptr;       // pinned staging
   size_t hst_bytes = 0;
   hipEvent_t done = nullptr;  // completion of the last submission
+  hipEvent_t k0 = nullptr, k1 = nullptr;  // bracket the last k_als launch (timing events)
+  bool timed = false;
   bool inflight = false;
   std::vector<Pending> pending;
   double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
@@ -435,7 +453,7 @@ struct SysPlan {
   size_t xsz_total = 0;
   size_t nL = 0, nR = 0, nLb = 0, nRb = 0;
   // metadata (host staging offsets)
-  size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_Loff, m_Roff, m_Lboff, m_Rboff, m_desc;
+  size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_x0c, m_Loff, m_Roff, m_Lboff, m_Rboff, m_desc;
   // device workspace offsets (doubles region)
   size_t w_L, w_R, w_Lb, w_Rb, w_f, w_g, w_t1, w_t2, w_V, w_dense, w_qw, w_qq, w_tau, w_rf, w_best, w_out;
   std::vector<long> Loff, Roff, Lboff, Rboff;
@@ -551,6 +569,7 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     P.m_ac = hl.take(sizeof(void*) * d);
     P.m_bc = hl.take(sizeof(void*) * d);
     P.m_bestc = hl.take(sizeof(void*) * d);
+    P.m_x0c = hl.take(sizeof(void*) * d);
     P.m_Loff = hl.take(sizeof(long) * d);
     P.m_Roff = hl.take(sizeof(long) * d);
     P.m_Lboff = hl.take(sizeof(long) * d);
@@ -573,14 +592,16 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     P.w_out = dl.take(8 * 16);
   }
   const size_t meta_bytes = hl.bytes;
-  // results region (ranks + scalars) in the staging buffer after the metadata
+  // compact results: one row of 16 scalars + (dmax+1) ranks per system, on
+  // the device after the workspace and in the staging buffer after the
+  // metadata
+  int dmax = 0;
+  for (int s = 0; s < count; ++s) dmax = std::max(dmax, plans[s].d);
+  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t w_res = dl.take(8 * res_stride * count);
   Layout rl;
   rl.bytes = meta_bytes;
-  std::vector<size_t> r_rx(count), r_out(count);
-  for (int s = 0; s < count; ++s) {
-    r_rx[s] = rl.take(sizeof(int) * (plans[s].d + 1));
-    r_out[s] = rl.take(8 * 16);
-  }
+  const size_t r_res = rl.take(8 * res_stride * count);
   int rc;
   if ((rc = ensure_hst(c, rl.bytes))) return rc;
   if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
@@ -601,8 +622,10 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     const double** acp = hptr<const double*>(c, P.m_ac);
     const double** bcp = hptr<const double*>(c, P.m_bc);
     double** bestp = hptr<double*>(c, P.m_bestc);
+    const double** x0p = hptr<const double*>(c, P.m_x0c);
     size_t boff = 0;
     for (int i = 0; i < d; ++i) {
+      x0p[i] = x0[s].cores[i];
       xcp[i] = xo[s].cores[i];
       acp[i] = A[s].cores[i];
       bcp[i] = kind == 0 ? b[s].cores[i] : nullptr;
@@ -654,29 +677,27 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     D.tau = wptr(P.w_tau);
     D.rf = wptr(P.w_rf);
     D.out = wptr(P.w_out);
+    D.x0c = (const double* const*)dptr(P.m_x0c);
+    D.res = wptr(w_res) + res_stride * s;
     D.trace = (c->trace && s == 0) ? c->trace : nullptr;
     D.trace_cap = c->trace_cap;
-    // x_out <- x0 (the engine works in place on x_out)
-    for (int i = 0; i < d; ++i) {
-      const size_t bytes = 8 * (size_t)P.rx[i] * P.n[i] * P.rx[i + 1];
-      if (xo[s].cores[i] != x0[s].cores[i])
-        HIPCHK(c, hipMemcpyAsync(xo[s].cores[i], x0[s].cores[i], bytes, hipMemcpyDeviceToDevice, stream));
-    }
   }
   HIPCHK(c, hipMemcpyAsync(dmeta, c->hst, meta_bytes, hipMemcpyHostToDevice, stream));
   const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
+  if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
+  if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
+  HIPCHK(c, hipEventRecord(c->k0, stream));
   hipLaunchKernelGGL(k_als<double>, dim3(count), dim3(NT), (size_t)arena * 8, stream,
                      (const SysDesc*)dptr(m_descs), arena);
   HIPCHK(c, hipGetLastError());
-  for (int s = 0; s < count; ++s) {
-    HIPCHK(c, hipMemcpyAsync(c->hst + r_rx[s], dptr(plans[s].m_rx), sizeof(int) * (plans[s].d + 1),
-                             hipMemcpyDeviceToHost, stream));
-    HIPCHK(c, hipMemcpyAsync(c->hst + r_out[s], wptr(plans[s].w_out), 8 * 16, hipMemcpyDeviceToHost, stream));
-  }
+  HIPCHK(c, hipEventRecord(c->k1, stream));
+  c->timed = true;
+  HIPCHK(c, hipMemcpyAsync(c->hst + r_res, wptr(w_res), 8 * res_stride * count, hipMemcpyDeviceToHost, stream));
   if (!c->done) HIPCHK(c, hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
   HIPCHK(c, hipEventRecord(c->done, stream));
   c->pending.clear();
-  for (int s = 0; s < count; ++s) c->pending.push_back(Pending{r_rx[s], r_out[s], plans[s].d, xo[s].ranks});
+  for (int s = 0; s < count; ++s)
+    c->pending.push_back(Pending{r_res + 8 * res_stride * s, plans[s].d, xo[s].ranks});
   c->inflight = true;
   return 0;
 }
@@ -689,9 +710,8 @@ int finish_batch(ttals_ctx* c, ttals_result* res) {
   int worst = 0;
   for (size_t s = 0; s < c->pending.size(); ++s) {
     const Pending& pd = c->pending[s];
-    const int* rr = hptr<int>(c, pd.r_rx);
-    for (int i = 0; i <= pd.d; ++i) pd.ranks[i] = rr[i];
-    const double* o = hptr<double>(c, pd.r_out);
+    const double* o = hptr<double>(c, pd.r_res);
+    for (int i = 0; i <= pd.d; ++i) pd.ranks[i] = (int)o[16 + i];
     ttals_result& R = res[s];
     R.lambda = o[0];
     R.sweeps = (int)o[1];
@@ -766,6 +786,8 @@ void ttals_ctx_destroy(ttals_ctx* c) {
     (void)hipEventSynchronize(c->done);
     (void)hipEventDestroy(c->done);
   }
+  if (c->k0) (void)hipEventDestroy(c->k0);
+  if (c->k1) (void)hipEventDestroy(c->k1);
   if (c->dws) (void)hipFree(c->dws);
   if (c->hst) (void)hipHostFree(c->hst);
   delete c;
@@ -801,6 +823,16 @@ int ttals_query(ttals_ctx* c) {
 int ttals_wait(ttals_ctx* c, ttals_result* res) {
   if (!c || !res) return TTALS_E_ARG;
   return finish_batch(c, res);
+}
+
+int ttals_last_launch_ms(ttals_ctx* c, double* ms) {
+  if (!c || !ms) return TTALS_E_ARG;
+  if (c->inflight || !c->timed) return fail(c, TTALS_E_ARG, "no finished submission to time");
+  float t = 0.0f;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipEventElapsedTime(&t, c->k0, c->k1));
+  *ms = (double)t;
+  return 0;
 }
 
 int ttals_sle_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
@@ -999,18 +1031,19 @@ int ttals_tt_norm(ttals_ctx* c, void* stream, const ttals_tt* t, double* out) {
   }
   const size_t oW = L.take(8 * wmax), oR = L.take(8 * rmax * rmax), oT = L.take(8 * tmax), otau = L.take(8 * rmax),
                oout = L.take(64);
-  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), optr = L.take(sizeof(void*) * d);
+  const size_t omn = L.take(sizeof(int) * d), orr = L.take(sizeof(int) * (d + 1)), optr = L.take(sizeof(void*) * d),
+               osrc = L.take(sizeof(void*) * d);
   if ((rc = ensure_dws(c, L.bytes))) return rc;
   if ((rc = ensure_hst(c, L.bytes))) return rc;
   char* db = reinterpret_cast<char*>(c->dws);
   int* hmn = reinterpret_cast<int*>(c->hst + omn);
   int* hrr = reinterpret_cast<int*>(c->hst + orr);
   double** hp = reinterpret_cast<double**>(c->hst + optr);
+  const double** hs = reinterpret_cast<const double**>(c->hst + osrc);
   for (int i = 0; i < d; ++i) {
     hmn[i] = t->row_dims[i] * t->col_dims[i];
     hp[i] = reinterpret_cast<double*>(db + coff[i]);
-    const size_t bytes = 8 * (size_t)t->ranks[i] * hmn[i] * t->ranks[i + 1];
-    HIPCHK(c, hipMemcpyAsync(hp[i], t->cores[i], bytes, hipMemcpyDeviceToDevice, st));
+    hs[i] = t->cores[i];
   }
   for (int i = 0; i <= d; ++i) hrr[i] = t->ranks[i];
   HIPCHK(c, hipMemcpyAsync(db + omn, c->hst + omn, L.bytes - omn, hipMemcpyHostToDevice, st));
@@ -1019,6 +1052,7 @@ int ttals_tt_norm(ttals_ctx* c, void* stream, const ttals_tt* t, double* out) {
   na.mn = reinterpret_cast<const int*>(db + omn);
   na.r = reinterpret_cast<const int*>(db + orr);
   na.c = reinterpret_cast<double* const*>(db + optr);
+  na.src = reinterpret_cast<const double* const*>(db + osrc);
   na.W = reinterpret_cast<double*>(db + oW);
   na.Rc = reinterpret_cast<double*>(db + oR);
   na.T = reinterpret_cast<double*>(db + oT);

@@ -48,7 +48,9 @@ struct SysDesc {
   double* qq;                 // nmax (explicit Q)
   double* tau;                // rmax
   double* rf;                 // rmax^2 (clean R factor)
-  double* out;                // [8] lambda, sweeps, status, krylov iters, unconverged, local solves
+  double* out;                // [16] lambda, sweeps, status, krylov iters, unconverged, local solves, ...
+  const double* const* x0c;   // [d] initial-guess cores, copied into xc at the start (skipped where equal)
+  double* res;                // compact result row: out[0..15], then the d+1 final ranks
   double* trace;              // optional: 6 doubles per local solve (nullptr = off)
   long trace_cap;             // capacity in local solves
 };

@@ -206,7 +206,7 @@ class Server:
         now = time.perf_counter()
         for k, t in enumerate(take):
             info = _engine.SolveInfo(res[k])
-            x = _engine.finish(cores[k], opacks[k], t.x0.row_dims)
+            x = _engine.finish(cores, k, opacks[k], t.x0)
             t.info = info
             t.result = x if t.kind == 'sle' else (info.eigenvalue, x, info.sweeps)
             t.t_done = now

@@ -44,24 +44,32 @@ def descs(tts):
 
 
 def outputs(initial_guesses):
-    """Output cores with the capacity of the initial guesses (ranks never grow)."""
-    cores = [[torch.empty_like(c) for c in x.device_cores] for x in initial_guesses]
-    packs = [_lib.DescPack(x.order, x.row_dims, x.col_dims, x.ranks, [c.data_ptr() for c in oc])
-             for x, oc in zip(initial_guesses, cores)]
-    return cores, packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
+    """Output cores with the capacity of the initial guesses (ranks never grow).
+
+    One device allocation holds every output core of the batch; system s
+    gets the offsets ``slots[s]`` (in doubles) into it.
+    """
+    dev = initial_guesses[0].device
+    slots, total = [], 0
+    for x in initial_guesses:
+        r = np.asarray(x.ranks, dtype=np.int64)
+        sizes = r[:-1] * np.asarray(x.row_dims, dtype=np.int64) * np.asarray(x.col_dims, dtype=np.int64) * r[1:]
+        offs = total + np.concatenate(([0], np.cumsum(sizes)[:-1]))
+        slots.append(offs)
+        total += int(sizes.sum())
+    buf = torch.empty(max(total, 1), dtype=torch.float64, device=dev)
+    base = buf.data_ptr()
+    packs = [_lib.DescPack(x.order, x.row_dims, x.col_dims, x.ranks, (base + 8 * offs).tolist())
+             for x, offs in zip(initial_guesses, slots)]
+    return (buf, slots), packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
 
 
-def finish(cores, pack, row_dims):
-    """View the engine's in-place cores with their final ranks."""
+def finish(cores, s, pack, x0):
+    """System s's solution: the engine's in-place cores with their final ranks
+    (each core at the start of its slot), as views of the batch buffer."""
+    buf, slots = cores
     ranks = pack.final_ranks()
-    out = []
-    for i, c in enumerate(cores):
-        shape = (ranks[i], row_dims[i], 1, ranks[i + 1])
-        if tuple(c.shape) == shape:
-            out.append(c)
-        else:
-            out.append(c.reshape(-1)[:int(np.prod(shape))].reshape(shape).clone())
-    return TT._from_device(out)
+    return TT._from_flat(buf, slots[s], x0.row_dims, x0.col_dims, ranks)
 
 
 def raise_status(info, what):

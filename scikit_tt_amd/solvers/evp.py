@@ -63,7 +63,7 @@ def als_batch(operators, initial_guesses, repeats=1, conv_eps=1e-10, solver='eig
             _engine.raise_status(info, 'evp.als system %d' % s if count > 1 else 'evp.als')
     out = []
     for s in range(count):
-        t = _engine.finish(cores[s], opacks[s], initial_guesses[s].row_dims)
+        t = _engine.finish(cores, s, opacks[s], initial_guesses[s])
         out.append((infos[s].eigenvalue, t, infos[s].sweeps))
     return (out, infos) if return_info else out
 

@@ -50,7 +50,7 @@ def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='s
     if not return_info:
         for s, info in enumerate(infos):
             _engine.raise_status(info, 'sle.als system %d' % s if count > 1 else 'sle.als')
-    sols = [_engine.finish(cores[s], opacks[s], initial_guesses[s].row_dims) for s in range(count)]
+    sols = [_engine.finish(cores, s, opacks[s], initial_guesses[s]) for s in range(count)]
     return (sols, infos) if return_info else sols
 
 

@@ -18,7 +18,7 @@ import torch
 
 from . import _lib
 
-__all__ = ['TT', 'zeros', 'ones', 'eye', 'rand', 'residual_error', 'device']
+__all__ = ['TT', 'zeros', 'ones', 'eye', 'rand', 'residual_error', 'device', 'from_host_batch']
 
 
 def device():
@@ -43,6 +43,44 @@ def _as_device_core(c, dev):
     if np.iscomplexobj(arr):
         raise NotImplementedError('complex tensor trains are not supported')
     return torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(dev)
+
+
+def from_host_batch(core_lists):
+    """Many tensor trains from host cores with ONE host-to-device copy.
+
+    Every core of every train is packed into one pinned host buffer, copied
+    once, and the device cores are views into that single allocation (a
+    batch of small trains would otherwise cost one copy per core).
+    """
+    dev = device()
+    shapes = [[tuple(np.shape(c)) for c in cl] for cl in core_lists]
+    for sh in shapes:
+        if not sh or not all(len(s) == 4 for s in sh):
+            raise ValueError('List elements must be 4-dimensional arrays.')
+        if not all(sh[i][3] == sh[i + 1][0] for i in range(len(sh) - 1)):
+            raise ValueError('Shapes of list elements do not match.')
+    total = sum(int(np.prod(s)) for sh in shapes for s in sh)
+    host = torch.empty(total, dtype=torch.float64, pin_memory=True)
+    hv = host.numpy()
+    off = 0
+    for cl in core_lists:
+        for c in cl:
+            a = np.asarray(c)
+            if np.iscomplexobj(a):
+                raise NotImplementedError('complex tensor trains are not supported')
+            hv[off:off + a.size] = a.reshape(-1)
+            off += a.size
+    buf = host.to(dev, non_blocking=True)
+    torch.cuda.current_stream(dev).synchronize()  # the pinned source must outlive the copy
+    out, off = [], 0
+    for sh in shapes:
+        cores = []
+        for s in sh:
+            n = int(np.prod(s))
+            cores.append(buf[off:off + n].view(s))
+            off += n
+        out.append(TT._from_device(cores))
+    return out
 
 
 class _CoreList:
@@ -132,8 +170,7 @@ class TT:
         return not (all(i == 1 for i in self.row_dims) or all(i == 1 for i in self.col_dims))
 
     def _desc(self):
-        return _lib.DescPack(self.order, self.row_dims, self.col_dims, self.ranks,
-                             [c.data_ptr() for c in self.device_cores])
+        return _lib.DescPack(self.order, self.row_dims, self.col_dims, self.ranks, self._core_ptrs())
 
     @classmethod
     def _from_device(cls, cores):
@@ -144,6 +181,48 @@ class TT:
         t.col_dims = [int(c.shape[2]) for c in cores]
         t.ranks = [int(c.shape[0]) for c in cores] + [int(cores[-1].shape[3])]
         return t
+
+    @classmethod
+    def _from_flat(cls, buf, offsets, row_dims, col_dims, ranks):
+        """Train whose core i is buf[offsets[i]:...] viewed as (r_i, m_i, n_i, r_i+1).
+
+        The per-core tensors are created on first access, so that a batch of
+        results costs no per-core work until a result is used.
+        """
+        t = cls.__new__(cls)
+        t._cores = None
+        t._lazy = (buf, offsets)
+        t.order = len(row_dims)
+        t.row_dims, t.col_dims, t.ranks = list(row_dims), list(col_dims), list(ranks)
+        return t
+
+    @property
+    def device_cores(self):
+        cores = self.__dict__.get('_cores')
+        if cores is None:
+            buf, offs = self._lazy
+            r, m, n = self.ranks, self.row_dims, self.col_dims
+            cores = []
+            for i in range(self.order):
+                shape = (r[i], m[i], n[i], r[i + 1])
+                o = int(offs[i])
+                cores.append(buf[o:o + r[i] * m[i] * n[i] * r[i + 1]].view(shape))
+            self._cores = cores
+            self._lazy = None
+        return cores
+
+    @device_cores.setter
+    def device_cores(self, cores):
+        self._cores = cores
+        self._lazy = None
+
+    def _core_ptrs(self):
+        """Device addresses of the cores (without creating per-core tensors)."""
+        if self.__dict__.get('_cores') is None and self.__dict__.get('_lazy') is not None:
+            buf, offs = self._lazy
+            base = buf.data_ptr()
+            return [base + 8 * int(o) for o in offs]
+        return [c.data_ptr() for c in self.device_cores]
 
     # -- copies and conversion ---------------------------------------------------
 

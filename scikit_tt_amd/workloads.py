@@ -19,7 +19,7 @@ import numpy as np
 __all__ = [
     'laplace_1d', 'laplace_operator', 'tfim_operator', 'tfim_ground_energy',
     'identity_operator', 'random_tt', 'ones_tt', 'shifted_operator', 'tt_sum',
-    'laplace_small', 'tfim_chain', 'laplace_large', 'shifted_batch_instance',
+    'laplace_small', 'tfim_chain', 'laplace_large', 'shifted_batch_instance', 'right_orthonormal',
 ]
 
 
@@ -188,3 +188,24 @@ def shifted_batch_instance(s, d_range=(8, 24), n_choices=(4, 8, 16), r_range=(4,
     b = random_tt(rng, [n] * d, rhs_rank)
     x0 = random_tt(rng, [n] * d, r)
     return A, x0, b
+
+
+def right_orthonormal(cores, normalize=True):
+    """The same tensor with cores 1..d-1 right-orthonormal (host QR sweep).
+
+    From the last core down: the transposed unfolding (n r' x r) of core i is
+    factored Q R, core i becomes Q^T (rank min(r, n r')) and R^T is absorbed
+    into core i-1.  With ``normalize`` the first core is scaled to unit
+    Frobenius norm, which is then the norm of the whole tensor.
+    """
+    out = [np.array(c, dtype=np.float64) for c in cores]
+    for i in range(len(out) - 1, 0, -1):
+        r, m, n, r2 = out[i].shape
+        q, rf = np.linalg.qr(out[i].reshape(r, m * n * r2).T)
+        k = q.shape[1]
+        out[i] = q.T.reshape(k, m, n, r2)
+        out[i - 1] = np.tensordot(out[i - 1], rf.T, axes=([3], [0]))
+    if normalize:
+        out[0] = out[0] / np.linalg.norm(out[0])
+    return out
+

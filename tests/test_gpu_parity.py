@@ -395,3 +395,20 @@ def test_ortho_left_truncated(env):
     t = TT(x).ortho_left(threshold=1e-7)
     assert t.ranks == tt_ops.ranks_of(ref)
     np.testing.assert_allclose(t.matricize(), tt_ops.matricize(ref), rtol=1e-8, atol=1e-10)
+
+
+def test_from_host_batch_and_norm_leave_cores_untouched(env):
+    from scikit_tt_amd.tensor_train import TT, from_host_batch
+    rng = np.random.default_rng(21)
+    lists = [W.random_tt(rng, [2, 3, 2, 2, 3], [1, 2, 4, 3, 2, 1]) for _ in range(3)] + \
+            [W.random_tt(rng, [4], [1, 1])]
+    tts = from_host_batch(lists)
+    for cl, t in zip(lists, tts):
+        assert t.ranks == [c.shape[0] for c in cl] + [1]
+        for c, dc in zip(cl, t.device_cores):
+            np.testing.assert_array_equal(dc.cpu().numpy(), c)
+        before = [dc.clone() for dc in t.device_cores]
+        nrm = t.norm()
+        assert abs(nrm - tt_ops.norm(cl)) <= 1e-13 * tt_ops.norm(cl)
+        for b, dc in zip(before, t.device_cores):
+            assert torch.equal(b, dc)
