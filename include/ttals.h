@@ -115,6 +115,20 @@ int ttals_evp_submit(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A,
 int ttals_query(ttals_ctx* ctx);
 int ttals_wait(ttals_ctx* ctx, ttals_result* res);
 
+/* Size of the on-chip (LDS) arena in KB, 0..62 (default 62): local
+ * problems whose operator and working vectors fit run LDS-resident on the
+ * FP64 matrix cores, the others from HBM/L2; 0 forces the global path.
+ * Applies to later submissions and pieces on this context.  No counterpart
+ * in the reference: a tuning knob. */
+int ttals_set_arena_kb(ttals_ctx* ctx, int kb);
+
+/* Local tolerance policy of later evp submissions on this context:
+ * 0 (default) every local eigenproblem to `tol`; k = 1..15 adaptive
+ * (inexact ALS): sweeps 1-2 to max(tol, 1e-k), then to max(tol, min(1e-k,
+ * 0.1 sqrt(|dlambda|/|lambda|))) of the last sweep-to-sweep change.  No
+ * counterpart in the reference (its local eigensolves are dense). */
+int ttals_set_local_tol_policy(ttals_ctx* ctx, int policy);
+
 /* Device time (ms) of the engine kernel of the context's last finished
  * submission, from HIP events recorded on its stream immediately before and
  * after the launch (host-side preparation excluded).  TTALS_E_ARG while a

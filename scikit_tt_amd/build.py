@@ -9,6 +9,9 @@ SRC = os.path.join(HERE, 'csrc', 'ttals_capi.hip')
 OUT = os.path.join(HERE, 'libttals.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+         # MFMA results in ordinary VGPRs: accumulation registers (AGPRs) would be
+         # allocated on top of the 256 VGPRs and halve the workgroups per CU
+         '-mllvm', '-amdgpu-mfma-vgpr-form=1',
          # never emit packed FP32 (v_pk_*_f32) instructions
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 

@@ -59,14 +59,20 @@ struct PieceArgs {
   double* res;       // [4] scalars out
   int method, sym, maxit, mode;
   double tol, sigma;
+  long arena;        // dynamic LDS of the launch in doubles (on-chip path when the problem fits)
 };
 
 enum PieceOp { P_LEFT_OP, P_RIGHT_OP, P_LEFT_RHS, P_RIGHT_RHS, P_MICRO_DENSE, P_MATVEC, P_MICRO_RHS, P_SOLVE,
                P_EIG, P_QR };
 
+// One workgroup.  The matvec, local solve and local eigensolve take the
+// engine's on-chip path (operator and working vectors in the dynamic LDS
+// arena) whenever the problem fits `A.arena`, else the global path.
 __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
   __shared__ double lds[GEMM_LDS_DOUBLES];
   __shared__ SolveLDS S;
+  extern __shared__ double dyn_lds[];
+  const long N = (long)A.a * A.n * A.b;
   switch (A.op) {
     case P_LEFT_OP:
       stack_left_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, A.t2, lds);
@@ -84,14 +90,22 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       micro_dense<double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.a * A.n * A.b);
       break;
     case P_MATVEC:
-      local_matvec<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.x, A.out, A.t1, A.t2, lds);
+      if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 2) <= A.arena) {
+        LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
+        double* vec = onchip_operator(op, dyn_lds);
+        wg_copy(A.x, vec, (int)N);
+        __syncthreads();
+        apply_op<double, double>(op, vec, vec + N, lds);
+        wg_copy(vec + N, A.out, (int)N);
+      } else {
+        local_matvec<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.x, A.out, A.t1, A.t2, lds);
+      }
       break;
     case P_MICRO_RHS:
       micro_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.t1, lds);
       break;
     case P_SOLVE: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
-      const int N = op.size();
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
       int its = 0, r = 0;
       int method = A.method;
       if (method == 0) method = (N <= NDENSE) ? 1 : (A.sym ? 2 : 3);
@@ -100,6 +114,16 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         wg_copy(A.f, A.v, N);
         __syncthreads();
         r = wg_lu_solve(A.V, N, A.v, reinterpret_cast<int*>(A.V + (long)N * N), S.red) ? 0 : 1;
+      } else if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 5) <= A.arena) {
+        Work W;
+        W.V = A.V;
+        onchip_work(W, onchip_operator(op, dyn_lds), (int)N, true);
+        wg_copy(A.f, W.f, (int)N);
+        wg_copy(A.v, W.x, (int)N);
+        __syncthreads();
+        if (method == 2) r = wg_cg<double, true>(op, W, A.tol, A.maxit, lds, S, &its);
+        if (method == 3 || r == 1) r = wg_gmres<double>(op, W, A.tol, A.maxit, KMAX, lds, S, &its);
+        wg_copy(W.x, A.v, (int)N);
       } else {
         Work W;
         W.V = A.V;
@@ -120,16 +144,21 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       break;
     }
     case P_EIG: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
-      const int N = op.size();
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
       Work W;
       W.V = A.V;
-      W.cur = A.V + (long)(KMAX + 1) * N;
-      W.w = A.V + (long)(KMAX + 2) * N;
-      W.r = A.V + (long)(KMAX + 3) * N;
-      int its = 0;
+      int its = 0, r;
       double lam = 0.0;
-      const int r = wg_lanczos<double, false>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 3) <= A.arena) {
+        onchip_work(W, onchip_operator(op, dyn_lds), (int)N, false);
+        __syncthreads();
+        r = wg_lanczos<double, true>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      } else {
+        W.cur = A.V + (long)(KMAX + 1) * N;
+        W.w = A.V + (long)(KMAX + 2) * N;
+        W.r = A.V + (long)(KMAX + 3) * N;
+        r = wg_lanczos<double, false>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+      }
       if (threadIdx.x == 0) {
         A.res[0] = r;
         A.res[1] = its;
@@ -384,6 +413,7 @@ struct ttals_ctx {
   // on-chip arena of k_als in doubles: 62 KB, which with the ~17 KB of static
   // LDS keeps two systems resident per CU (160 KB of LDS)
   long arena = 7936;
+  int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
   std::string err;
 };
 
@@ -645,6 +675,7 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     D.ortho_x0 = (kind == 0) ? 1 : 0;
     D.sym = -1;
     D.gm_restart = KMAX;
+    D.adapt = (kind == 1) ? c->tol_policy : 0;
     D.maxit = maxit;
     D.tol = tol;
     D.sigma = sigma;
@@ -745,7 +776,8 @@ int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double
 }
 
 int launch_piece(ttals_ctx* c, hipStream_t st, PieceArgs& a) {
-  hipLaunchKernelGGL(k_piece, dim3(1), dim3(NT), 0, st, a);
+  a.arena = c->arena;
+  hipLaunchKernelGGL(k_piece, dim3(1), dim3(NT), (size_t)a.arena * 8, st, a);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -823,6 +855,20 @@ int ttals_query(ttals_ctx* c) {
 int ttals_wait(ttals_ctx* c, ttals_result* res) {
   if (!c || !res) return TTALS_E_ARG;
   return finish_batch(c, res);
+}
+
+int ttals_set_arena_kb(ttals_ctx* c, int kb) {
+  if (!c || kb < 0 || kb > 62) return TTALS_E_ARG;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
+  c->arena = (long)kb * 1024 / 8;
+  return 0;
+}
+
+int ttals_set_local_tol_policy(ttals_ctx* c, int policy) {
+  if (!c || policy < 0 || policy > 15) return TTALS_E_ARG;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
+  c->tol_policy = policy;
+  return 0;
 }
 
 int ttals_last_launch_ms(ttals_ctx* c, double* ms) {

@@ -11,9 +11,6 @@
 namespace ttals {
 
 constexpr int NT = 256;            // threads per workgroup: 4 x wave64
-#ifndef TTALS_MFMA_MATVEC
-#define TTALS_MFMA_MATVEC 1        // on-chip local matvec on the FP64 matrix cores
-#endif
 constexpr int NWAVE = NT / 64;
 
 // ---------------------------------------------------------------------------
@@ -335,46 +332,6 @@ struct KStep4 {
   }
 };
 
-// C[M,N] = A[M,K] B[K,N]; 16x16 output tiles round-robin over the waves.
-// Indices past M, N or K contribute zeros and are not stored.
-__device__ void mf_gemm_lds(int M, int N, int K, CView<double> A, CView<double> B, MView<double> C) {
-  const TT_LDS double* pa = (const TT_LDS double*)A.p;
-  const TT_LDS double* pb = (const TT_LDS double*)B.p;
-  TT_LDS double* pc = (TT_LDS double*)C.p;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int li = lane & 15, lk = lane >> 4;
-  const int tn = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tn;
-  for (int t = w; t < tiles; t += NWAVE) {
-    const int ti = t / tn, tj = t - ti * tn;
-    const int i = (ti << 4) + li, j = (tj << 4) + li;
-    const bool iv = i < M, jv = j < N;
-    const int ar = iv ? (int)imap(A.r, i) : 0;
-    const int bc = jv ? (int)imap(B.c, j) : 0;
-    KStep4 ka, kb;
-    ka.start(A.c, lk);
-    kb.start(B.r, lk);
-    dx4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const bool kv = k0 + lk < K;
-      double av = 0.0, bv = 0.0;
-      if (iv && kv) av = pa[ar + ka.off];
-      if (jv && kv) bv = pb[kb.off + bc];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      ka.next();
-      kb.next();
-    }
-    if (jv) {
-      const int cc = (int)imap(C.c, j);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = (ti << 4) + lk + 4 * e;
-        if (row < M) pc[(int)imap(C.r, row) + cc] = acc[e];
-      }
-    }
-  }
-  __syncthreads();
-}
-
 // unscaled 2-norm (for vectors of moderate magnitude in the hot loops)
 template <class T>
 __device__ __forceinline__ double wg_nrm2u(const T* x, int n, double* red) {
@@ -416,10 +373,16 @@ __device__ void local_matvec(int a, int P, int n, int Q, int b, const T* L, cons
   wg_gemm<T, ACC>(a * n, b, b * Q, crm<T>(t2, b * Q), crm<T>(R, b), mrm<T>(y, b), lds);
 }
 
-// Same contraction with every operand on-chip (ds_gemm stages).  The two
-// intermediates get odd row strides (P*n | 1, b*Q | 1 doubles) so rows read
-// by neighbouring threads fall into different LDS banks; their sizes are
-// a*b*(P*n+1) and a*n*(b*Q+1) (ds_t1_size / ds_t2_size).
+// ---------------------------------------------------------------------------
+// On-chip local matvec (every operand in LDS) on the FP64 matrix cores.
+//
+// Layouts: L (a, P, a'), R (b, Q, b'), the operator core stored PERMUTED as
+// A'(P, n, m, Q) (so that the second stage reads it with plain strides), v and
+// y (a, n, b).  The two intermediates use odd row strides (P*n | 1, b*Q | 1
+// doubles) so rows read by neighbouring lanes fall into different LDS banks:
+//   T1[a', b | P, n] = sum_a L[a, P, a'] v[a, n, b]           (a*b*(P*n|1))
+//   T2[a', m | b, Q] = sum_{P,n} T1[a', b | P, n] A'[P, n | m, Q]  (a*n*(b*Q|1))
+//   y[a', m, b']     = sum_{b,Q} T2[a', m | b, Q] R[b, Q, b']
 __host__ __device__ __forceinline__ int odd_stride(int s) { return s | 1; }
 __host__ __device__ __forceinline__ long ds_t1_size(int a, int P, int n, int b) {
   return (long)a * b * odd_stride(P * n);
@@ -427,30 +390,161 @@ __host__ __device__ __forceinline__ long ds_t1_size(int a, int P, int n, int b) 
 __host__ __device__ __forceinline__ long ds_t2_size(int a, int n, int Q, int b) {
   return (long)a * n * odd_stride(b * Q);
 }
-template <class T, class ACC>
-__device__ void local_matvec_ds(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* R,
-                                const T* v, T* y, T* t1, T* t2) {
-  const int a2 = a;
+
+// i -> (i / div) * hi + (i % div) * lo for 0 <= i < 2^22 and values below
+// 2^24, with full-rate arithmetic only (32-bit integer multiplies are
+// quarter rate on CDNA): the quotient is floor((i + 0.5) * (1/div)) in FP32
+// (the exact quotient lies at least 0.5/div from an integer, more than the
+// two FP32 roundings can move it while i + 0.5 < 2^22) and the products are
+// 24-bit multiplies.
+struct FMap {
+  float inv;
+  int div, hi, lo;
+};
+__device__ __forceinline__ FMap fmap(int div, int hi, int lo) { return FMap{1.0f / (float)div, div, hi, lo}; }
+__device__ __forceinline__ int fmap_at(const FMap& f, int i) {
+  const int q = (int)(((float)i + 0.5f) * f.inv);
+  return __mul24(q, f.hi) + __mul24(i - __mul24(q, f.div), f.lo);
+}
+
+// Output maps of the first two stages (computed once per local problem).
+struct MvPlan {
+  FMap c1r, c1c, c2r, c2c;
+};
+__device__ __forceinline__ MvPlan mv_plan(int a, int P, int n, int Q, int b) {
   const int s1 = odd_stride(P * n), s2 = odd_stride(b * Q);
-#if TTALS_MFMA_MATVEC
-  static_assert(sizeof(T) == 8, "matrix-core matvec is FP64");
-  // T1[a', b | P, n] (row stride s1)
-  mf_gemm_lds(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
-              mview<T>(t1, split(a2, n, b * s1), split(b, 1, s1)));
-  // T2[a', m | b, Q] (row stride s2)
-  mf_gemm_lds(a * b, n * Q, P * n, crm<T>(t1, s1), cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
-              mview<T>(t2, split(b, n * s2, Q), split(Q, s2, 1)));
-  mf_gemm_lds(a * n, b, b * Q, crm<T>(t2, s2), crm<T>(R, b), mrm<T>(y, b));
-#else
-  // T1[a', b | P, n] (row stride s1)
-  ds_gemm<T, ACC, true, true, true>(P * a2, n * b, a, cview<T>(L, plain(1), plain(P * a2)), crm<T>(v, n * b),
-                                    mview<T>(t1, split(a2, n, b * s1), split(b, 1, s1)));
-  // T2[a', m | b, Q] (row stride s2)
-  ds_gemm<T, ACC, true, true, true>(a * b, n * Q, P * n, crm<T>(t1, s1),
-                                    cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
-                                    mview<T>(t2, split(b, n * s2, Q), split(Q, s2, 1)));
-  ds_gemm<T, ACC, true, true, true>(a * n, b, b * Q, crm<T>(t2, s2), crm<T>(R, b), mrm<T>(y, b));
-#endif
+  return MvPlan{fmap(a, n, b * s1), fmap(b, 1, s1), fmap(b, n * s2, Q), fmap(Q, s2, 1)};
+}
+
+// The operator core (P, m, n, Q) copied into LDS as A'(P, n, m, Q).
+__device__ __forceinline__ void wg_copy_opcore_pnmq(const double* src, double* dst, int P, int n, int Q) {
+  const int tot = P * n * n * Q;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int q = e % Q, r1 = e / Q;
+    const int nn = r1 % n, r2 = r1 / n;
+    const int mm = r2 % n, p = r2 / n;
+    dst[((p * n + nn) * n + mm) * Q + q] = src[e];
+  }
+}
+
+// C[M,N] = A[M,K] B[K,N] with every operand in LDS and plain operand strides:
+//   A(i,k) = pa[i*ars + k*acs], B(k,j) = pb[k*brs + j*bcs],
+//   C(i,j) = pc[fmap_at(cr, i) + fmap_at(cc, j)].
+// 16x16 output tiles round-robin over the waves.  When there are at most
+// half as many tiles as waves and `red` holds (g-1)*tiles*256 doubles
+// (g = NWAVE / tiles), the K range of each tile is split over g waves and the
+// partial sums meet in `red` in a fixed order.  Operands are loaded four
+// K-steps ahead of their MFMAs; indices past M, N or K read a clamped
+// in-range address and contribute zero.
+__device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int acs, const double* B_, int brs,
+                            int bcs, double* C_, const FMap& cr, const FMap& cc, double* red_, int red_cap) {
+  M = __builtin_amdgcn_readfirstlane(M);
+  N = __builtin_amdgcn_readfirstlane(N);
+  K = __builtin_amdgcn_readfirstlane(K);
+  const TT_LDS double* pa = (const TT_LDS double*)A_;
+  const TT_LDS double* pb = (const TT_LDS double*)B_;
+  TT_LDS double* pc = (TT_LDS double*)C_;
+  TT_LDS double* red = (TT_LDS double*)red_;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  const int tn = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tn;
+  const int ks = (K + 3) >> 2;
+  int groups = 1;
+  if (2 * tiles <= NWAVE && NWAVE % tiles == 0) {  // then every wave gets exactly one (tile, part)
+    groups = NWAVE / tiles;
+    if ((groups - 1) * tiles * 256 > red_cap) groups = 1;
+  }
+  const int per = (ks + groups - 1) / groups;
+  const int acs4 = acs << 2, brs4 = brs << 2;
+  for (int u = w; u < tiles * groups; u += NWAVE) {
+    const int t = u % tiles, part = u / tiles;
+    const int ti = t / tn, tj = t - ti * tn;
+    const int i = (ti << 4) + li, j = (tj << 4) + li;
+    const bool iv = i < M, jv = j < N;
+    const int s0 = part * per, s1 = min(ks, s0 + per);
+    // Running operand pointers (no multiplies in the K loop).  Rows past M and
+    // columns past N read a clamped address: their outputs are discarded.  A
+    // K index past the end reads the last column/row instead and its B value
+    // is zeroed (A finite there, so the product vanishes).
+    const TT_LDS double* ap = pa + __mul24(iv ? i : M - 1, ars);
+    const TT_LDS double* bp = pb + __mul24(jv ? j : N - 1, bcs);
+    const TT_LDS double* alast = ap + __mul24(K - 1, acs);
+    const TT_LDS double* blast = bp + __mul24(K - 1, brs);
+    ap += __mul24((s0 << 2) + lk, acs);
+    bp += __mul24((s0 << 2) + lk, brs);
+    int k = (s0 << 2) + lk;
+    dx4_t acc = {0.0, 0.0, 0.0, 0.0};
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+      double av[4], bv[4];
+      bool kv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        kv[q] = k < K;
+        av[q] = (kv[q] ? ap : alast)[0];
+        bv[q] = (kv[q] ? bp : blast)[0];
+        ap += acs4;
+        bp += brs4;
+        k += 4;
+      }
+      // all eight loads in flight before the first MFMA waits on one
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], kv[q] ? bv[q] : 0.0, acc, 0, 0, 0);
+    }
+    for (; s < s1; ++s) {
+      const bool kq = k < K;
+      const double x = (kq ? ap : alast)[0];
+      const double y = (kq ? bp : blast)[0];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, kq ? y : 0.0, acc, 0, 0, 0);
+      ap += acs4;
+      bp += brs4;
+      k += 4;
+    }
+    if (groups > 1) {
+      // one work item per wave: partial parts park their sums, part 0 adds them
+      if (part > 0) {
+        TT_LDS double* r = red + ((part - 1) * tiles + t) * 256 + lane * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[e] = acc[e];
+      }
+      __syncthreads();
+      if (part > 0) break;
+      for (int p2 = 1; p2 < groups; ++p2) {
+        const TT_LDS double* r = red + ((p2 - 1) * tiles + t) * 256 + lane * 4;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] += r[e];
+      }
+    }
+    if (jv) {
+      const int cj = fmap_at(cc, j);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = (ti << 4) + lk + 4 * e;
+        if (row < M) pc[fmap_at(cr, row) + cj] = acc[e];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Kept out of line: its MFMA accumulators would otherwise be allocated next
+// to the Krylov solvers' registers and cost the second workgroup per CU.
+template <class T>
+__device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, int Q, int b, const T* L,
+                                                          const T* Ap, const T* R, const T* v, T* y, T* t1, T* t2,
+                                                          const MvPlan& pl) {
+  static_assert(sizeof(T) == 8, "the on-chip matvec is FP64");
+  const int s1 = odd_stride(P * n), s2 = odd_stride(b * Q);
+  const FMap plain_r = fmap(1, b, 0), plain_c = fmap(1, 1, 0);
+  // T1[(P,a'), (n,b)]: A = L viewed (P*a, a) with rows contiguous, B = v (a, n*b)
+  mf_gemm_lin(P * a, n * b, a, L, 1, P * a, v, n * b, 1, t1, pl.c1r, pl.c1c, nullptr, 0);
+  // T2[(a',b), (m,Q)]: A = T1 rows (stride s1), B = A' (P*n, n*Q)
+  mf_gemm_lin(a * b, n * Q, P * n, t1, s1, 1, Ap, n * Q, 1, t2, pl.c2r, pl.c2c, nullptr, 0);
+  // y[(a',m), b']: A = T2 rows (stride s2), B = R (b*Q, b); T1 is free: K-split scratch
+  mf_gemm_lin(a * n, b, b * Q, t2, s2, 1, R, b, 1, y, plain_r, plain_c, t1, (int)ds_t1_size(a, P, n, b));
 }
 
 // Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']

@@ -20,6 +20,7 @@ enum Status : int {
 struct SysDesc {
   int d, kind, solver, repeats;
   int ortho_x0, sym, gm_restart, maxit;
+  int adapt, pad1[3];         // adapt k > 0: evp local tolerance follows the eigenvalue change, capped at 1e-k
   double tol, sigma, conv_eps, pad0;
   const int* n;               // [d]
   int* rx;                    // [d+1] x ranks (in/out)
@@ -241,15 +242,54 @@ struct StepStats {
   double last_res = 0.0;
   long nsolve = 0;
   double flops = 0.0;  // algorithmic flops of the whole solve
+  double tol = 0.0;    // local Krylov tolerance of the current sweep
 };
 
-// Doubles of on-chip arena a local problem needs.
-__device__ __forceinline__ long arena_need(const SysDesc& D, int a, int P, int n, int Q, int b) {
+// Doubles of on-chip arena a local problem needs: the operator part
+// (L, R, A', two intermediates) plus `nvec` working vectors of length N.
+__host__ __device__ __forceinline__ long arena_need_vec(int a, int P, int n, int Q, int b, int nvec) {
   const long N = (long)a * n * b;
-  long need = (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + ds_t1_size(a, P, n, b) +
-              ds_t2_size(a, n, Q, b);
-  need += (D.kind == 0) ? 5 * N : 3 * N;
-  return need;
+  return (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + ds_t1_size(a, P, n, b) +
+         ds_t2_size(a, n, Q, b) + nvec * N;
+}
+__device__ __forceinline__ long arena_need(const SysDesc& D, int a, int P, int n, int Q, int b) {
+  return arena_need_vec(a, P, n, Q, b, D.kind == 0 ? 5 : 3);
+}
+
+// Copies the local operator of `op` into the arena `ar` (L, R, the operator
+// core as A'(P, n, m, Q), room for the two intermediates) and switches `op`
+// to the on-chip matvec.  Returns the first free double of the arena (room
+// for the working vectors).  No barrier.
+__device__ __forceinline__ double* onchip_operator(LocalOp<double>& op, double* ar) {
+  const int a = op.a, P = op.P, n = op.n, Q = op.Q, b = op.b;
+  double* Lc = ar;
+  double* Rc = Lc + (long)a * P * a;
+  double* Ac = Rc + (long)b * Q * b;
+  double* t1 = Ac + (long)P * n * n * Q;
+  double* t2 = t1 + ds_t1_size(a, P, n, b);
+  wg_copy(op.L, Lc, a * P * a);
+  wg_copy(op.R, Rc, b * Q * b);
+  wg_copy_opcore_pnmq(op.A, Ac, P, n, Q);
+  op = LocalOp<double>{a, P, n, Q, b, Lc, Ac, Rc, t1, t2, true, mv_plan(a, P, n, Q, b)};
+  return t2 + ds_t2_size(a, n, Q, b);
+}
+
+// Working vectors of an on-chip local solve: sle (CG/GMRES) 5 vectors, evp
+// (Lanczos) 3 vectors, from `vec` on.
+__device__ __forceinline__ void onchip_work(Work& W, double* vec, int N, bool sle) {
+  if (sle) {
+    W.f = vec;
+    W.x = vec + N;
+    W.r = vec + 2 * N;
+    W.p = vec + 3 * N;
+    W.q = vec + 4 * N;
+    W.cur = W.r;
+    W.w = W.p;
+  } else {
+    W.cur = vec;
+    W.w = vec + N;
+    W.r = vec + 2 * N;
+  }
 }
 
 // Solves the local problem at position i (warm start in D.g, solution left in
@@ -263,7 +303,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
   const int P = D.ra[i], Q = D.ra[i + 1];
   const int N = a * n * b;
-  LocalOp<double> op{a, P, n, Q, b, D.Ls + D.Loff[i], D.ac[i], D.Rs + D.Roff[i], D.t1, D.t2, false};
+  LocalOp<double> op{a, P, n, Q, b, D.Ls + D.Loff[i], D.ac[i], D.Rs + D.Roff[i], D.t1, D.t2, false, MvPlan{}};
   int its = 0;
   const double fmv = fl_op(a, P, n, Q, b);
   const bool dense = (D.kind == 0 && N <= NDENSE);
@@ -278,31 +318,11 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   W.V = D.V;
   const bool onchip = !dense && arena_need(D, a, P, n, Q, b) <= arena;
   if (onchip) {
-    double* ar = reinterpret_cast<double*>(lds);
-    double* Lc = ar;
-    double* Rc = Lc + (long)a * P * a;
-    double* Ac = Rc + (long)b * Q * b;
-    double* t1 = Ac + (long)P * n * n * Q;
-    double* t2 = t1 + ds_t1_size(a, P, n, b);
-    double* vec = t2 + ds_t2_size(a, n, Q, b);
-    wg_copy(op.L, Lc, a * P * a);
-    wg_copy(op.R, Rc, b * Q * b);
-    wg_copy(op.A, Ac, P * n * n * Q);
-    op = LocalOp<double>{a, P, n, Q, b, Lc, Ac, Rc, t1, t2, true};
+    double* vec = onchip_operator(op, reinterpret_cast<double*>(lds));
+    onchip_work(W, vec, N, D.kind == 0);
     if (D.kind == 0) {
-      W.f = vec;
-      W.x = vec + N;
-      W.r = vec + 2 * N;
-      W.p = vec + 3 * N;
-      W.q = vec + 4 * N;
-      W.cur = W.r;
-      W.w = W.p;
       wg_copy(D.f, W.f, N);
       wg_copy(D.g, W.x, N);
-    } else {
-      W.cur = vec;
-      W.w = vec + N;
-      W.r = vec + 2 * N;
     }
     __syncthreads();
   } else {
@@ -327,9 +347,9 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       st.flops += 3.0 * (double)N * N * P * Q + (2.0 / 3.0) * (double)N * N * N + 2.0 * (double)N * N;
     } else {
       int r = 1;
-      if (sym) r = onchip ? wg_cg<ACC, true>(op, W, D.tol, D.maxit, lds, S, &its)
-                          : wg_cg<ACC, false>(op, W, D.tol, D.maxit, lds, S, &its);
-      if (r == 1) r = wg_gmres<ACC>(op, W, D.tol, D.maxit, D.gm_restart, lds, S, &its);
+      if (sym) r = onchip ? wg_cg<ACC, true>(op, W, st.tol, D.maxit, lds, S, &its)
+                          : wg_cg<ACC, false>(op, W, st.tol, D.maxit, lds, S, &its);
+      if (r == 1) r = wg_gmres<ACC>(op, W, st.tol, D.maxit, D.gm_restart, lds, S, &its);
       if (r != 0) ++st.nunconv;
       // final true relative residual (diagnostic)
       apply_op<double, ACC>(op, W.x, W.w, lds);
@@ -348,9 +368,9 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
-    const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, D.tol, D.maxit, D.gm_restart, lds, S, lam,
+    const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, D.gm_restart, lds, S, lam,
                                                  &its)
-                         : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, D.tol, D.maxit, D.gm_restart, lds, S,
+                         : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, D.gm_restart, lds, S,
                                                   lam, &its);
     // matvecs + two Gram-Schmidt passes against on average half the basis
     st.flops += its * fmv + (double)its * 8.0 * N * 0.5 * (its < D.gm_restart ? its : D.gm_restart);
@@ -479,7 +499,19 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   for (int i = d - 2; i >= 0; --i) st.flops += build_right<ACC>(i, D, lds, arena);
 
   bool converged = false;
+  double lam1 = 0.0, lam2 = 0.0;  // eigenvalue after the last and the second-to-last sweep
+  st.tol = D.tol;
   while (sweeps < D.repeats && !converged && status == ST_OK) {
+    if (D.kind == 1 && D.adapt) {
+      // Inexact ALS: the local eigenproblems need only be solved to the
+      // accuracy the sweep itself has reached.  The eigenvalue error of a
+      // Lanczos pair with residual r is O(r^2), so the tolerance follows the
+      // square root of the last relative eigenvalue change, 1e-adapt at most.
+      const double cap = exp10(-(double)D.adapt);
+      double t = cap;
+      if (sweeps >= 2) t = fmin(cap, 0.1 * sqrt(fabs(lam1 - lam2) / fmax(fabs(lam1), 1e-300)));
+      st.tol = fmax(D.tol, t);
+    }
     // warm start for position 0: the current core 0
     wg_copy(D.xc[0], D.g, D.rx[0] * D.n[0] * D.rx[1]);
     __syncthreads();
@@ -501,6 +533,8 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     }
     if (status != ST_OK) break;
     ++sweeps;
+    lam2 = lam1;
+    lam1 = lam;
     if (D.kind == 1) {
       // best-so-far pair (strict improvement in |lambda - sigma|), then the
       // convergence test over the last <= 3 sweeps

@@ -40,11 +40,21 @@ def empty(*shape):
     return torch.empty(shape, dtype=torch.float64, device='cuda')
 
 
+@pytest.fixture(params=[62, 0], ids=['onchip', 'global'])
+def arena(request, env):
+    """Run a piece on the engine's on-chip (LDS, matrix-core) path where the
+    problem fits, or force the global (HBM/L2) path."""
+    lib, ctx, _ = env
+    assert lib.ttals_set_arena_kb(ctx, request.param) == 0
+    yield request.param
+    assert lib.ttals_set_arena_kb(ctx, 62) == 0
+
+
 SHAPES = [(1, 1, 2, 3, 2), (3, 2, 4, 3, 5), (5, 2, 8, 2, 5), (20, 3, 8, 2, 17), (16, 3, 2, 3, 16), (2, 4, 16, 4, 48)]
 
 
 @pytest.mark.parametrize('a,P,n,Q,b', SHAPES)
-def test_stacks_matvec_micro(env, a, P, n, Q, b):
+def test_stacks_matvec_micro(env, arena, a, P, n, Q, b):
     lib, ctx, _ = env
     rng = np.random.default_rng(a * 100 + b)
     L = rng.standard_normal((a, P, a))
@@ -104,7 +114,7 @@ def spd_local(rng, a, P, n, Q, b, shift=5.0):
 
 @pytest.mark.parametrize('a,P,n,Q,b,method', [(2, 2, 3, 2, 4, 1), (4, 2, 6, 2, 4, 1), (5, 2, 8, 2, 5, 2),
                                               (8, 3, 4, 3, 8, 2), (5, 2, 8, 2, 5, 3), (8, 3, 4, 3, 9, 3)])
-def test_local_solve(env, a, P, n, Q, b, method):
+def test_local_solve(env, arena, a, P, n, Q, b, method):
     lib, ctx, _ = env
     rng = np.random.default_rng(7 + method)
     L, Ac, R = spd_local(rng, a, P, n, Q, b)
@@ -136,7 +146,7 @@ def test_local_lu_singular_detected(env):
 
 @pytest.mark.parametrize('N_shape,mode', [((2, 2, 3, 2, 4), 0), ((4, 2, 6, 2, 4), 1), ((8, 3, 4, 3, 8), 0),
                                           ((8, 3, 4, 3, 8), 2), ((2, 3, 2, 3, 4), 2)])
-def test_local_eig(env, N_shape, mode):
+def test_local_eig(env, arena, N_shape, mode):
     lib, ctx, _ = env
     a, P, n, Q, b = N_shape
     rng = np.random.default_rng(11)
@@ -412,3 +422,26 @@ def test_from_host_batch_and_norm_leave_cores_untouched(env):
         assert abs(nrm - tt_ops.norm(cl)) <= 1e-13 * tt_ops.norm(cl)
         for b, dc in zip(before, t.device_cores):
             assert torch.equal(b, dc)
+
+
+def test_evp_adaptive_tolerance_matches_fixed(env):
+    """Inexact ALS (adaptive local tolerance) stays within 1e-10 of the
+    fixed-tolerance eigenvalue and needs fewer Krylov steps."""
+    from scikit_tt_amd.tensor_train import from_host_batch
+    import scikit_tt_amd.solvers.evp as evp
+    lists = []
+    for s in range(4):
+        H, y0, J, g = W.tfim_chain(seed=30 + s, d=16, rank=8)
+        lists += [H, W.right_orthonormal(y0)]
+    tts = from_host_batch(lists)
+    ops, xs = tts[0::2], tts[1::2]
+    kw = dict(repeats=6, conv_eps=0.0, solver='eig', sigma=-100.0, return_info=True)
+    fixed, fi = evp.als_batch(ops, xs, **kw)
+    adapt, ai = evp.als_batch(ops, xs, adaptive_tol=True, **kw)
+    for f, a_, i1, i2 in zip(fixed, adapt, fi, ai):
+        assert i1.status == 0 and i2.status == 0
+        assert abs(f[0] - a_[0]) <= 1e-10 * abs(f[0])
+    assert sum(i.krylov_iters for i in ai) < sum(i.krylov_iters for i in fi)
+    # the policy applies to that call only
+    again, _ = evp.als_batch(ops[:1], xs[:1], **kw)
+    assert again[0][0] == fixed[0][0]
