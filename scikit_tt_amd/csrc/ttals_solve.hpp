@@ -410,7 +410,7 @@ __device__ __forceinline__ int sturm_count_rows(double a_l, double b2_l, int m, 
 // tw: LDS scratch of 128 doubles.  Returns false (s untouched) if the
 // eigenvector overflowed.
 __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in, int idx_in, double* s, double* tw,
-                                    SolveLDS& S) {
+                                    SolveLDS& S, double hint_lo, double hint_hi) {
   const int m = __builtin_amdgcn_readfirstlane(m_in);
   const int idx = __builtin_amdgcn_readfirstlane(idx_in);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -425,6 +425,30 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   const double pad = 1e-12 * fmax(hi - lo, 1e-300) + 1e-300;
   lo -= pad;
   hi += pad;
+  // Warm start: a hint interval [hint_lo, hint_hi] (hint_lo < hint_hi) is
+  // searched first with NT points that include both ends, so the first round
+  // also tells whether the eigenvalue lies below or above it; the bracket then
+  // continues from whichever side holds it.
+  if (hint_lo < hint_hi && hint_lo > lo && hint_hi < hi) {
+    const double h = (hint_hi - hint_lo) / (double)(NT - 1);
+    const double x = (threadIdx.x == NT - 1) ? hint_hi : hint_lo + h * (double)threadIdx.x;
+    const int c = sturm_count_rows(a_l, b2_l, m, x);
+    const unsigned long long ball = __ballot(c <= idx);
+    int* cnt = S.cnt + NWAVE;
+    if (lane == 0) cnt[wv] = __popcll(ball);
+    __syncthreads();
+    int nle = 0;
+#pragma unroll
+    for (int q = 0; q < NWAVE; ++q) nle += cnt[q];
+    if (nle == 0) {
+      hi = hint_lo;  // below the hint
+    } else if (nle == NT) {
+      lo = hint_hi;  // above the hint
+    } else {
+      lo = hint_lo + h * (double)(nle - 1);
+      hi = (nle == NT - 1) ? hint_hi : hint_lo + h * (double)nle;
+    }
+  }
   for (int round = 0; round < 64; ++round) {
     if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
     const double h = (hi - lo) / (double)(NT + 1);
@@ -606,6 +630,8 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
   }
   int target = (mode == 1) ? 1 : 0;
   bool decided = (mode != 2);
+  double th_prev = 0.0, res_prev = 0.0;  // last check's Ritz value and residual estimate
+  bool have_hint = false;
   int status = 2;
   double th = 0.0;
   for (int restart = 0; restart < 1000; ++restart) {
@@ -623,6 +649,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
     double beta_prev = 0.0;
     int next_check = 2;
     bool have_vec = false;
+    int m_prev = 0;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
       long long t0 = ph_now();
@@ -775,7 +802,20 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         } else {
           // extreme eigenvalue: multisection to full precision, eigenvector
           // by twisted factorization, residual estimate bn |s_m|
-          have_vec = tri_extreme(S.alpha, S.beta, m, idx, S.s, S.H + (KMAX + 1), S);
+          // hint from the last check: by interlacing the extreme Ritz value only
+          // moves outward, and a residual bound puts it within res of the old one
+          double hlo = 0.0, hhi = -1.0;
+          if (have_hint) {
+            const double slack = 1.5 * res_prev + 1e-14 * fabs(th_prev) + 1e-300;
+            if (target == 0) {
+              hlo = th_prev - slack;
+              hhi = th_prev + 4e-16 * fabs(th_prev) + 1e-300;
+            } else {
+              hlo = th_prev - 4e-16 * fabs(th_prev) - 1e-300;
+              hhi = th_prev + slack;
+            }
+          }
+          have_vec = tri_extreme(S.alpha, S.beta, m, idx, S.s, S.H + (KMAX + 1), S, hlo, hhi);
           th = S.scal[1];
           if (!have_vec) {
             tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
@@ -786,7 +826,21 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         ph_add(S, PH_TRI, tc);
         const double goal = tol * fmax(fabs(th), 1e-3 * anorm);
         if (res <= goal) { conv = true; break; }
-        next_check = m + ((res > 1e6 * goal && m >= 10) ? 8 : 4);
+        // next check: where the residual's geometric decay since the last
+        // check reaches the goal (80 % of the way), 2..10 steps ahead
+        int step = 4;
+        if (have_hint && res < res_prev && m > m_prev && res > 0.0) {
+          const double rate = log(res_prev / res) / (double)(m - m_prev);
+          const double need = log(res / goal) / rate;
+          step = (int)fmin(10.0, fmax(2.0, floor(0.8 * need)));
+        } else if (res > 1e6 * goal && m >= 10) {
+          step = 8;
+        }
+        next_check = m + step;
+        th_prev = th;
+        res_prev = res;
+        m_prev = m;
+        have_hint = !full;
         if (*iters >= maxit) break;
       }
       const double ib = 1.0 / bn;
