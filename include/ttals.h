@@ -65,7 +65,8 @@ typedef struct ttals_result {
   double worst_residual;   /* sle: worst final relative residual of a local Krylov solve */
   double flops;            /* algorithmic flops of the solve (from shapes and iteration counts) */
   double cycles[7];        /* shader clock cycles: total, matvec, Gram-Schmidt, tridiagonal eigen,
-                              QR + warm start, stack updates, on-chip set-up (diagnostics) */
+                              QR + warm start, stack updates, on-chip set-up (diagnostics; only in
+                              a library built with -DTTALS_PROFILE=1, else 0) */
 } ttals_result;
 
 /* ---- context ------------------------------------------------------------ */

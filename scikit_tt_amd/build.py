@@ -31,22 +31,24 @@ def _resource_usage(text):
     return usage
 
 
-def build(verbose=False):
-    tmp = OUT + '.tmp'
-    cmd = [HIPCC] + FLAGS + ['-Rpass-analysis=kernel-resource-usage', '-o', tmp, SRC]
+def build(verbose=False, out=OUT, defines=()):
+    """Compile libttals.so; ``defines`` adds -D options (e.g. TTALS_PROFILE=1
+    for a build whose results carry per-phase cycle counters)."""
+    tmp = out + '.tmp'
+    cmd = [HIPCC] + FLAGS + ['-D' + d for d in defines] + ['-Rpass-analysis=kernel-resource-usage', '-o', tmp, SRC]
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError('hipcc failed building libttals.so')
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     # the batched engine is sized for two resident systems per CU
     for name, u in _resource_usage(r.stderr).items():
         if name.startswith('_Z5k_als') and u.get('Occupancy', 2) < 2:
             sys.stderr.write('warning: %s occupancy %d wave/SIMD (VGPRs %s, AGPRs %s): one system per CU\n'
                              % (name, u['Occupancy'], u.get('VGPRs'), u.get('AGPRs')))
-    return OUT
+    return out
 
 
 if __name__ == '__main__':

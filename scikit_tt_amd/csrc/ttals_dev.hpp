@@ -78,10 +78,30 @@ struct Mem<true> {
 // ---------------------------------------------------------------------------
 // reductions (fixed order: deterministic for a given shape)
 
+// DPP lane moves of a double (two 32-bit halves); every control used here
+// reads a lane of the same 16-lane row, so all lanes are valid sources.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// value of v in lane l (l uniform across the wave)
+__device__ __forceinline__ double rdlane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// Sum over the wave, returned in every lane.  Within each 16-lane row:
+// quad_perm [1,0,3,2] and [2,3,0,1] (quad sums), row_half_mirror (8-lane
+// sums), row_mirror (row sums) -- DPP moves, no LDS round trip; then the four
+// row sums are read back with v_readlane.  Fixed order: deterministic.
+// (About 200 cycles of latency against 530 for a shuffle tree.)
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
 __device__ __forceinline__ float wave_sumf(float v) {
 #pragma unroll
@@ -532,6 +552,36 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
 
 // Kept out of line: its MFMA accumulators would otherwise be allocated next
 // to the Krylov solvers' registers and cost the second workgroup per CU.
+// Stage 2 on the vector units when it is tiny (K = P*n and N = n*Q with
+// K*N <= 64, e.g. n = 2, P = Q = 3): one output row (a', b) per thread, the
+// row of T1 in registers, A' read as broadcasts.  A 16x16 matrix-core tile
+// would pad N = 6 to 16 and spend its time on tile bookkeeping.
+__device__ __forceinline__ void stage2_valu(int a, int P, int n, int Q, int b, const double* t1_, const double* Ap_,
+                                            double* t2_) {
+  const TT_LDS double* t1 = (const TT_LDS double*)t1_;
+  const TT_LDS double* Ap = (const TT_LDS double*)Ap_;
+  TT_LDS double* t2 = (TT_LDS double*)t2_;
+  const int K = P * n, N = n * Q;
+  const int s1 = odd_stride(K), s2 = odd_stride(b * Q);
+  const float invb = 1.0f / (float)b, invQ = 1.0f / (float)Q;
+  for (int r = threadIdx.x; r < a * b; r += NT) {
+    double x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = k < K ? t1[__mul24(r, s1) + k] : 0.0;
+    const int ar = (int)(((float)r + 0.5f) * invb);
+    const int base = __mul24(__mul24(ar, n), s2) + __mul24(r - __mul24(ar, b), Q);
+    for (int c = 0; c < N; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (k < K) acc = fma(x[k], Ap[__mul24(k, N) + c], acc);
+      const int mm = (int)(((float)c + 0.5f) * invQ);
+      t2[base + __mul24(mm, s2) + (c - __mul24(mm, Q))] = acc;
+    }
+  }
+  __syncthreads();
+}
+
 template <class T>
 __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, int Q, int b, const T* L,
                                                           const T* Ap, const T* R, const T* v, T* y, T* t1, T* t2,
@@ -542,7 +592,10 @@ __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, i
   // T1[(P,a'), (n,b)]: A = L viewed (P*a, a) with rows contiguous, B = v (a, n*b)
   mf_gemm_lin(P * a, n * b, a, L, 1, P * a, v, n * b, 1, t1, pl.c1r, pl.c1c, nullptr, 0);
   // T2[(a',b), (m,Q)]: A = T1 rows (stride s1), B = A' (P*n, n*Q)
-  mf_gemm_lin(a * b, n * Q, P * n, t1, s1, 1, Ap, n * Q, 1, t2, pl.c2r, pl.c2c, nullptr, 0);
+  if (P * n <= 16 && P * n * n * Q <= 64)
+    stage2_valu(a, P, n, Q, b, t1, Ap, t2);
+  else
+    mf_gemm_lin(a * b, n * Q, P * n, t1, s1, 1, Ap, n * Q, 1, t2, pl.c2r, pl.c2c, nullptr, 0);
   // y[(a',m), b']: A = T2 rows (stride s2), B = R (b*Q, b); T1 is free: K-split scratch
   mf_gemm_lin(a * n, b, b * Q, t2, s2, 1, R, b, 1, y, plain_r, plain_c, t1, (int)ds_t1_size(a, P, n, b));
 }

@@ -49,11 +49,29 @@ struct SolveLDS {
   int cnt[2 * NWAVE];               // per-wave ballot counts (double-buffered)
 };
 
-// phase cycle counters (thread 0 accumulates; read back as diagnostics)
+// Phase cycle counters (thread 0 accumulates; read back as diagnostics).
+// Reading the shader clock (s_memtime) costs about 600 cycles, several times
+// per Krylov step, so the counters are compiled in only with
+// -DTTALS_PROFILE=1; otherwise they read 0.
+#ifndef TTALS_PROFILE
+#define TTALS_PROFILE 0
+#endif
 enum Phase { PH_MATVEC = 0, PH_ORTH, PH_TRI, PH_QR, PH_STACK, PH_SETUP, PH_OTHER };
-__device__ __forceinline__ long long ph_now() { return (long long)clock64(); }
+__device__ __forceinline__ long long ph_now() {
+#if TTALS_PROFILE
+  return (long long)clock64();
+#else
+  return 0;
+#endif
+}
 __device__ __forceinline__ void ph_add(SolveLDS& S, int ph, long long t0) {
+#if TTALS_PROFILE
   if (threadIdx.x == 0) S.prof[ph] += ph_now() - t0;
+#else
+  (void)S;
+  (void)ph;
+  (void)t0;
+#endif
 }
 
 // Working vectors of a local solve.  V (global): (KMAX+1) x N basis;
@@ -349,13 +367,7 @@ __device__ __forceinline__ double rcp_nr(double x) {  // 1/x: hardware estimate 
   return r * (2.0 - x * r);
 }
 
-// value of v in lane l (l uniform across the wave)
-__device__ __forceinline__ double rdlane(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
+
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
