@@ -552,36 +552,6 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
 
 // Kept out of line: its MFMA accumulators would otherwise be allocated next
 // to the Krylov solvers' registers and cost the second workgroup per CU.
-// Stage 2 on the vector units when it is tiny (K = P*n and N = n*Q with
-// K*N <= 64, e.g. n = 2, P = Q = 3): one output row (a', b) per thread, the
-// row of T1 in registers, A' read as broadcasts.  A 16x16 matrix-core tile
-// would pad N = 6 to 16 and spend its time on tile bookkeeping.
-__device__ __forceinline__ void stage2_valu(int a, int P, int n, int Q, int b, const double* t1_, const double* Ap_,
-                                            double* t2_) {
-  const TT_LDS double* t1 = (const TT_LDS double*)t1_;
-  const TT_LDS double* Ap = (const TT_LDS double*)Ap_;
-  TT_LDS double* t2 = (TT_LDS double*)t2_;
-  const int K = P * n, N = n * Q;
-  const int s1 = odd_stride(K), s2 = odd_stride(b * Q);
-  const float invb = 1.0f / (float)b, invQ = 1.0f / (float)Q;
-  for (int r = threadIdx.x; r < a * b; r += NT) {
-    double x[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = k < K ? t1[__mul24(r, s1) + k] : 0.0;
-    const int ar = (int)(((float)r + 0.5f) * invb);
-    const int base = __mul24(__mul24(ar, n), s2) + __mul24(r - __mul24(ar, b), Q);
-    for (int c = 0; c < N; ++c) {
-      double acc = 0.0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k)
-        if (k < K) acc = fma(x[k], Ap[__mul24(k, N) + c], acc);
-      const int mm = (int)(((float)c + 0.5f) * invQ);
-      t2[base + __mul24(mm, s2) + (c - __mul24(mm, Q))] = acc;
-    }
-  }
-  __syncthreads();
-}
-
 template <class T>
 __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, int Q, int b, const T* L,
                                                           const T* Ap, const T* R, const T* v, T* y, T* t1, T* t2,
@@ -709,17 +679,27 @@ __device__ void wg_householder(double* W, int m, int c, double* tau, double* red
     if (threadIdx.x == 0) tau[j] = t;
     __syncthreads();
     if (t != 0.0) {
-      // apply H_j to columns j+1..c-1: one wave per column
-      for (int col = j + 1 + w; col < c; col += NWAVE) {
-        double s = 0.0;
+      // apply H_j to columns j+1..c-1: wave w takes columns j+1+w, +NWAVE, ...,
+      // four at a time so that their reductions overlap
+      for (int c0 = j + 1 + w; c0 < c; c0 += 4 * NWAVE) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = j + lane; i < m; i += 64) {
           const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-          s += v * W[(long)i * c + col];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int col = c0 + q * NWAVE;
+            if (col < c) s[q] += v * W[(long)i * c + col];
+          }
         }
-        s = wave_sum(s) * t;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
         for (int i = j + lane; i < m; i += 64) {
           const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-          W[(long)i * c + col] -= v * s;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int col = c0 + q * NWAVE;
+            if (col < c) W[(long)i * c + col] -= v * s[q];
+          }
         }
       }
     }
@@ -739,16 +719,25 @@ __device__ void wg_form_q(const double* W, int m, int c, const double* tau, doub
   for (int j = k - 1; j >= 0; --j) {
     const double t = tau[j];
     if (t != 0.0) {
-      for (int col = j + w; col < k; col += NWAVE) {
-        double s = 0.0;
+      for (int c0 = j + w; c0 < k; c0 += 4 * NWAVE) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = j + lane; i < m; i += 64) {
           const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-          s += v * Qm[(long)i * k + col];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int col = c0 + q * NWAVE;
+            if (col < k) s[q] += v * Qm[(long)i * k + col];
+          }
         }
-        s = wave_sum(s) * t;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
         for (int i = j + lane; i < m; i += 64) {
           const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-          Qm[(long)i * k + col] -= v * s;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int col = c0 + q * NWAVE;
+            if (col < k) Qm[(long)i * k + col] -= v * s[q];
+          }
         }
       }
     }
