@@ -550,6 +550,36 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
   __syncthreads();
 }
 
+// Stage 2 on the vector units when it is tiny (K = P*n and N = n*Q with
+// K*N <= 64, e.g. n = 2, P = Q = 3): one output row (a', b) per thread, the
+// row of T1 in registers, A' read as broadcasts.  A 16x16 matrix-core tile
+// would pad N = 6 to 16 and spend its time on tile bookkeeping.
+__device__ __forceinline__ void stage2_valu(int a, int P, int n, int Q, int b, const double* t1_, const double* Ap_,
+                                            double* t2_) {
+  const TT_LDS double* t1 = (const TT_LDS double*)t1_;
+  const TT_LDS double* Ap = (const TT_LDS double*)Ap_;
+  TT_LDS double* t2 = (TT_LDS double*)t2_;
+  const int K = P * n, N = n * Q;
+  const int s1 = odd_stride(K), s2 = odd_stride(b * Q);
+  const float invb = 1.0f / (float)b, invQ = 1.0f / (float)Q;
+  for (int r = threadIdx.x; r < a * b; r += NT) {
+    double x[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = k < K ? t1[__mul24(r, s1) + k] : 0.0;
+    const int ar = (int)(((float)r + 0.5f) * invb);
+    const int base = __mul24(__mul24(ar, n), s2) + __mul24(r - __mul24(ar, b), Q);
+    for (int c = 0; c < N; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        if (k < K) acc = fma(x[k], Ap[__mul24(k, N) + c], acc);
+      const int mm = (int)(((float)c + 0.5f) * invQ);
+      t2[base + __mul24(mm, s2) + (c - __mul24(mm, Q))] = acc;
+    }
+  }
+  __syncthreads();
+}
+
 // Kept out of line: its MFMA accumulators would otherwise be allocated next
 // to the Krylov solvers' registers and cost the second workgroup per CU.
 template <class T>
