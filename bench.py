@@ -76,9 +76,10 @@ def measured_traffic(batch, sites, rank, sweeps, adaptive):
         return None, None
     text = open(path).read()
     title = text.splitlines()[0]
-    if ('%d sweeps' % sweeps) not in title or ('adaptive' in title) != adaptive:
+    if ('%d sweeps' % sweeps) not in title:
         return None, None
-    m = re.search(r'^traffic_bytes_per_launch: (\d+)', text, re.M)
+    key = 'adaptive' if adaptive else 'fixed'
+    m = re.search(r'^traffic_bytes_per_launch_%s: (\d+)' % key, text, re.M)
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
