@@ -142,6 +142,54 @@ def check(rc, ctx=None, what='ttals'):
     raise RuntimeError('%s (code %d)' % (msg, rc))
 
 
+# ttals_tt as a numpy record (same layout as the C struct: int, pad, 4 pointers)
+TT_DTYPE = np.dtype([('order', '<i4'), ('pad', '<i4'), ('row_dims', '<u8'), ('col_dims', '<u8'),
+                     ('ranks', '<u8'), ('cores', '<u8')])
+assert TT_DTYPE.itemsize == ctypes.sizeof(TTDesc)
+assert [TT_DTYPE.fields[f][1] for f in ('order', 'row_dims', 'col_dims', 'ranks', 'cores')] == \
+    [getattr(TTDesc, f).offset for f in ('order', 'row_dims', 'col_dims', 'ranks', 'cores')]
+
+
+class DescBatch:
+    """ttals_tt[count] for many trains, built in bulk: one record array plus
+    flat arrays holding every train's dims, ranks and core pointers (a batch
+    of ctypes structures would cost several Python objects per train).
+    ``ranks_of(s)`` reads train s's ranks (the library writes final ranks of
+    outputs there).  Keep the object alive while the library may read it."""
+
+    def __init__(self, orders, rows, cols, ranks, ptrs):
+        self.orders = np.ascontiguousarray(orders, dtype=np.int32)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int32)
+        self.cols = np.ascontiguousarray(cols, dtype=np.int32)
+        self.ranks = np.ascontiguousarray(ranks, dtype=np.int32)
+        self.ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        d = self.orders.astype(np.int64)
+        self.doff = np.concatenate(([0], np.cumsum(d)[:-1]))
+        self.roff = np.concatenate(([0], np.cumsum(d + 1)[:-1]))
+        self.arr = np.zeros(len(d), dtype=TT_DTYPE)
+        self.arr['order'] = self.orders
+        self.arr['row_dims'] = self.rows.ctypes.data + 4 * self.doff
+        self.arr['col_dims'] = self.cols.ctypes.data + 4 * self.doff
+        self.arr['ranks'] = self.ranks.ctypes.data + 4 * self.roff
+        self.arr['cores'] = self.ptrs.ctypes.data + 8 * self.doff
+        self.ptr = ctypes.cast(self.arr.ctypes.data, ctypes.POINTER(TTDesc))
+
+    @classmethod
+    def from_tts(cls, tts):
+        orders, rows, cols, ranks, ptrs = [], [], [], [], []
+        for t in tts:
+            orders.append(t.order)
+            rows += t.row_dims
+            cols += t.col_dims
+            ranks += t.ranks
+            ptrs += t._core_ptrs()
+        return cls(orders, rows, cols, ranks, ptrs)
+
+    def ranks_of(self, s):
+        o = int(self.roff[s])
+        return [int(r) for r in self.ranks[o:o + int(self.orders[s]) + 1]]
+
+
 class DescPack:
     """Keeps the ctypes arrays behind a TTDesc alive."""
 

@@ -175,21 +175,21 @@ class Server:
             return
         kind, key = head.kind, head.key
         count = len(take)
-        _, A = _engine.descs([t.A for t in take])
-        _, X = _engine.descs([t.x0 for t in take])
+        dA, A = _engine.descs([t.A for t in take])
+        dX, X = _engine.descs([t.x0 for t in take])
         cores, opacks, O = _engine.outputs([t.x0 for t in take])
         stream = ctypes.c_void_p(slot.stream.cuda_stream)
         # the input tensors were produced on other streams: order the launch after them
         slot.stream.wait_stream(torch.cuda.current_stream(self.device))
         if kind == 'sle':
-            _, B = _engine.descs([t.b for t in take])
+            dB, B = _engine.descs([t.b for t in take])
             rc = self.lib.ttals_sle_submit(slot.ctx, stream, count, A, X, B, O, key[1], _SLE_SOLVERS[key[2]],
                                            self.tol, self.maxit)
-            keep = (A, X, B, O)
+            keep = (dA, dX, dB, A, X, B, O)  # descriptor owners stay alive until ttals_wait
         else:
             rc = self.lib.ttals_evp_submit(slot.ctx, stream, count, A, X, O, key[1], key[2], _EVP_SOLVERS[key[3]],
                                            key[4], self.tol, self.maxit)
-            keep = (A, X, O)
+            keep = (dA, dX, A, X, O)
         _lib.check(rc, slot.ctx, 'submit')
         now = time.perf_counter()
         for t in take:
@@ -206,7 +206,7 @@ class Server:
         now = time.perf_counter()
         for k, t in enumerate(take):
             info = _engine.SolveInfo(res[k])
-            x = _engine.finish(cores, k, opacks[k], t.x0)
+            x = _engine.finish(cores, k, opacks, t.x0)
             t.info = info
             t.result = x if t.kind == 'sle' else (info.eigenvalue, x, info.sweeps)
             t.t_done = now

@@ -39,37 +39,41 @@ def check_tts(*groups):
 
 
 def descs(tts):
-    packs = [t._desc() for t in tts]
-    return packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
+    """ttals_tt[] for the trains: (owner, pointer); keep the owner alive while the library reads it."""
+    b = _lib.DescBatch.from_tts(tts)
+    return b, b.ptr
 
 
 def outputs(initial_guesses):
     """Output cores with the capacity of the initial guesses (ranks never grow).
 
     One device allocation holds every output core of the batch; system s
-    gets the offsets ``slots[s]`` (in doubles) into it.
+    gets the offsets ``slots[s]`` (in doubles) into it.  Returns
+    ((buffer, slots), descriptor owner, descriptor pointer).
     """
     dev = initial_guesses[0].device
-    slots, total = [], 0
+    orders, rows, cols, ranks, slots = [], [], [], [], []
+    total = 0
     for x in initial_guesses:
         r = np.asarray(x.ranks, dtype=np.int64)
         sizes = r[:-1] * np.asarray(x.row_dims, dtype=np.int64) * np.asarray(x.col_dims, dtype=np.int64) * r[1:]
-        offs = total + np.concatenate(([0], np.cumsum(sizes)[:-1]))
-        slots.append(offs)
+        slots.append(total + np.concatenate(([0], np.cumsum(sizes)[:-1])))
         total += int(sizes.sum())
+        orders.append(x.order)
+        rows += x.row_dims
+        cols += x.col_dims
+        ranks += x.ranks
     buf = torch.empty(max(total, 1), dtype=torch.float64, device=dev)
-    base = buf.data_ptr()
-    packs = [_lib.DescPack(x.order, x.row_dims, x.col_dims, x.ranks, (base + 8 * offs).tolist())
-             for x, offs in zip(initial_guesses, slots)]
-    return (buf, slots), packs, (_lib.TTDesc * len(packs))(*[p.desc for p in packs])
+    ptrs = buf.data_ptr() + 8 * np.concatenate(slots).astype(np.uint64)
+    b = _lib.DescBatch(orders, rows, cols, ranks, ptrs)
+    return (buf, slots), b, b.ptr
 
 
-def finish(cores, s, pack, x0):
+def finish(cores, s, batch, x0):
     """System s's solution: the engine's in-place cores with their final ranks
     (each core at the start of its slot), as views of the batch buffer."""
     buf, slots = cores
-    ranks = pack.final_ranks()
-    return TT._from_flat(buf, slots[s], x0.row_dims, x0.col_dims, ranks)
+    return TT._from_flat(buf, slots[s], x0.row_dims, x0.col_dims, batch.ranks_of(s))
 
 
 def raise_status(info, what):

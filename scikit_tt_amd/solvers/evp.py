@@ -55,8 +55,8 @@ def als_batch(operators, initial_guesses, repeats=1, conv_eps=1e-10, solver='eig
         raise ValueError('operators and initial guesses must be non-empty lists of equal length')
     _engine.check_tts(operators, initial_guesses)
     dev = operators[0].device
-    _, A = _engine.descs(operators)
-    _, X = _engine.descs(initial_guesses)
+    dA, A = _engine.descs(operators)
+    dX, X = _engine.descs(initial_guesses)
     cores, opacks, O = _engine.outputs(initial_guesses)
     res = (_lib.Result * count)()
     ctx = _lib.context(dev.index)
@@ -74,7 +74,7 @@ def als_batch(operators, initial_guesses, repeats=1, conv_eps=1e-10, solver='eig
             _engine.raise_status(info, 'evp.als system %d' % s if count > 1 else 'evp.als')
     out = []
     for s in range(count):
-        t = _engine.finish(cores, s, opacks[s], initial_guesses[s])
+        t = _engine.finish(cores, s, opacks, initial_guesses[s])
         out.append((infos[s].eigenvalue, t, infos[s].sweeps))
     return (out, infos) if return_info else out
 

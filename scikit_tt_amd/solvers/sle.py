@@ -37,9 +37,9 @@ def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='s
         raise ValueError('operators, initial guesses and right-hand sides must be non-empty lists of equal length')
     _engine.check_tts(operators, initial_guesses, right_hand_sides)
     dev = operators[0].device
-    _, A = _engine.descs(operators)
-    _, X = _engine.descs(initial_guesses)
-    _, B = _engine.descs(right_hand_sides)
+    dA, A = _engine.descs(operators)
+    dX, X = _engine.descs(initial_guesses)
+    dB, B = _engine.descs(right_hand_sides)
     cores, opacks, O = _engine.outputs(initial_guesses)
     res = (_lib.Result * count)()
     ctx = _lib.context(dev.index)
@@ -50,7 +50,7 @@ def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='s
     if not return_info:
         for s, info in enumerate(infos):
             _engine.raise_status(info, 'sle.als system %d' % s if count > 1 else 'sle.als')
-    sols = [_engine.finish(cores, s, opacks[s], initial_guesses[s]) for s in range(count)]
+    sols = [_engine.finish(cores, s, opacks, initial_guesses[s]) for s in range(count)]
     return (sols, infos) if return_info else sols
 
 
