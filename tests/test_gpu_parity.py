@@ -445,3 +445,38 @@ def test_evp_adaptive_tolerance_matches_fixed(env):
     # the policy applies to that call only
     again, _ = evp.als_batch(ops[:1], xs[:1], **kw)
     assert again[0][0] == fixed[0][0]
+
+
+def test_sle_laplace_large_full_size(env):
+    """C3 at its real size (d=20, n=32, TT rank 32, N = 32768 per interior
+    local problem, global-memory path): two sweeps reach rel.res 2e-9
+    (the rank-32 approximation levels off near 1e-9)."""
+    from scikit_tt_amd.tensor_train import from_host_batch, residual_error
+    import scikit_tt_amd.solvers.sle as sle
+    A, x0, b = W.laplace_large()
+    tA, tx, tb = from_host_batch([A, x0, b])
+    (x,), (info,) = sle.als_batch([tA], [tx], [tb], repeats=2, return_info=True)
+    assert info.status == 0 and info.unconverged == 0
+    assert x.ranks == [1] + [32] * 19 + [1]
+    rr = residual_error(tA, x, tb) / tb.norm()
+    assert rr <= 2e-9, rr
+
+
+def test_sle_uneven_batch_full_size(env):
+    """C4 at its real size: 512 uneven shifted-identity systems in one launch.
+    Every system succeeds, the residual falls, and sample systems give the
+    same cores alone as in the batch."""
+    from scikit_tt_amd.tensor_train import from_host_batch, residual_error
+    import scikit_tt_amd.solvers.sle as sle
+    insts = [W.shifted_batch_instance(s) for s in range(512)]
+    tts = from_host_batch([c for inst in insts for c in inst])
+    As, xs, bs = tts[0::3], tts[1::3], tts[2::3]
+    sols, infos = sle.als_batch(As, xs, bs, repeats=2, return_info=True)
+    assert all(i.status == 0 for i in infos)
+    for s in (0, 17, 255, 511):
+        r0 = residual_error(As[s], xs[s], bs[s])
+        r1 = residual_error(As[s], sols[s], bs[s])
+        assert r1 < 1e-2 * r0, (s, r0, r1)
+        (alone,), _ = sle.als_batch([As[s]], [xs[s]], [bs[s]], repeats=2, return_info=True)
+        for c1, c2 in zip(alone.device_cores, sols[s].device_cores):
+            assert torch.equal(c1, c2)
