@@ -151,28 +151,48 @@ assert [TT_DTYPE.fields[f][1] for f in ('order', 'row_dims', 'col_dims', 'ranks'
 
 
 class DescBatch:
-    """ttals_tt[count] for many trains, built in bulk: one record array plus
-    flat arrays holding every train's dims, ranks and core pointers (a batch
-    of ctypes structures would cost several Python objects per train).
-    ``ranks_of(s)`` reads train s's ranks (the library writes final ranks of
-    outputs there).  Keep the object alive while the library may read it."""
+    """ttals_tt[count] for many trains, built in bulk.
+
+    One byte buffer holds the record array (same layout as the C struct)
+    followed by every train's dims, ranks and core pointers; ``self.c`` is a
+    ctypes ``ttals_tt`` array created over that buffer, so whatever holds
+    ``self.c`` (or this object) keeps all of it alive.  ``ranks_of(s)`` reads
+    train s's ranks (the library writes final ranks of outputs there).
+    """
 
     def __init__(self, orders, rows, cols, ranks, ptrs):
-        self.orders = np.ascontiguousarray(orders, dtype=np.int32)
-        self.rows = np.ascontiguousarray(rows, dtype=np.int32)
-        self.cols = np.ascontiguousarray(cols, dtype=np.int32)
-        self.ranks = np.ascontiguousarray(ranks, dtype=np.int32)
-        self.ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64)
-        d = self.orders.astype(np.int64)
-        self.doff = np.concatenate(([0], np.cumsum(d)[:-1]))
-        self.roff = np.concatenate(([0], np.cumsum(d + 1)[:-1]))
-        self.arr = np.zeros(len(d), dtype=TT_DTYPE)
-        self.arr['order'] = self.orders
-        self.arr['row_dims'] = self.rows.ctypes.data + 4 * self.doff
-        self.arr['col_dims'] = self.cols.ctypes.data + 4 * self.doff
-        self.arr['ranks'] = self.ranks.ctypes.data + 4 * self.roff
-        self.arr['cores'] = self.ptrs.ctypes.data + 8 * self.doff
-        self.ptr = ctypes.cast(self.arr.ctypes.data, ctypes.POINTER(TTDesc))
+        orders = np.asarray(orders, dtype=np.int64)
+        n = len(orders)
+        nd, nr = int(orders.sum()), int((orders + 1).sum())
+        rec = n * TT_DTYPE.itemsize
+        o_rows = rec
+        o_cols = o_rows + 4 * nd
+        o_ranks = o_cols + 4 * nd
+        o_ptrs = (o_ranks + 4 * nr + 7) // 8 * 8
+        self.blob = np.zeros(o_ptrs + 8 * nd, dtype=np.uint8)
+        arr = self.blob[:rec].view(TT_DTYPE)
+        self.rows = self.blob[o_rows:o_cols].view(np.int32)
+        self.cols = self.blob[o_cols:o_ranks].view(np.int32)
+        self.ranks = self.blob[o_ranks:o_ranks + 4 * nr].view(np.int32)
+        self.ptrs = self.blob[o_ptrs:].view(np.uint64)
+        self.rows[:] = rows
+        self.cols[:] = cols
+        self.ranks[:] = ranks
+        self.ptrs[:] = ptrs
+        self.orders = orders
+        self.doff = np.concatenate(([0], np.cumsum(orders)[:-1])).astype(np.int64)
+        self.roff = np.concatenate(([0], np.cumsum(orders + 1)[:-1])).astype(np.int64)
+        base = self.blob.ctypes.data
+        arr['order'] = orders
+        arr['row_dims'] = base + o_rows + 4 * self.doff
+        arr['col_dims'] = base + o_cols + 4 * self.doff
+        arr['ranks'] = base + o_ranks + 4 * self.roff
+        arr['cores'] = base + o_ptrs + 8 * self.doff
+        self.c = (TTDesc * n).from_buffer(self.blob)
+
+    @property
+    def ptr(self):
+        return self.c
 
     @classmethod
     def from_tts(cls, tts):

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       break;
     case P_MATVEC:
       if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 2) <= A.arena) {
-        LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
+        LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
         double* vec = onchip_operator(op, dyn_lds);
         wg_copy(A.x, vec, (int)N);
         __syncthreads();
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       micro_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.t1, lds);
       break;
     case P_SOLVE: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
       int its = 0, r = 0;
       int method = A.method;
       if (method == 0) method = (N <= NDENSE) ? 1 : (A.sym ? 2 : 3);
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       break;
     }
     case P_EIG: {
-      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false, MvPlan{}};
+      LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
       Work W;
       W.V = A.V;
       int its = 0, r;

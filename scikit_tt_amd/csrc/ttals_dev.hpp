@@ -584,8 +584,10 @@ __device__ __forceinline__ void stage2_valu(int a, int P, int n, int Q, int b, c
 // to the Krylov solvers' registers and cost the second workgroup per CU.
 template <class T>
 __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, int Q, int b, const T* L,
-                                                          const T* Ap, const T* R, const T* v, T* y, T* t1, T* t2,
-                                                          const MvPlan& pl) {
+                                                          const T* Ap, const T* R, const T* v, T* y, T* t1, T* t2) {
+  // the output maps are rebuilt here (four FP32 reciprocals) rather than
+  // passed in: a plan passed by reference would be read from the stack
+  const MvPlan pl = mv_plan(a, P, n, Q, b);
   static_assert(sizeof(T) == 8, "the on-chip matvec is FP64");
   const int s1 = odd_stride(P * n), s2 = odd_stride(b * Q);
   const FMap plain_r = fmap(1, b, 0), plain_c = fmap(1, 1, 0);

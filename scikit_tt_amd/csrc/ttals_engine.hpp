@@ -270,7 +270,7 @@ __device__ __forceinline__ double* onchip_operator(LocalOp<double>& op, double* 
   wg_copy(op.L, Lc, a * P * a);
   wg_copy(op.R, Rc, b * Q * b);
   wg_copy_opcore_pnmq(op.A, Ac, P, n, Q);
-  op = LocalOp<double>{a, P, n, Q, b, Lc, Ac, Rc, t1, t2, true, mv_plan(a, P, n, Q, b)};
+  op = LocalOp<double>{a, P, n, Q, b, Lc, Ac, Rc, t1, t2, true};
   return t2 + ds_t2_size(a, n, Q, b);
 }
 
@@ -303,7 +303,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
   const int P = D.ra[i], Q = D.ra[i + 1];
   const int N = a * n * b;
-  LocalOp<double> op{a, P, n, Q, b, D.Ls + D.Loff[i], D.ac[i], D.Rs + D.Roff[i], D.t1, D.t2, false, MvPlan{}};
+  LocalOp<double> op{a, P, n, Q, b, D.Ls + D.Loff[i], D.ac[i], D.Rs + D.Roff[i], D.t1, D.t2, false};
   int its = 0;
   const double fmv = fl_op(a, P, n, Q, b);
   const bool dense = (D.kind == 0 && N <= NDENSE);

@@ -22,14 +22,13 @@ struct LocalOp {
   T* t1;
   T* t2;
   bool ds;    // operands on-chip (operator core stored as (P, n, m, Q))
-  MvPlan pl;  // output maps of the on-chip matvec
   __device__ int size() const { return a * n * b; }
 };
 
 template <class T, class ACC>
 __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y, ACC* lds) {
   if (op.ds)
-    local_matvec_mf<T>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, op.pl);
+    local_matvec_mf<T>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2);
   else
     local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);
 }

@@ -39,7 +39,7 @@ def check_tts(*groups):
 
 
 def descs(tts):
-    """ttals_tt[] for the trains: (owner, pointer); keep the owner alive while the library reads it."""
+    """ttals_tt[] for the trains: (DescBatch, ctypes array to pass); the array owns all descriptor memory."""
     b = _lib.DescBatch.from_tts(tts)
     return b, b.ptr
 
