@@ -83,7 +83,7 @@ def measured_traffic(batch, sites, rank, sweeps, adaptive):
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
-def serving_phase(rate, seconds, seed=7):
+def serving_phase(rate, seconds, seed=7, adaptive=True):
     """Poisson arrivals of mixed evp (TFIM, d 8..32, rank 4..16 log-uniform) and
     sle (-Laplacian, d 6..16, n 4/8, rank 4..8) requests into the
     continuous-batching server; returns latency percentiles and throughput."""
@@ -115,7 +115,7 @@ def serving_phase(rate, seconds, seed=7):
         while time.perf_counter() < next_t:
             srv.poll()
         if kind == 'evp':
-            srv.submit_evp(A, x0, repeats=4, conv_eps=1e-10, solver='eig', sigma=-100.0)
+            srv.submit_evp(A, x0, repeats=4, conv_eps=1e-10, solver='eig', sigma=-100.0, adaptive_tol=adaptive)
         else:
             srv.submit_sle(A, x0, b, repeats=2)
         srv.poll()
@@ -263,7 +263,7 @@ def main():
         line['serving'] = []
         for rate in [float(r) for r in args.serve_rates.split(',') if r]:
             print('[bench] serving at %.0f req/s' % rate, file=sys.stderr, flush=True)
-            line['serving'].append(serving_phase(rate, args.serve_seconds))
+            line['serving'].append(serving_phase(rate, args.serve_seconds, adaptive=not args.fixed_tol))
     if not args.no_cpu:
         print('[bench] cpu baseline', file=sys.stderr, flush=True)
         line['cpu_baseline'] = cpu_baseline(raw[0], args.sweeps)

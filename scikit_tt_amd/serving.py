@@ -24,6 +24,7 @@ import torch
 
 from . import _lib
 from .solvers import _engine
+from .solvers import evp as evp_mod
 from .tensor_train import TT
 
 __all__ = ['Server', 'footprint_bytes']
@@ -123,11 +124,14 @@ class Server:
         _engine.check_tts([A, x0, b])
         return self._enqueue('sle', ('sle', int(repeats), solver), A, x0, b)
 
-    def submit_evp(self, A, x0, repeats=1, conv_eps=1e-10, solver='eig', sigma=1.0):
+    def submit_evp(self, A, x0, repeats=1, conv_eps=1e-10, solver='eig', sigma=1.0, adaptive_tol=False):
+        """Queue an evp.als solve; ``adaptive_tol`` as in ``evp.als_batch``
+        (requests are batched only with requests of the same options)."""
         if solver not in _EVP_SOLVERS:
             raise NotImplementedError("solver must be 'eig' or 'eigh'")
         _engine.check_tts([A, x0])
-        return self._enqueue('evp', ('evp', int(repeats), float(conv_eps), solver, float(sigma)), A, x0, None)
+        key = ('evp', int(repeats), float(conv_eps), solver, float(sigma), bool(adaptive_tol))
+        return self._enqueue('evp', key, A, x0, None)
 
     def _enqueue(self, kind, key, A, x0, b):
         nb = footprint_bytes(kind, A, x0, b)
@@ -187,6 +191,8 @@ class Server:
                                            self.tol, self.maxit)
             keep = (dA, dX, dB, A, X, B, O)  # descriptor owners stay alive until ttals_wait
         else:
+            policy = evp_mod.ADAPTIVE_CAP_EXP if key[5] else 0
+            _lib.check(self.lib.ttals_set_local_tol_policy(slot.ctx, policy), slot.ctx, 'tol policy')
             rc = self.lib.ttals_evp_submit(slot.ctx, stream, count, A, X, O, key[1], key[2], _EVP_SOLVERS[key[3]],
                                            key[4], self.tol, self.maxit)
             keep = (dA, dX, A, X, O)
