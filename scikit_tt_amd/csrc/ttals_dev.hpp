@@ -198,12 +198,30 @@ __device__ __forceinline__ void wg_fill(T* y, T v, int n) {
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 constexpr int GEMM_LDS_DOUBLES = GBK * (GBM + 1) + GBK * (GBN + 1);
 
+// i -> (i / div) * hi + (i % div) * lo for 0 <= i < 2^22 with an FP32
+// reciprocal quotient (exact there: the quotient sits at least 0.5/div from an
+// integer) and ordinary 32-bit products (offsets may exceed 2^24; a system's
+// arrays stay below 2^31 elements).
+struct GMap {
+  float inv;
+  int div, hi, lo;
+};
+__device__ __forceinline__ GMap gmap(const IMap& m) {
+  const int div = m.div < (1 << 22) ? m.div : (1 << 22);  // plain maps: quotient 0 below 2^22
+  return GMap{1.0f / (float)div, div, m.hi, m.lo};
+}
+__device__ __forceinline__ int gmap_at(const GMap& g, int i) {
+  const int q = (int)(((float)i + 0.5f) * g.inv);
+  return q * g.hi + (i - q * g.div) * g.lo;
+}
+
 template <class T, class ACC>
 __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
   ACC* As = lds;                      // [GBK][GBM+1]
   ACC* Bs = lds + GBK * (GBM + 1);    // [GBK][GBN+1]
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
+  const GMap Ar = gmap(A.r), Ac = gmap(A.c), Br = gmap(B.r), Bc = gmap(B.c);
   // contiguous direction of each operand decides the load order (coalescing)
   const bool a_kfast = (A.c.lo == 1 && A.c.div >= K);
   const bool b_nfast = (B.c.lo == 1 && B.c.div >= N);
@@ -219,13 +237,13 @@ __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C,
           int mi, ki;
           if (a_kfast) { ki = e % GBK; mi = e / GBK; } else { mi = e % GBM; ki = e / GBM; }
           const int gm = m0 + mi, gk = k0 + ki;
-          As[ki * (GBM + 1) + mi] = (gm < M && gk < K) ? (ACC)A.at(gm, gk) : (ACC)0;
+          As[ki * (GBM + 1) + mi] = (gm < M && gk < K) ? (ACC)A.p[gmap_at(Ar, gm) + gmap_at(Ac, gk)] : (ACC)0;
         }
         for (int e = tid; e < GBN * GBK; e += NT) {
           int ni, ki;
           if (b_nfast) { ni = e % GBN; ki = e / GBN; } else { ki = e % GBK; ni = e / GBK; }
           const int gn = n0 + ni, gk = k0 + ki;
-          Bs[ki * (GBN + 1) + ni] = (gn < N && gk < K) ? (ACC)B.at(gk, gn) : (ACC)0;
+          Bs[ki * (GBN + 1) + ni] = (gn < N && gk < K) ? (ACC)B.p[gmap_at(Br, gk) + gmap_at(Bc, gn)] : (ACC)0;
         }
         __syncthreads();
         const int kmax = (K - k0) < GBK ? (K - k0) : GBK;
@@ -242,14 +260,16 @@ __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C,
         }
         __syncthreads();
       }
+      const GMap Cr = gmap(C.r), Cc = gmap(C.c);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + ty + 16 * r;
         if (gm >= M) continue;
+        const int cr = gmap_at(Cr, gm);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
           const int gn = n0 + tx + 16 * c;
-          if (gn < N) C.at(gm, gn) = (T)acc[r][c];
+          if (gn < N) C.p[cr + gmap_at(Cc, gn)] = (T)acc[r][c];
         }
       }
     }

@@ -195,13 +195,18 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
     wg_sum3(rr, d1, d2, S.red);
     if (sqrt(rr) <= tol * fn) return 0;
     while (*iters < maxit) {
+      long long t0 = ph_now();
       apply_op<double, ACC>(op, p, q, lds);
+      ph_add(S, PH_MATVEC, t0);
+      t0 = ph_now();
       double pq = 0.0;
+#pragma unroll 8
       for (int e = threadIdx.x; e < N; e += NT) pq += M::ld(p, e) * M::ld(q, e);
       wg_sum3(pq, d1, d2, S.red);
       if (!(pq > 0.0)) return 1;
       const double al = rr / pq;
       double rn = 0.0;
+#pragma unroll 4
       for (int e = threadIdx.x; e < N; e += NT) {
         M::st(x, e, M::ld(x, e) + al * M::ld(p, e));
         const double re = M::ld(r, e) - al * M::ld(q, e);
@@ -213,8 +218,10 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
       if (sqrt(rn) <= tol * fn) break;
       const double be = rn / rr;
       rr = rn;
+#pragma unroll 8
       for (int e = threadIdx.x; e < N; e += NT) M::st(p, e, M::ld(r, e) + be * M::ld(p, e));
       __syncthreads();
+      ph_add(S, PH_ORTH, t0);
     }
     // true residual check (the recurrence drifts)
     apply_op<double, ACC>(op, x, q, lds);
