@@ -172,6 +172,10 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       double* W = A.t1;
       for (long e = threadIdx.x; e < (long)m * c; e += NT) W[e] = A.x[e];
       __syncthreads();
+      if (m <= 64 && c <= 64) {  // the engine's small-panel path
+        wg_qr_small(W, m, c, A.t2, A.out, A.v);
+        break;
+      }
       wg_householder(W, m, c, A.t2, S.red);
       wg_form_q(W, m, c, A.t2, A.out);
       const int k = m < c ? m : c;

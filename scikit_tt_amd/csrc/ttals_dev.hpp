@@ -759,6 +759,90 @@ __device__ void wg_householder(double* W, int m, int c, double* tau, double* red
   }
 }
 
+// Householder QR of a small panel (m <= 64 rows, c <= 64 columns): rows in
+// lanes, wave w keeps columns w, w+NWAVE, ... (up to 16) in registers.  Per
+// column one barrier (the owner publishes the updated column j to W); every
+// wave builds the reflector from it with DPP reductions and applies it to its
+// own columns, their dot products reduced together.  Same reflectors and sign
+// conventions as wg_householder (LAPACK: beta = -sign(alpha) |x|); on return
+// W holds R above and the reflectors below the diagonal, tau the scalars,
+// Qm (m x k, ld k) the explicit Q and Rm (k x c) the clean R.
+__device__ void wg_qr_small(double* W, int m, int c, double* tau, double* Qm, double* Rm) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k = m < c ? m : c;
+  const bool rv = lane < m;
+  double col[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int jj = w + NWAVE * u;
+    col[u] = (rv && jj < c) ? W[lane * c + jj] : 0.0;
+  }
+  for (int j = 0; j < k; ++j) {
+    const int own = j % NWAVE;
+    if (w == own) {
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (w + NWAVE * u == j && rv) W[lane * c + j] = col[u];
+    }
+    __syncthreads();
+    const double x = (rv && lane >= j) ? W[lane * c + j] : 0.0;
+    const double alpha = rdlane(x, j);
+    const double s2 = wave_sum(lane > j ? x * x : 0.0);
+    double t = 0.0, beta = alpha, sc = 0.0;
+    if (s2 != 0.0) {
+      const double nrm = sqrt(alpha * alpha + s2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    const double v = (lane == j) ? 1.0 : (lane > j ? x * sc : 0.0);
+    __syncthreads();  // everyone has read column j before its owner rewrites it
+    if (w == own && rv && lane >= j) W[lane * c + j] = (lane == j) ? beta : v;
+    if (threadIdx.x == 0) tau[j] = t;
+    if (t != 0.0) {
+      double p[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) p[u] = (w + NWAVE * u > j && w + NWAVE * u < c) ? v * col[u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (w + NWAVE * u > j && w + NWAVE * u < c) col[u] -= v * (wave_sum(p[u]) * t);
+    }
+  }
+  // columns k..c-1 (wide panels) are R rows only: publish them
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int jj = w + NWAVE * u;
+    if (jj >= k && jj < c && rv) W[lane * c + jj] = col[u];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * c; e += NT) {
+    const int i = e / c, jj = e - (e / c) * c;
+    Rm[e] = (jj >= i) ? W[i * c + jj] : 0.0;
+  }
+  // explicit Q by backward accumulation; wave w owns Q columns w, w+NWAVE, ...
+  // (H_j leaves columns < j untouched), so no barrier is needed
+  double q[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) q[u] = (lane == w + NWAVE * u) ? 1.0 : 0.0;
+  for (int j = k - 1; j >= 0; --j) {
+    const double t = tau[j];
+    if (t == 0.0) continue;
+    const double v = (lane == j) ? 1.0 : ((lane > j && rv) ? W[lane * c + j] : 0.0);
+    double p[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) p[u] = (w + NWAVE * u >= j && w + NWAVE * u < k) ? v * q[u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (w + NWAVE * u >= j && w + NWAVE * u < k) q[u] -= v * (wave_sum(p[u]) * t);
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int qq = w + NWAVE * u;
+    if (qq < k && rv) Qm[lane * k + qq] = q[u];
+  }
+  __syncthreads();
+}
+
 // Explicit Q (m x k, row-major, ld = k) from the reflectors in W.
 __device__ void wg_form_q(const double* W, int m, int c, const double* tau, double* Qm) {
   const int k = m < c ? m : c;

@@ -96,9 +96,13 @@ __device__ int qr_factor(const double* v, int m, int c, bool transposed, const Q
     }
   }
   __syncthreads();
+  const int k = m < c ? m : c;
+  if (m <= 64 && c <= 64) {
+    wg_qr_small(W, m, c, B.tau, B.Q, B.R);
+    return k;
+  }
   wg_householder(W, m, c, B.tau, S.red);
   wg_form_q(W, m, c, B.tau, B.Q);
-  const int k = m < c ? m : c;
   for (long e = threadIdx.x; e < (long)k * c; e += NT) {
     const int i = (int)(e / c), j = (int)(e % c);
     B.R[e] = (j >= i) ? W[(long)i * c + j] : 0.0;

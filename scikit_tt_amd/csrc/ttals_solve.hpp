@@ -668,6 +668,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
     int next_check = 2;
     bool have_vec = false;
     int m_prev = 0;
+    double anorm = 0.0;
     const int kk = kmax < N ? kmax : N;
     for (int j = 0; j < kk; ++j) {
       long long t0 = ph_now();
@@ -730,9 +731,8 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       ph_add(S, PH_ORTH, t0);
       ++*iters;
       m = j + 1;
-      double anorm = 0.0;
-      for (int i = 0; i <= j; ++i)
-        anorm = fmax(anorm, fabs(S.alpha[i]) + (i > 0 ? fabs(S.beta[i - 1]) : 0.0));
+      // running max of |alpha_i| + |beta_{i-1}| (rows of T are final once written)
+      anorm = fmax(anorm, fabs(S.alpha[j]) + (j > 0 ? fabs(S.beta[j - 1]) : 0.0));
       // breakdown: the residual is at the rounding level of A v_j
       const bool brk = !(bn > 1e-12 * fmax(w0, 1e-300));
       full = (m == N);

@@ -184,7 +184,8 @@ def test_local_eig_nearest_interior_small(env):
     assert abs(lam.value - w[np.argmin(np.abs(w - sigma))]) < 1e-9 * np.abs(w).max()
 
 
-@pytest.mark.parametrize('m,c', [(12, 5), (5, 12), (200, 48), (7, 7), (1, 9), (9, 1)])
+@pytest.mark.parametrize('m,c', [(12, 5), (5, 12), (200, 48), (7, 7), (1, 9), (9, 1), (32, 16), (64, 64),
+                                 (64, 40), (30, 64), (65, 16)])
 def test_qr(env, m, c):
     lib, ctx, _ = env
     Y = np.random.default_rng(m * c).standard_normal((m, c))
