@@ -79,11 +79,12 @@ struct Mem<true> {
 // reductions (fixed order: deterministic for a given shape)
 
 // DPP lane moves of a double (two 32-bit halves); every control used here
-// reads a lane of the same 16-lane row, so all lanes are valid sources.
+// reads a lane of the same 16-lane row, so all lanes are valid sources and
+// no "old" value is needed (none is materialised: two instructions a move).
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 // value of v in lane l (l uniform across the wave)
@@ -759,88 +760,167 @@ __device__ void wg_householder(double* W, int m, int c, double* tau, double* red
   }
 }
 
-// Householder QR of a small panel (m <= 64 rows, c <= 64 columns): rows in
-// lanes, wave w keeps columns w, w+NWAVE, ... (up to 16) in registers.  Per
-// column one barrier (the owner publishes the updated column j to W); every
-// wave builds the reflector from it with DPP reductions and applies it to its
-// own columns, their dot products reduced together.  Same reflectors and sign
-// conventions as wg_householder (LAPACK: beta = -sign(alpha) |x|); on return
-// W holds R above and the reflectors below the diagonal, tau the scalars,
-// Qm (m x k, ld k) the explicit Q and Rm (k x c) the clean R.
-__device__ void wg_qr_small(double* W, int m, int c, double* tau, double* Qm, double* Rm) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// Sums over each 16-lane row (DPP row) of the wave, C of them advanced stage
+// by stage so their latencies overlap; every lane of a row gets its row's sum.
+template <int C>
+__device__ __forceinline__ void row16_sum_n(double (&p)[C]) {
+#pragma unroll
+  for (int u = 0; u < C; ++u) p[u] += dpp_mov<0xB1>(p[u]);
+#pragma unroll
+  for (int u = 0; u < C; ++u) p[u] += dpp_mov<0x4E>(p[u]);
+#pragma unroll
+  for (int u = 0; u < C; ++u) p[u] += dpp_mov<0x141>(p[u]);
+#pragma unroll
+  for (int u = 0; u < C; ++u) p[u] += dpp_mov<0x140>(p[u]);
+}
+
+// Householder QR of a small (m x c) row-major panel W, m <= 16 R, c <= 16 C,
+// LAPACK convention (v_j = 1 at row j, H_j = I - tau_j v_j v_j^T, beta =
+// -sign(alpha) |x|).  The workgroup's 16 DPP rows (4 waves x 4 rows of 16
+// lanes) each own the columns r, r+16, ... (r = DPP row); lane l of a row
+// holds the panel rows l, l+16, ... of those columns in registers, so a
+// column's dot product is an in-register sum plus a 16-lane DPP reduction,
+// with no cross-row traffic.  Step j: column j's owner row forms the
+// reflector and publishes it in W's column j (finished R entries above the
+// diagonal, beta on it, v below) with tau[j]; one barrier; every row applies
+// it to its later columns.  Explicit Q (m x k, ld k) goes to Qm by backward
+// accumulation in the same layout, the clean R (k x c) to Rm; W keeps the
+// reflectors.
+template <int R, int C>
+__device__ void wg_qr_small_t(double* W, int m, int c, double* tau, double* Qm, double* Rm) {
+  const int l16 = threadIdx.x & 15, r = threadIdx.x >> 4;  // lane in row, DPP row 0..15
   const int k = m < c ? m : c;
-  const bool rv = lane < m;
-  double col[16];
+  double a[R][C];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int jj = w + NWAVE * u;
-    col[u] = (rv && jj < c) ? W[lane * c + jj] : 0.0;
-  }
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int i = l16 + 16 * q, jj = r + 16 * u;
+      a[q][u] = (i < m && jj < c) ? W[i * c + jj] : 0.0;
+    }
+  __syncthreads();  // W is rewritten column by column below
   for (int j = 0; j < k; ++j) {
-    const int own = j % NWAVE;
-    if (w == own) {
+    if (r == (j & 15)) {
+      const int uj = j >> 4;
+      double x[R];
 #pragma unroll
-      for (int u = 0; u < 16; ++u)
-        if (w + NWAVE * u == j && rv) W[lane * c + j] = col[u];
+      for (int q = 0; q < R; ++q) {
+        x[q] = 0.0;
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+          if (u == uj) x[q] = a[q][u];
+      }
+      double sa[2] = {0.0, 0.0};  // sum of squares below the diagonal, alpha
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = l16 + 16 * q;
+        if (i > j && i < m) sa[0] += x[q] * x[q];
+        if (i == j) sa[1] = x[q];
+      }
+      row16_sum_n<2>(sa);  // alpha is the row's one nonzero term: exact
+      const double alpha = sa[1];
+      double t = 0.0, beta = alpha, sc = 0.0;
+      if (sa[0] != 0.0) {
+        const double nrm = sqrt(alpha * alpha + sa[0]);
+        beta = alpha >= 0.0 ? -nrm : nrm;
+        t = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = l16 + 16 * q;
+        if (i < m) W[i * c + j] = i < j ? x[q] : (i == j ? beta : x[q] * sc);
+      }
+      if (l16 == 0) tau[j] = t;
     }
     __syncthreads();
-    const double x = (rv && lane >= j) ? W[lane * c + j] : 0.0;
-    const double alpha = rdlane(x, j);
-    const double s2 = wave_sum(lane > j ? x * x : 0.0);
-    double t = 0.0, beta = alpha, sc = 0.0;
-    if (s2 != 0.0) {
-      const double nrm = sqrt(alpha * alpha + s2);
-      beta = alpha >= 0.0 ? -nrm : nrm;
-      t = (beta - alpha) / beta;
-      sc = 1.0 / (alpha - beta);
-    }
-    const double v = (lane == j) ? 1.0 : (lane > j ? x * sc : 0.0);
-    __syncthreads();  // everyone has read column j before its owner rewrites it
-    if (w == own && rv && lane >= j) W[lane * c + j] = (lane == j) ? beta : v;
-    if (threadIdx.x == 0) tau[j] = t;
+    const double t = tau[j];
     if (t != 0.0) {
-      double p[16];
+      double v[R];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) p[u] = (w + NWAVE * u > j && w + NWAVE * u < c) ? v * col[u] : 0.0;
+      for (int q = 0; q < R; ++q) {
+        const int i = l16 + 16 * q;
+        v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? W[i * c + j] : 0.0);
+      }
+      double p[C];
 #pragma unroll
-      for (int u = 0; u < 16; ++u)
-        if (w + NWAVE * u > j && w + NWAVE * u < c) col[u] -= v * (wave_sum(p[u]) * t);
+      for (int u = 0; u < C; ++u) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) s += v[q] * a[q][u];
+        p[u] = (r + 16 * u > j) ? s : 0.0;
+      }
+      row16_sum_n<C>(p);
+#pragma unroll
+      for (int u = 0; u < C; ++u) {
+        const double pt = p[u] * t;
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[q][u] -= v[q] * pt;
+      }
     }
   }
   // columns k..c-1 (wide panels) are R rows only: publish them
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int jj = w + NWAVE * u;
-    if (jj >= k && jj < c && rv) W[lane * c + jj] = col[u];
-  }
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int i = l16 + 16 * q, jj = r + 16 * u;
+      if (jj >= k && jj < c && i < m) W[i * c + jj] = a[q][u];
+    }
   __syncthreads();
   for (int e = threadIdx.x; e < k * c; e += NT) {
     const int i = e / c, jj = e - (e / c) * c;
     Rm[e] = (jj >= i) ? W[i * c + jj] : 0.0;
   }
-  // explicit Q by backward accumulation; wave w owns Q columns w, w+NWAVE, ...
-  // (H_j leaves columns < j untouched), so no barrier is needed
-  double q[16];
+  // explicit Q = H_0 ... H_{k-1} I by backward accumulation; H_j leaves the
+  // identity columns < j untouched (v_j vanishes above row j), and W, tau are
+  // final, so no barrier is needed
+  double qm[R][C];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) q[u] = (lane == w + NWAVE * u) ? 1.0 : 0.0;
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < C; ++u) qm[q][u] = (l16 + 16 * q == r + 16 * u) ? 1.0 : 0.0;
   for (int j = k - 1; j >= 0; --j) {
     const double t = tau[j];
     if (t == 0.0) continue;
-    const double v = (lane == j) ? 1.0 : ((lane > j && rv) ? W[lane * c + j] : 0.0);
-    double p[16];
+    double v[R];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) p[u] = (w + NWAVE * u >= j && w + NWAVE * u < k) ? v * q[u] : 0.0;
+    for (int q = 0; q < R; ++q) {
+      const int i = l16 + 16 * q;
+      v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? W[i * c + j] : 0.0);
+    }
+    double p[C];
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (w + NWAVE * u >= j && w + NWAVE * u < k) q[u] -= v * (wave_sum(p[u]) * t);
+    for (int u = 0; u < C; ++u) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) s += v[q] * qm[q][u];
+      p[u] = s;
+    }
+    row16_sum_n<C>(p);
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const double pt = p[u] * t;
+#pragma unroll
+      for (int q = 0; q < R; ++q) qm[q][u] -= v[q] * pt;
+    }
   }
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int qq = w + NWAVE * u;
-    if (qq < k && rv) Qm[lane * k + qq] = q[u];
-  }
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < C; ++u) {
+      const int i = l16 + 16 * q, jj = r + 16 * u;
+      if (i < m && jj < k) Qm[i * k + jj] = qm[q][u];
+    }
   __syncthreads();
+}
+
+// m, c <= 64
+__device__ void wg_qr_small(double* W, int m, int c, double* tau, double* Qm, double* Rm) {
+  if (m <= 16 && c <= 16) wg_qr_small_t<1, 1>(W, m, c, tau, Qm, Rm);
+  else if (m <= 32 && c <= 16) wg_qr_small_t<2, 1>(W, m, c, tau, Qm, Rm);
+  else if (m <= 32 && c <= 32) wg_qr_small_t<2, 2>(W, m, c, tau, Qm, Rm);
+  else wg_qr_small_t<4, 4>(W, m, c, tau, Qm, Rm);
 }
 
 // Explicit Q (m x k, row-major, ld = k) from the reflectors in W.
