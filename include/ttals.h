@@ -130,6 +130,14 @@ int ttals_set_arena_kb(ttals_ctx* ctx, int kb);
  * counterpart in the reference (its local eigensolves are dense). */
 int ttals_set_local_tol_policy(ttals_ctx* ctx, int policy);
 
+/* Form of the on-chip local matvec for later submissions and pieces on this
+ * context: 0 (default) the two-stage form -- the operator core merged with
+ * the right stack once per local problem, then two matrix-core GEMMs per
+ * matvec -- wherever it fits the arena, else the three-stage form; 1 always
+ * the three-stage form.  Results agree to rounding.  No counterpart in the
+ * reference: a tuning and test knob. */
+int ttals_set_matvec_variant(ttals_ctx* ctx, int variant);
+
 /* Device time (ms) of the engine kernel of the context's last finished
  * submission, from HIP events recorded on its stream immediately before and
  * after the launch (host-side preparation excluded).  TTALS_E_ARG while a

@@ -59,6 +59,7 @@ SIGNATURES = {
     'ttals_last_launch_ms': (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     'ttals_set_arena_kb': (_i, [_vp, _i]),
     'ttals_set_local_tol_policy': (_i, [_vp, _i]),
+    'ttals_set_matvec_variant': (_i, [_vp, _i]),
     'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_left_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),

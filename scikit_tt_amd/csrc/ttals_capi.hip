@@ -60,6 +60,7 @@ struct PieceArgs {
   int method, sym, maxit, mode;
   double tol, sigma;
   long arena;        // dynamic LDS of the launch in doubles (on-chip path when the problem fits)
+  int mvar;          // on-chip matvec form (SysDesc::mvar)
 };
 
 enum PieceOp { P_LEFT_OP, P_RIGHT_OP, P_LEFT_RHS, P_RIGHT_RHS, P_MICRO_DENSE, P_MATVEC, P_MICRO_RHS, P_SOLVE,
@@ -90,9 +91,9 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       micro_dense<double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.out, A.a * A.n * A.b);
       break;
     case P_MATVEC:
-      if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 2) <= A.arena) {
+      if (arena_need_onchip(A.a, A.P, A.n, A.Q, A.b, 2, A.mvar) <= A.arena) {
         LocalOp<double> op{A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.R, A.t1, A.t2, false};
-        double* vec = onchip_operator(op, dyn_lds);
+        double* vec = onchip_operator(op, dyn_lds, A.arena, 2, A.mvar);
         wg_copy(A.x, vec, (int)N);
         __syncthreads();
         apply_op<double, double>(op, vec, vec + N, lds);
@@ -114,10 +115,10 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         wg_copy(A.f, A.v, N);
         __syncthreads();
         r = wg_lu_solve(A.V, N, A.v, reinterpret_cast<int*>(A.V + (long)N * N), S.red) ? 0 : 1;
-      } else if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 5) <= A.arena) {
+      } else if (arena_need_onchip(A.a, A.P, A.n, A.Q, A.b, 5, A.mvar) <= A.arena) {
         Work W;
         W.V = A.V;
-        onchip_work(W, onchip_operator(op, dyn_lds), (int)N, true);
+        onchip_work(W, onchip_operator(op, dyn_lds, A.arena, 5, A.mvar), (int)N, true);
         wg_copy(A.f, W.f, (int)N);
         wg_copy(A.v, W.x, (int)N);
         __syncthreads();
@@ -149,8 +150,8 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       W.V = A.V;
       int its = 0, r;
       double lam = 0.0;
-      if (arena_need_vec(A.a, A.P, A.n, A.Q, A.b, 3) <= A.arena) {
-        onchip_work(W, onchip_operator(op, dyn_lds), (int)N, false);
+      if (arena_need_onchip(A.a, A.P, A.n, A.Q, A.b, 3, A.mvar) <= A.arena) {
+        onchip_work(W, onchip_operator(op, dyn_lds, A.arena, 3, A.mvar), (int)N, false);
         __syncthreads();
         r = wg_lanczos<double, true>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
       } else {
@@ -418,6 +419,7 @@ struct ttals_ctx {
   // LDS keeps two systems resident per CU (160 KB of LDS)
   long arena = 7936;
   int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
+  int matvec_variant = 0;    // SysDesc::mvar
   std::string err;
 };
 
@@ -680,6 +682,7 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     D.sym = -1;
     D.gm_restart = KMAX;
     D.adapt = (kind == 1) ? c->tol_policy : 0;
+    D.mvar = c->matvec_variant;
     D.maxit = maxit;
     D.tol = tol;
     D.sigma = sigma;
@@ -781,6 +784,7 @@ int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double
 
 int launch_piece(ttals_ctx* c, hipStream_t st, PieceArgs& a) {
   a.arena = c->arena;
+  a.mvar = c->matvec_variant;
   hipLaunchKernelGGL(k_piece, dim3(1), dim3(NT), (size_t)a.arena * 8, st, a);
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -872,6 +876,13 @@ int ttals_set_local_tol_policy(ttals_ctx* c, int policy) {
   if (!c || policy < 0 || policy > 15) return TTALS_E_ARG;
   if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
   c->tol_policy = policy;
+  return 0;
+}
+
+int ttals_set_matvec_variant(ttals_ctx* c, int variant) {
+  if (!c || variant < 0 || variant > 1) return TTALS_E_ARG;
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
+  c->matvec_variant = variant;
   return 0;
 }
 

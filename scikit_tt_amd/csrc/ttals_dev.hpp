@@ -623,6 +623,64 @@ __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, i
   mf_gemm_lin(a * n, b, b * Q, t2, s2, 1, R, b, 1, y, plain_r, plain_c, t1, (int)ds_t1_size(a, P, n, b));
 }
 
+// ---------------------------------------------------------------------------
+// Two-stage on-chip matvec.  Per local solve the operator core and the right
+// stack are merged once,
+//   M2[(P,n,b), (m,b')] = sum_Q A[P,m,n,Q] R[b,Q,b'],
+// and every Krylov matvec is then two matrix-core GEMMs:
+//   T1'[a', (P,n,b)] = sum_a L[a,P,a'] x[a,(n,b)]                 (K = a)
+//   y[a', (m,b')]    = sum_(P,n,b) T1'[a',(P,n,b)] M2[(P,n,b),(m,b')] (K = P n b)
+// one stage, one barrier and the tiny vector-unit middle stage fewer than the
+// three-stage form, for (P n b / (P n + b Q)) more flops in the second GEMM.
+// T1' rows have the odd stride odd_stride(P n b) (conflict-free row reads).
+
+// K-split scratch of the second GEMM (mf_gemm_lin's rule), in doubles.
+__host__ __device__ __forceinline__ int m2_red_need(int a, int nb) {
+  const int tiles = ((a + 15) >> 4) * ((nb + 15) >> 4);
+  return (2 * tiles <= NWAVE && NWAVE % tiles == 0) ? (NWAVE / tiles - 1) * tiles * 256 : 0;
+}
+
+// floor(i / div) for 0 <= i < 2^22 with the FP32 reciprocal of div (see FMap)
+__device__ __forceinline__ int fquot(int i, float inv) { return (int)(((float)i + 0.5f) * inv); }
+
+// M2 from the operator core A (P, m, n, Q) and the right stack R (b, Q, b') in
+// global memory, both staged through `tmp` (b Q b + P n n Q doubles).  Ends
+// with a barrier: M2 is complete and `tmp` free.
+__device__ void build_m2(const double* A, const double* R, int P, int n, int Q, int b, double* M2, double* tmp) {
+  const int bqb = b * Q * b, nb = n * b, k2 = P * n * b;
+  TT_LDS double* Rs = (TT_LDS double*)tmp;
+  TT_LDS double* As = Rs + bqb;
+  for (int e = threadIdx.x; e < bqb; e += NT) Rs[e] = R[e];
+  for (int e = threadIdx.x; e < P * n * n * Q; e += NT) As[e] = A[e];
+  __syncthreads();
+  const float inb = 1.0f / (float)nb, ib = 1.0f / (float)b, in = 1.0f / (float)n;
+  TT_LDS double* out = (TT_LDS double*)M2;
+  for (int e = threadIdx.x; e < k2 * nb; e += NT) {
+    const int k = fquot(e, inb), j = e - k * nb;
+    const int pn = fquot(k, ib), bb = k - pn * b;
+    const int p = fquot(pn, in), nn = pn - p * n;
+    const int mm = fquot(j, ib), bp = j - mm * b;
+    const TT_LDS double* ap = As + ((p * n + mm) * n + nn) * Q;
+    const TT_LDS double* rp = Rs + bb * Q * b + bp;
+    double s = 0.0;
+    for (int q = 0; q < Q; ++q) s = fma(ap[q], rp[q * b], s);
+    out[e] = s;
+  }
+  __syncthreads();
+}
+
+// y = (L x M2) v on-chip; t1 holds T1' (a * odd_stride(P n b)), red the
+// K-split scratch (m2_red_need).  Out of line for the same register reason
+// as local_matvec_mf.
+__device__ __attribute__((noinline)) void local_matvec_m2(int a, int P, int n, int b, const double* L,
+                                                          const double* M2, const double* v, double* y, double* t1,
+                                                          double* red) {
+  const int nb = n * b, k2 = P * n * b, s = odd_stride(k2);
+  const FMap plain = fmap(1, 1, 0);
+  mf_gemm_lin(P * a, nb, a, L, 1, P * a, v, nb, 1, t1, fmap(a, nb, s), plain, nullptr, 0);
+  mf_gemm_lin(a, nb, k2, t1, s, 1, M2, nb, 1, y, fmap(1, nb, 0), plain, red, m2_red_need(a, nb));
+}
+
 // Lout[b, Q, b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']
 template <class T, class ACC>
 __device__ void stack_left_op(int a, int P, int n, int Q, int b, const T* L, const T* Ac, const T* x,

@@ -20,7 +20,9 @@ enum Status : int {
 struct SysDesc {
   int d, kind, solver, repeats;
   int ortho_x0, sym, gm_restart, maxit;
-  int adapt, pad1[3];         // adapt k > 0: evp local tolerance follows the eigenvalue change, capped at 1e-k
+  int adapt;                  // k > 0: evp local tolerance follows the eigenvalue change, capped at 1e-k
+  int mvar;                   // on-chip matvec: 0 two-stage where it fits, 1 always three-stage
+  int pad1[2];
   double tol, sigma, conv_eps, pad0;
   const int* n;               // [d]
   int* rx;                    // [d+1] x ranks (in/out)
@@ -256,16 +258,47 @@ __host__ __device__ __forceinline__ long arena_need_vec(int a, int P, int n, int
   return (long)a * P * a + (long)b * Q * b + (long)P * n * n * Q + ds_t1_size(a, P, n, b) +
          ds_t2_size(a, n, Q, b) + nvec * N;
 }
+// Two-stage form: L, M2, T1', the K-split scratch and the vectors; while M2
+// is built, R and the operator core are staged where T1' and the vectors go.
+__host__ __device__ __forceinline__ long arena_need_m2(int a, int P, int n, int Q, int b, int nvec) {
+  const long N = (long)a * n * b, nb = (long)n * b, k2 = (long)P * n * b;
+  const long body = (long)a * odd_stride((int)k2) + m2_red_need(a, (int)nb) + nvec * N;
+  const long stage = (long)b * Q * b + (long)P * n * n * Q;
+  return (long)a * P * a + k2 * nb + (body > stage ? body : stage);
+}
+// Arena doubles an on-chip local problem needs (mvar 0: the smaller form).
+__host__ __device__ __forceinline__ long arena_need_onchip(int a, int P, int n, int Q, int b, int nvec, int mvar) {
+  const long n3 = arena_need_vec(a, P, n, Q, b, nvec);
+  if (mvar != 0) return n3;
+  const long n2 = arena_need_m2(a, P, n, Q, b, nvec);
+  return n2 < n3 ? n2 : n3;
+}
 __device__ __forceinline__ long arena_need(const SysDesc& D, int a, int P, int n, int Q, int b) {
-  return arena_need_vec(a, P, n, Q, b, D.kind == 0 ? 5 : 3);
+  return arena_need_onchip(a, P, n, Q, b, D.kind == 0 ? 5 : 3, D.mvar);
 }
 
-// Copies the local operator of `op` into the arena `ar` (L, R, the operator
-// core as A'(P, n, m, Q), room for the two intermediates) and switches `op`
-// to the on-chip matvec.  Returns the first free double of the arena (room
-// for the working vectors).  No barrier.
-__device__ __forceinline__ double* onchip_operator(LocalOp<double>& op, double* ar) {
+// Sets up the on-chip local operator of `op` in the arena `ar` (`cap`
+// doubles, `nvec` working vectors) and switches `op` to it: the two-stage
+// form (L, M2) when mvar == 0 and it fits, else the three-stage form (L, R,
+// the operator core as A'(P, n, m, Q), room for the two intermediates).
+// Returns the first free double of the arena (the working vectors).  The
+// caller has checked arena_need_onchip <= cap and puts a barrier between
+// this and the first matvec.
+__device__ __forceinline__ double* onchip_operator(LocalOp<double>& op, double* ar, long cap, int nvec, int mvar) {
   const int a = op.a, P = op.P, n = op.n, Q = op.Q, b = op.b;
+  if (mvar == 0 && arena_need_m2(a, P, n, Q, b, nvec) <= cap) {
+    const int nb = n * b, k2 = P * n * b;
+    double* Lc = ar;
+    double* M2 = Lc + (long)a * P * a;
+    double* t1 = M2 + (long)k2 * nb;
+    double* red = t1 + (long)a * odd_stride(k2);
+    wg_copy(op.L, Lc, a * P * a);
+    build_m2(op.A, op.R, P, n, Q, b, M2, t1);
+    LocalOp<double> o{a, P, n, Q, b, Lc, nullptr, nullptr, t1, red, true};
+    o.M2 = M2;
+    op = o;
+    return red + m2_red_need(a, nb);
+  }
   double* Lc = ar;
   double* Rc = Lc + (long)a * P * a;
   double* Ac = Rc + (long)b * Q * b;
@@ -322,7 +355,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   W.V = D.V;
   const bool onchip = !dense && arena_need(D, a, P, n, Q, b) <= arena;
   if (onchip) {
-    double* vec = onchip_operator(op, reinterpret_cast<double*>(lds));
+    double* vec = onchip_operator(op, reinterpret_cast<double*>(lds), arena, D.kind == 0 ? 5 : 3, D.mvar);
     onchip_work(W, vec, N, D.kind == 0);
     if (D.kind == 0) {
       wg_copy(D.f, W.f, N);

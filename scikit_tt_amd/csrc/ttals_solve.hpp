@@ -22,12 +22,15 @@ struct LocalOp {
   T* t1;
   T* t2;
   bool ds;    // operands on-chip (operator core stored as (P, n, m, Q))
+  const T* M2 = nullptr;  // on-chip two-stage form (L, M2; t1 = T1', t2 = K-split scratch)
   __device__ int size() const { return a * n * b; }
 };
 
 template <class T, class ACC>
 __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y, ACC* lds) {
-  if (op.ds)
+  if (op.ds && op.M2)
+    local_matvec_m2(op.a, op.P, op.n, op.b, op.L, op.M2, v, y, op.t1, op.t2);
+  else if (op.ds)
     local_matvec_mf<T>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2);
   else
     local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);

@@ -40,14 +40,18 @@ def empty(*shape):
     return torch.empty(shape, dtype=torch.float64, device='cuda')
 
 
-@pytest.fixture(params=[62, 0], ids=['onchip', 'global'])
+@pytest.fixture(params=[(62, 0), (62, 1), (0, 0)], ids=['onchip', 'onchip3', 'global'])
 def arena(request, env):
     """Run a piece on the engine's on-chip (LDS, matrix-core) path where the
-    problem fits, or force the global (HBM/L2) path."""
+    problem fits -- two-stage matvec (operator core merged with the right
+    stack) or the three-stage one -- or force the global (HBM/L2) path."""
     lib, ctx, _ = env
-    assert lib.ttals_set_arena_kb(ctx, request.param) == 0
-    yield request.param
+    kb, variant = request.param
+    assert lib.ttals_set_arena_kb(ctx, kb) == 0
+    assert lib.ttals_set_matvec_variant(ctx, variant) == 0
+    yield kb
     assert lib.ttals_set_arena_kb(ctx, 62) == 0
+    assert lib.ttals_set_matvec_variant(ctx, 0) == 0
 
 
 SHAPES = [(1, 1, 2, 3, 2), (3, 2, 4, 3, 5), (5, 2, 8, 2, 5), (20, 3, 8, 2, 17), (16, 3, 2, 3, 16), (2, 4, 16, 4, 48)]
