@@ -412,6 +412,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     // matvecs + two Gram-Schmidt passes against on average half the basis
     st.flops += its * fmv + (double)its * 8.0 * N * 0.5 * (its < D.gm_restart ? its : D.gm_restart);
     if (r == 3) return ST_UNSUPPORTED;
+    if (r == 4) return ST_NONFINITE;
     if (r != 0) ++st.nunconv;
   }
   st.kit += its;
@@ -426,7 +427,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     t[5] = st.last_res;
   }
   ++st.nsolve;
-  {
+  if (D.kind == 0) {  // (the eigensolver checks its Ritz vector itself)
     const double nv = wg_nrm2(D.g, N, S.red);
     if (!isfinite(nv)) return ST_NONFINITE;
   }

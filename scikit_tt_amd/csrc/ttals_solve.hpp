@@ -633,7 +633,8 @@ __device__ TT_COLD void tri_eigvec(const double* al, const double* be, int m, do
 // Gram-Schmidt against the basis V (global).  Breakdowns (an invariant
 // subspace, e.g. a symmetry sector) continue with a fresh orthogonal
 // direction.  Returns 0 converged, 2 not converged, 3 sigma inside the
-// spectrum of a problem larger than the Krylov space (unsupported).
+// spectrum of a problem larger than the Krylov space (unsupported), 4 a
+// non-finite Ritz vector.  On 0 and 2, v holds the unit Ritz vector.
 template <class ACC, bool OC>
 __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, int mode, double sigma, double tol,
                           int maxit, int kmax, ACC* lds, SolveLDS& S, double* lambda, int* iters) {
@@ -643,11 +644,28 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
   double *prv = W.r, *cur = W.cur, *w = W.w;
   if (kmax > KMAX) kmax = KMAX;
   const bool full_orth = (N <= KMAX);
-  double vn = wg_nrm2(v, N, S.red);
-  if (!(vn > 0.0) || !isfinite(vn)) {
-    for (int e = threadIdx.x; e < N; e += NT) v[e] = 1.0 + 0.1 * (double)((e * 2654435761u) % 97) / 97.0;
-    __syncthreads();
-    vn = wg_nrm2(v, N, S.red);
+  // start vector into cur (one pass over v, which may be in global memory);
+  // every vector loop below gives element e to thread e mod NT
+  double vn;
+  {
+    double ss = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int e = threadIdx.x; e < N; e += NT) {
+      const double x = v[e];
+      M::st(cur, e, x);
+      ss += x * x;
+    }
+    wg_sum3(ss, d1, d2, S.red);
+    vn = sqrt(ss);
+    if (!(vn > 0.0) || !isfinite(vn)) {
+      ss = 0.0;
+      for (int e = threadIdx.x; e < N; e += NT) {
+        const double x = 1.0 + 0.1 * (double)((e * 2654435761u) % 97) / 97.0;
+        M::st(cur, e, x);
+        ss += x * x;
+      }
+      wg_sum3(ss, d1, d2, S.red);
+      vn = sqrt(ss);
+    }
   }
   int target = (mode == 1) ? 1 : 0;
   bool decided = (mode != 2);
@@ -658,7 +676,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
   for (int restart = 0; restart < 1000; ++restart) {
     const double iv = 1.0 / vn;
     for (int e = threadIdx.x; e < N; e += NT) {
-      const double t = v[e] * iv;
+      const double t = M::ld(cur, e) * iv;
       V[e] = t;
       M::st(cur, e, t);
       M::st(prv, e, 0.0);
@@ -882,17 +900,36 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
       ph_add(S, PH_TRI, tv);
     }
-    // Ritz vector
+    // Ritz vector into w (free now), basis rows read four at a time (the
+    // basis may be in global memory), its norm from registers
+    double ss = 0.0;
     for (int e = threadIdx.x; e < N; e += NT) {
       double t = 0.0;
-      for (int i = 0; i < m; ++i) t += S.s[i] * V[(long)i * N + e];
-      v[e] = t;
+      int i = 0;
+      for (; i + 4 <= m; i += 4) {
+        const double x0 = V[(long)i * N + e], x1 = V[(long)(i + 1) * N + e];
+        const double x2 = V[(long)(i + 2) * N + e], x3 = V[(long)(i + 3) * N + e];
+        t += S.s[i] * x0;
+        t += S.s[i + 1] * x1;
+        t += S.s[i + 2] * x2;
+        t += S.s[i + 3] * x3;
+      }
+      for (; i < m; ++i) t += S.s[i] * V[(long)i * N + e];
+      M::st(w, e, t);
+      ss += t * t;
     }
-    __syncthreads();
-    vn = wg_nrm2(v, N, S.red);
+    {
+      double d1 = 0.0, d2 = 0.0;
+      wg_sum3(ss, d1, d2, S.red);
+    }
+    vn = sqrt(ss);
+    if (!isfinite(vn) || !(vn > 0.0)) {
+      status = 4;
+      break;
+    }
     if (conv || *iters >= maxit) {
       const double iv2 = 1.0 / vn;
-      for (int e = threadIdx.x; e < N; e += NT) v[e] *= iv2;
+      for (int e = threadIdx.x; e < N; e += NT) v[e] = M::ld(w, e) * iv2;
       __syncthreads();
       status = conv ? 0 : 2;
       if (conv && mode == 2 && !full) {
@@ -902,6 +939,9 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       }
       break;
     }
+    double* t = cur;  // restart from the Ritz vector
+    cur = w;
+    w = t;
   }
   *lambda = th;
   return status;
