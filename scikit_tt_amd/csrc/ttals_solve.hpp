@@ -388,31 +388,45 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Sturm count #{eigenvalues of T < x} with T held one row per lane (a_l =
-// alpha, b2_l = beta^2 of the coupling to the next row), read back with
-// v_readlane so the recurrence never waits on memory.  Signs of p_k =
-// det(T_k - x I) from the three-term recurrence (one dependent FMA a step),
-// rescaled by 2^-600 / 2^600 when |p_k| leaves [2^-600, 2^600]; an exact
-// zero takes the sign of its predecessor.
-__device__ __forceinline__ int sturm_count_rows(double a_l, double b2_l, int m, double x) {
-  double p0 = 1.0, p1 = rdlane(a_l, 0) - x;
-  bool neg = p1 < 0.0;
-  int c = neg ? 1 : 0;
-  for (int k = 1; k < m; ++k) {
-    double p2 = fma(rdlane(a_l, k) - x, p1, -rdlane(b2_l, k - 1) * p0);
-    const double ap = fabs(p2);
-    if (ap > 0x1p+600) {
-      p2 *= 0x1p-600;
-      p1 *= 0x1p-600;
-    } else if (ap < 0x1p-600 && fabs(p1) < 0x1p-600) {
-      p2 *= 0x1p+600;
-      p1 *= 0x1p+600;
+
+// Sturm count (eigenvalues below x) of a tridiagonal T scaled so that
+// |entries| and |x| are O(1): ta[k] = alpha_k, tb[k] = beta_{k-1}^2 (tb[0] =
+// 0), k < m4, m4 a multiple of 4 (rows past m padded decoupled, above the
+// spectrum).  Every lane reads the coefficients as LDS broadcasts four rows
+// at a time; the three-term recurrence p_{k+1} = (alpha_k - x) p_k -
+// beta_{k-1}^2 p_{k-1} is rescaled by powers of two between groups (per
+// row the magnitude grows at most ~4x, so 2^600 leaves hundreds of binades).
+// A zero p_k takes the sign of p_{k-1}.
+__device__ __forceinline__ int sturm_count_lds4(const double* ta_, const double* tb_, int m4, double x) {
+  const TT_LDS double* ta = (const TT_LDS double*)ta_;
+  const TT_LDS double* tb = (const TT_LDS double*)tb_;
+  double p0 = 0.0, p1 = 1.0;
+  bool neg = false;
+  int c = 0;
+  for (int k = 0; k < m4; k += 4) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      av[q] = ta[k + q];
+      bv[q] = tb[k + q];
     }
-    const bool ng = (p2 < 0.0) || (p2 == 0.0 && neg);
-    c += (ng != neg) ? 1 : 0;
-    neg = ng;
-    p0 = p1;
-    p1 = p2;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double p2 = fma(av[q] - x, p1, -bv[q] * p0);
+      const bool ng = (p2 < 0.0) || (p2 == 0.0 && neg);
+      c += (ng != neg) ? 1 : 0;
+      neg = ng;
+      p0 = p1;
+      p1 = p2;
+    }
+    const double ap = fabs(p1);
+    if (ap > 0x1p+600) {
+      p1 *= 0x1p-600;
+      p0 *= 0x1p-600;
+    } else if (ap < 0x1p-600 && fabs(p0) < 0x1p-600) {
+      p1 *= 0x1p+600;
+      p0 *= 0x1p+600;
+    }
   }
   return c;
 }
@@ -428,6 +442,9 @@ __device__ __forceinline__ int sturm_count_rows(double a_l, double b2_l, int m, 
 //    s_k = -(beta_k / q+_k) s_{k+1} above r, s_k = -(beta_{k-1} / q-_k) s_{k-1}
 //    below r.  Stable where one-sided recurrences are not: a converged Ritz
 //    vector decays toward the bottom of T.
+// The Sturm counts run on T scaled by a power of two (exact) to |T| ~ 1, its
+// coefficients broadcast from tw; the pivot recurrences read al / be as
+// broadcasts, one row ahead.
 // tw: LDS scratch of 128 doubles.  Returns false (s untouched) if the
 // eigenvector overflowed.
 __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in, int idx_in, double* s, double* tw,
@@ -439,13 +456,22 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   const double a_l = row ? al[lane] : 0.0;
   const double b_l = (lane < m - 1) ? be[lane] : 0.0;
   const double bm_l = (row && lane > 0) ? be[lane - 1] : 0.0;
-  const double b2_l = b_l * b_l;
   const double rad = fabs(b_l) + fabs(bm_l);
   double lo = wave_min(row ? a_l - rad : 1e300);
   double hi = wave_max(row ? a_l + rad : -1e300);
   const double pad = 1e-12 * fmax(hi - lo, 1e-300) + 1e-300;
   lo -= pad;
   hi += pad;
+  // scaled coefficients for the counts: sc = 2^-e with |lo|, |hi| < 2^e
+  const int e2 = ilogb(fmax(fabs(lo), fabs(hi))) + 1;
+  const double sc = ldexp(1.0, -e2);
+  const int m4 = (m + 3) & ~3;
+  if (threadIdx.x < 64) {
+    const double bs = (lane > 0 && lane < m) ? bm_l * sc : 0.0;
+    tw[lane] = row ? a_l * sc : 4.0;  // padding rows: decoupled, above [-1, 1]
+    tw[64 + lane] = bs * bs;
+  }
+  __syncthreads();
   // Warm start: a hint interval [hint_lo, hint_hi] (hint_lo < hint_hi) is
   // searched first with NT points that include both ends, so the first round
   // also tells whether the eigenvalue lies below or above it; the bracket then
@@ -453,7 +479,7 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   if (hint_lo < hint_hi && hint_lo > lo && hint_hi < hi) {
     const double h = (hint_hi - hint_lo) / (double)(NT - 1);
     const double x = (threadIdx.x == NT - 1) ? hint_hi : hint_lo + h * (double)threadIdx.x;
-    const int c = sturm_count_rows(a_l, b2_l, m, x);
+    const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
     const unsigned long long ball = __ballot(c <= idx);
     int* cnt = S.cnt + NWAVE;
     if (lane == 0) cnt[wv] = __popcll(ball);
@@ -475,7 +501,7 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
     const double h = (hi - lo) / (double)(NT + 1);
     if (!(lo + h > lo) || !(lo + h * NT < hi)) break;  // spacing below the rounding of lo / hi
     const double x = lo + h * (double)(threadIdx.x + 1);
-    const int c = sturm_count_rows(a_l, b2_l, m, x);
+    const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
     const unsigned long long ball = __ballot(c <= idx);
     int* cnt = S.cnt + (round & 1) * NWAVE;
     if (lane == 0) cnt[wv] = __popcll(ball);
@@ -491,29 +517,46 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   }
   const double th = 0.5 * (lo + hi);
   double q_l = 0.0, r_l = 0.0;  // this lane's pivot and ratio
+  __syncthreads();  // the counts are done with tw
+  const TT_LDS double* la = (const TT_LDS double*)al;
+  const TT_LDS double* lb = (const TT_LDS double*)be;
   if (wv == 0) {
-    double rprev = 0.0;
+    // q+_k = (alpha_k - th) - beta_{k-1} r_{k-1}, r_k = beta_k / q+_k
+    double rprev = 0.0, bprev = 0.0;
+    double ak = la[0], bk = (m > 1) ? lb[0] : 0.0;
     for (int k = 0; k < m; ++k) {
-      double q = (rdlane(a_l, k) - th) - (k > 0 ? rdlane(b_l, k - 1) * rprev : 0.0);
+      const double an = (k + 1 < m) ? la[k + 1] : 0.0;
+      const double bn = (k + 2 < m) ? lb[k + 1] : 0.0;
+      double q = (ak - th) - bprev * rprev;
       if (q == 0.0) q = 1e-300;
-      const double rk = (k < m - 1) ? rdlane(b_l, k) * rcp_nr(q) : 0.0;
+      const double rk = bk * rcp_nr(q);
       if (lane == k) {
         q_l = q;
         r_l = rk;
       }
       rprev = rk;
+      bprev = bk;
+      ak = an;
+      bk = bn;
     }
   } else if (wv == 1) {
-    double rnext = 0.0;
+    // q-_k = (alpha_k - th) - beta_k r_{k+1}, r_k = beta_{k-1} / q-_k
+    double rnext = 0.0, bnext = 0.0;
+    double ak = la[m - 1], bk = (m > 1) ? lb[m - 2] : 0.0;  // beta_{k-1}
     for (int k = m - 1; k >= 0; --k) {
-      double q = (rdlane(a_l, k) - th) - (k < m - 1 ? rdlane(b_l, k) * rnext : 0.0);
+      const double an = (k > 0) ? la[k - 1] : 0.0;
+      const double bn = (k > 1) ? lb[k - 2] : 0.0;
+      double q = (ak - th) - bnext * rnext;
       if (q == 0.0) q = 1e-300;
-      const double rk = (k > 0) ? rdlane(b_l, k - 1) * rcp_nr(q) : 0.0;
+      const double rk = bk * rcp_nr(q);
       if (lane == k) {
         q_l = q;
         r_l = rk;
       }
       rnext = rk;
+      bnext = bk;
+      ak = an;
+      bk = bn;
     }
     tw[lane] = q_l;
     tw[64 + lane] = r_l;
