@@ -396,7 +396,9 @@ __device__ __forceinline__ double wave_max(double v) {
 // at a time; the three-term recurrence p_{k+1} = (alpha_k - x) p_k -
 // beta_{k-1}^2 p_{k-1} is rescaled by powers of two between groups (per
 // row the magnitude grows at most ~4x, so 2^600 leaves hundreds of binades).
-// A zero p_k takes the sign of p_{k-1}.
+// A zero p_k counts as positive: for an interior zero p_{k-1} and p_{k+1}
+// have opposite signs, so the count does not depend on it; p_m = 0 means x
+// is an eigenvalue, where either count keeps the bracket invariant.
 __device__ __forceinline__ int sturm_count_lds4(const double* ta_, const double* tb_, int m4, double x) {
   const TT_LDS double* ta = (const TT_LDS double*)ta_;
   const TT_LDS double* tb = (const TT_LDS double*)tb_;
@@ -413,7 +415,7 @@ __device__ __forceinline__ int sturm_count_lds4(const double* ta_, const double*
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double p2 = fma(av[q] - x, p1, -bv[q] * p0);
-      const bool ng = (p2 < 0.0) || (p2 == 0.0 && neg);
+      const bool ng = p2 < 0.0;
       c += (ng != neg) ? 1 : 0;
       neg = ng;
       p0 = p1;
@@ -477,7 +479,7 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   // also tells whether the eigenvalue lies below or above it; the bracket then
   // continues from whichever side holds it.
   if (hint_lo < hint_hi && hint_lo > lo && hint_hi < hi) {
-    const double h = (hint_hi - hint_lo) / (double)(NT - 1);
+    const double h = (hint_hi - hint_lo) * (1.0 / (double)(NT - 1));
     const double x = (threadIdx.x == NT - 1) ? hint_hi : hint_lo + h * (double)threadIdx.x;
     const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
     const unsigned long long ball = __ballot(c <= idx);
@@ -498,7 +500,7 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
   }
   for (int round = 0; round < 64; ++round) {
     if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
-    const double h = (hi - lo) / (double)(NT + 1);
+    const double h = (hi - lo) * (1.0 / (double)(NT + 1));
     if (!(lo + h > lo) || !(lo + h * NT < hi)) break;  // spacing below the rounding of lo / hi
     const double x = lo + h * (double)(threadIdx.x + 1);
     const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
@@ -741,6 +743,9 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       t0 = ph_now();
       double alpha, w0, bn;
       bool dgks = full_orth;
+      // the second local pass's correction w -= c1 cur + c0 prv is applied
+      // lazily, fused into the normalisation at the end of the step
+      double cc1 = 0.0, cc0 = 0.0;
       if (!full_orth) {
         // pass 1: alpha = <w, cur>, |w|^2
         double a1 = 0.0, ww = 0.0, dummy = 0.0;
@@ -761,13 +766,16 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           w2 += we * we;
         }
         wg_sum3(c1, c0, w2, S.red);
-        for (int e = threadIdx.x; e < N; e += NT)
-          M::st(w, e, M::ld(w, e) - c1 * M::ld(cur, e) - c0 * M::ld(prv, e));
+        cc1 = c1;
+        cc0 = c0;
         alpha = a1 + c1;
         const double b2 = w2 - c1 * c1 - c0 * c0;
         bn = sqrt(fmax(b2, 0.0));
         if (!(bn > 1e-3 * w0)) {
           dgks = true;  // severe cancellation: restore orthogonality to the whole basis
+          for (int e = threadIdx.x; e < N; e += NT)
+            M::st(w, e, M::ld(w, e) - c1 * M::ld(cur, e) - c0 * M::ld(prv, e));
+          cc1 = cc0 = 0.0;
           __syncthreads();
         }
         if (!dgks && threadIdx.x == 0) {
@@ -931,7 +939,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       cur = w;
       w = t;
       for (int e = threadIdx.x; e < N; e += NT) {
-        const double x = M::ld(cur, e) * ib;
+        const double x = (M::ld(cur, e) - cc1 * M::ld(prv, e) - cc0 * M::ld(w, e)) * ib;
         M::st(cur, e, x);
         V[(long)(j + 1) * N + e] = x;
       }
