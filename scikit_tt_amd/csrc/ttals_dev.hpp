@@ -818,6 +818,12 @@ __device__ void wg_householder(double* W, int m, int c, double* tau, double* red
   }
 }
 
+__device__ __forceinline__ double rcp_nr(double x) {  // 1/x: hardware estimate + two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
+
 // Sums over each 16-lane row (DPP row) of the wave, C of them advanced stage
 // by stage so their latencies overlap; every lane of a row gets its row's sum.
 template <int C>
@@ -881,8 +887,8 @@ __device__ void wg_qr_small_t(double* W, int m, int c, double* tau, double* Qm, 
       if (sa[0] != 0.0) {
         const double nrm = sqrt(alpha * alpha + sa[0]);
         beta = alpha >= 0.0 ? -nrm : nrm;
-        t = (beta - alpha) / beta;
-        sc = 1.0 / (alpha - beta);
+        t = (beta - alpha) * rcp_nr(beta);
+        sc = rcp_nr(alpha - beta);
       }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
@@ -892,14 +898,16 @@ __device__ void wg_qr_small_t(double* W, int m, int c, double* tau, double* Qm, 
       if (l16 == 0) tau[j] = t;
     }
     __syncthreads();
+    // tau and the reflector in one batch of LDS reads (rows past m read row m-1)
     const double t = tau[j];
-    if (t != 0.0) {
-      double v[R];
+    double v[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = l16 + 16 * q;
-        v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? W[i * c + j] : 0.0);
-      }
+    for (int q = 0; q < R; ++q) {
+      const int i = l16 + 16 * q;
+      const double wv = W[(i < m ? i : m - 1) * c + j];
+      v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? wv : 0.0);
+    }
+    if (t != 0.0) {
       double p[C];
 #pragma unroll
       for (int u = 0; u < C; ++u) {
@@ -938,15 +946,35 @@ __device__ void wg_qr_small_t(double* W, int m, int c, double* tau, double* Qm, 
   for (int q = 0; q < R; ++q)
 #pragma unroll
     for (int u = 0; u < C; ++u) qm[q][u] = (l16 + 16 * q == r + 16 * u) ? 1.0 : 0.0;
+  // reflector j-1 is read while H_j is applied
+  double tn = 0.0, wn[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) wn[q] = 0.0;
+  if (k > 0) {
+    tn = tau[k - 1];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = l16 + 16 * q;
+      wn[q] = W[(i < m ? i : m - 1) * c + (k - 1)];
+    }
+  }
   for (int j = k - 1; j >= 0; --j) {
-    const double t = tau[j];
-    if (t == 0.0) continue;
+    const double t = tn;
     double v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int i = l16 + 16 * q;
-      v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? W[i * c + j] : 0.0);
+      v[q] = (i == j) ? 1.0 : ((i > j && i < m) ? wn[q] : 0.0);
     }
+    if (j > 0) {
+      tn = tau[j - 1];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = l16 + 16 * q;
+        wn[q] = W[(i < m ? i : m - 1) * c + (j - 1)];
+      }
+    }
+    if (t == 0.0) continue;
     double p[C];
 #pragma unroll
     for (int u = 0; u < C; ++u) {

@@ -370,11 +370,6 @@ __device__ __forceinline__ int sturm_count(const double* al, const double* be, i
 // budget of two workgroups per CU.
 #define TT_COLD __attribute__((noinline))
 
-__device__ __forceinline__ double rcp_nr(double x) {  // 1/x: hardware estimate + two Newton steps
-  double r = __builtin_amdgcn_rcp(x);
-  r = r * (2.0 - x * r);
-  return r * (2.0 - x * r);
-}
 
 
 __device__ __forceinline__ double wave_min(double v) {
