@@ -76,11 +76,39 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
   const long N = (long)A.a * A.n * A.b;
   switch (A.op) {
     case P_LEFT_OP:
-      stack_left_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, A.t2, lds);
+    case P_RIGHT_OP: {
+      // on-chip matrix-core update (the engine's) when it fits the arena; a
+      // right stack is the left-form update of the mirrored problem
+      const bool left = A.op == P_LEFT_OP;
+      const int al = left ? A.a : A.b, Pl = left ? A.P : A.Q, Ql = left ? A.Q : A.P, bl = left ? A.b : A.a;
+      const long need = (long)al * Pl * al + N + (long)A.P * A.n * A.n * A.Q + ds_t1_size(al, Pl, A.n, bl) +
+                        ds_t2_size(al, A.n, Ql, bl) + (long)bl * Ql * bl;
+      if (need <= A.arena) {
+        double* Lc = dyn_lds;
+        double* xc = Lc + (long)al * Pl * al;
+        double* Ac = xc + N;
+        double* t1 = Ac + (long)A.P * A.n * A.n * A.Q;
+        double* t2 = t1 + ds_t1_size(al, Pl, A.n, bl);
+        double* out = t2 + ds_t2_size(al, A.n, Ql, bl);
+        if (left) {
+          wg_copy(A.L, Lc, al * Pl * al);
+          wg_copy(A.x, xc, (int)N);
+          wg_copy_opcore_pnmq(A.A, Ac, A.P, A.n, A.Q);
+        } else {
+          wg_copy(A.R, Lc, al * Pl * al);
+          wg_copy_core_t(A.x, xc, A.a, A.n, A.b);
+          wg_copy_opcore_qnmp(A.A, Ac, A.P, A.n, A.Q);
+        }
+        __syncthreads();
+        stack_left_mf(al, Pl, A.n, Ql, bl, Lc, Ac, xc, out, t1, t2);
+        wg_copy(out, A.out, bl * Ql * bl);
+      } else if (left) {
+        stack_left_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, A.t2, lds);
+      } else {
+        stack_right_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.R, A.A, A.x, A.out, A.t1, A.t2, lds);
+      }
       break;
-    case P_RIGHT_OP:
-      stack_right_op<double, double>(A.a, A.P, A.n, A.Q, A.b, A.R, A.A, A.x, A.out, A.t1, A.t2, lds);
-      break;
+    }
     case P_LEFT_RHS:
       stack_left_rhs<double, double>(A.a, A.P, A.n, A.Q, A.b, A.L, A.A, A.x, A.out, A.t1, lds);
       break;

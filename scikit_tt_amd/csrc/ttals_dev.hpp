@@ -623,6 +623,48 @@ __device__ __attribute__((noinline)) void local_matvec_mf(int a, int P, int n, i
   mf_gemm_lin(a * n, b, b * Q, t2, s2, 1, R, b, 1, y, plain_r, plain_c, t1, (int)ds_t1_size(a, P, n, b));
 }
 
+// Interface stack update on-chip, left form:
+//   out[(b,Q), b'] = sum L[a,P,a'] x[a,n,b] A[P,m,n,Q] x[a',m,b']
+// L (a, P, a'), x (a, n, b), Ap = A'(P, n, m, Q) and out (b, Q, b') in LDS;
+// t1 / t2 the three-stage matvec's intermediates.  Stages 1 and 2 are the
+// matvec's with the core as the vector; stage 3 contracts T2[(a',m),(b,Q)]
+// with the core.  A right stack is this update on the mirrored problem (core
+// transposed to (b, n, a), operator core with P and Q swapped).
+__device__ __attribute__((noinline)) void stack_left_mf(int a, int P, int n, int Q, int b, const double* L,
+                                                        const double* Ap, const double* x, double* out, double* t1,
+                                                        double* t2) {
+  const MvPlan pl = mv_plan(a, P, n, Q, b);
+  const int s1 = odd_stride(P * n), s2 = odd_stride(b * Q);
+  mf_gemm_lin(P * a, n * b, a, L, 1, P * a, x, n * b, 1, t1, pl.c1r, pl.c1c, nullptr, 0);
+  if (P * n <= 16 && P * n * n * Q <= 64)
+    stage2_valu(a, P, n, Q, b, t1, Ap, t2);
+  else
+    mf_gemm_lin(a * b, n * Q, P * n, t1, s1, 1, Ap, n * Q, 1, t2, pl.c2r, pl.c2c, nullptr, 0);
+  mf_gemm_lin(b * Q, b, a * n, t2, 1, s2, x, b, 1, out, fmap(1, b, 0), fmap(1, 1, 0), nullptr, 0);
+}
+
+// The operator core (P, m, n, Q) copied into LDS as A'(Q, n, m, P): the
+// stage-2 layout of the mirrored problem (P and Q swapped).
+__device__ __forceinline__ void wg_copy_opcore_qnmp(const double* src, double* dst, int P, int n, int Q) {
+  const int tot = P * n * n * Q;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int q = e % Q, r1 = e / Q;
+    const int nn = r1 % n, r2 = r1 / n;
+    const int mm = r2 % n, p = r2 / n;
+    dst[((q * n + nn) * n + mm) * P + p] = src[e];
+  }
+}
+
+// core (a, n, b) copied into LDS transposed to (b, n, a)
+__device__ __forceinline__ void wg_copy_core_t(const double* src, double* dst, int a, int n, int b) {
+  const int tot = a * n * b;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int bb = e % b, r = e / b;
+    const int nn = r % n, aa = r / n;
+    dst[(bb * n + nn) * a + aa] = src[e];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Two-stage on-chip matvec.  Per local solve the operator core and the right
 // stack are merged once,

@@ -143,34 +143,28 @@ __device__ double build_left(int i, const SysDesc& D, ACC* lds, long cap) {
   const bool rhs = (D.kind == 0);
   const int p = D.rb[j], q = D.rb[i];
   const double fl = fl_op(a, P, n, Q, b) + (rhs ? fl_rhs(a, p, n, q, b) : 0.0);
-  const long mx = (long)(P > Q ? P : Q) > (long)(p > q ? p : q) ? (P > Q ? P : Q) : (p > q ? p : q);
-  const long need = (long)a * P * a + (long)a * n * b + (long)P * n * n * Q + 2L * a * n * b * mx + (long)b * Q * b +
-                    (rhs ? (long)p * a + (long)p * n * q + (long)q * b : 0);
+  const long t1n = rhs && (long)a * n * q > ds_t1_size(a, P, n, b) ? (long)a * n * q : ds_t1_size(a, P, n, b);
+  const long need = (long)a * P * a + (long)a * n * b + (long)P * n * n * Q + t1n + ds_t2_size(a, n, Q, b) +
+                    (long)b * Q * b + (rhs ? (long)p * a + (long)p * n * q + (long)q * b : 0);
   if (need <= cap) {
     double* Lc = reinterpret_cast<double*>(lds);
     double* xc = Lc + (long)a * P * a;
     double* Ac = xc + (long)a * n * b;
     double* t1 = Ac + (long)P * n * n * Q;
-    double* t2 = t1 + (long)a * n * b * mx;
-    double* Lo = t2 + (long)a * n * b * mx;
+    double* t2 = t1 + t1n;
+    double* Lo = t2 + ds_t2_size(a, n, Q, b);
     double* Lb = Lo + (long)b * Q * b;
     double* Bc = Lb + (long)p * a;
     double* Lbo = Bc + (long)p * n * q;
     wg_copy(D.Ls + D.Loff[j], Lc, a * P * a);
     wg_copy(D.xc[j], xc, a * n * b);
-    wg_copy(D.ac[j], Ac, P * n * n * Q);
+    wg_copy_opcore_pnmq(D.ac[j], Ac, P, n, Q);
     if (rhs) {
       wg_copy(D.Lbs + D.Lboff[j], Lb, p * a);
       wg_copy(D.bc[j], Bc, p * n * q);
     }
     __syncthreads();
-    // same three stages as stack_left_op, LDS to LDS
-    ds_gemm<double, double, true, true, true>(P * a, n * b, a, cview<double>(Lc, plain(1), plain(P * a)), crm<double>(xc, n * b),
-                            mview<double>(t1, split(a, n, b * P * n), split(b, 1, P * n)));
-    ds_gemm<double, double, true, true, true>(a * b, n * Q, P * n, crm<double>(t1, P * n),
-                            cview<double>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
-                            mview<double>(t2, split(b, n, Q * a * n), split(Q, 1, a * n)));
-    ds_gemm<double, double, true, true, true>(b * Q, b, a * n, crm<double>(t2, a * n), crm<double>(xc, b), mrm<double>(Lo, b));
+    stack_left_mf(a, P, n, Q, b, Lc, Ac, xc, Lo, t1, t2);
     if (rhs) {
       ds_gemm<double, double, true, true, true>(a, n * q, p, ctr<double>(Lb, a), crm<double>(Bc, n * q),
                               mview<double>(t1, plain(n), split(q, 1, a * n)));
@@ -189,40 +183,37 @@ __device__ double build_left(int i, const SysDesc& D, ACC* lds, long cap) {
 
 template <class ACC>
 __device__ double build_right(int i, const SysDesc& D, ACC* lds, long cap) {
-  // R_i from R_{i+1} and core i+1
+  // R_i from R_{i+1} and core i+1: the left-form update of the mirrored
+  // problem (ranks a <-> b, operator ranks P <-> Q)
   const int j = i + 1;
   const int a = D.rx[j], n = D.n[j], b = D.rx[j + 1], P = D.ra[j], Q = D.ra[j + 1];
   const bool rhs = (D.kind == 0);
   const int p = D.rb[j], q = D.rb[j + 1];
   const double fl = fl_op(a, P, n, Q, b) + (rhs ? fl_rhs(a, p, n, q, b) : 0.0);
-  const long mx = (long)(P > Q ? P : Q) > (long)(p > q ? p : q) ? (P > Q ? P : Q) : (p > q ? p : q);
-  const long need = (long)b * Q * b + (long)a * n * b + (long)P * n * n * Q + 2L * a * n * b * mx + (long)a * P * a +
-                    (rhs ? (long)q * b + (long)p * n * q + (long)p * a : 0);
+  const long t1n = rhs && (long)a * n * q > ds_t1_size(b, Q, n, a) ? (long)a * n * q : ds_t1_size(b, Q, n, a);
+  const long need = (long)b * Q * b + (long)a * n * b * (rhs ? 2 : 1) + (long)P * n * n * Q + t1n +
+                    ds_t2_size(b, n, P, a) + (long)a * P * a + (rhs ? (long)q * b + (long)p * n * q + (long)p * a : 0);
   if (need <= cap) {
     double* Rc = reinterpret_cast<double*>(lds);
-    double* xc = Rc + (long)b * Q * b;
-    double* Ac = xc + (long)a * n * b;
+    double* xt = Rc + (long)b * Q * b;
+    double* Ac = xt + (long)a * n * b;
     double* t1 = Ac + (long)P * n * n * Q;
-    double* t2 = t1 + (long)a * n * b * mx;
-    double* Ro = t2 + (long)a * n * b * mx;
-    double* Rb = Ro + (long)a * P * a;
+    double* t2 = t1 + t1n;
+    double* Ro = t2 + ds_t2_size(b, n, P, a);
+    double* xc = Ro + (long)a * P * a;  // rhs only: the core untransposed
+    double* Rb = xc + (rhs ? (long)a * n * b : 0);
     double* Bc = Rb + (long)q * b;
     double* Rbo = Bc + (long)p * n * q;
     wg_copy(D.Rs + D.Roff[j], Rc, b * Q * b);
-    wg_copy(D.xc[j], xc, a * n * b);
-    wg_copy(D.ac[j], Ac, P * n * n * Q);
+    wg_copy_core_t(D.xc[j], xt, a, n, b);
+    wg_copy_opcore_qnmp(D.ac[j], Ac, P, n, Q);
     if (rhs) {
+      wg_copy(D.xc[j], xc, a * n * b);
       wg_copy(D.Rbs + D.Rboff[j], Rb, q * b);
       wg_copy(D.bc[j], Bc, p * n * q);
     }
     __syncthreads();
-    // same three stages as stack_right_op, LDS to LDS
-    ds_gemm<double, double, true, true, true>(a * n, b * Q, b, crm<double>(xc, b), ctr<double>(Rc, b),
-                            mview<double>(t1, split(n, b * n * Q, Q), split(Q, n * Q, 1)));
-    ds_gemm<double, double, true, true, true>(a * b, P * n, n * Q, crm<double>(t1, n * Q),
-                            cview<double>(Ac, split(Q, n * Q, 1), split(n, n * n * Q, Q)),
-                            mview<double>(t2, split(b, 1, P * a), split(n, a, b * P * a)));
-    ds_gemm<double, double, true, true, true>(a, P * a, n * b, crm<double>(xc, n * b), crm<double>(t2, P * a), mrm<double>(Ro, P * a));
+    stack_left_mf(b, Q, n, P, a, Rc, Ac, xt, Ro, t1, t2);
     if (rhs) {
       ds_gemm<double, double, true, true, true>(a * n, q, b, crm<double>(xc, b), ctr<double>(Rb, b),
                               mview<double>(t1, split(n, 1, q * a), plain(a)));
