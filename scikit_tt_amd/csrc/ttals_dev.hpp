@@ -498,6 +498,83 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
   }
   const int per = (ks + groups - 1) / groups;
   const int acs4 = acs << 2, brs4 = brs << 2;
+  if (groups == 1 && tiles > NWAVE) {
+    // more tiles than waves: a wave runs tiles u and u + NWAVE together, their
+    // two MFMA chains interleaved (a wave without a second tile repeats its
+    // first and discards the copy)
+    for (int u = w; u < tiles; u += 2 * NWAVE) {
+      const bool two = u + NWAVE < tiles;
+      const int tt[2] = {u, two ? u + NWAVE : u};
+      const TT_LDS double *ap[2], *bp[2], *alast[2], *blast[2];
+      int ti[2], jj[2];
+      bool jv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        ti[h] = tt[h] / tn;
+        const int tj = tt[h] - ti[h] * tn;
+        const int i = (ti[h] << 4) + li;
+        jj[h] = (tj << 4) + li;
+        jv[h] = jj[h] < N;
+        ap[h] = pa + __mul24(i < M ? i : M - 1, ars);
+        bp[h] = pb + __mul24(jv[h] ? jj[h] : N - 1, bcs);
+        alast[h] = ap[h] + __mul24(K - 1, acs);
+        blast[h] = bp[h] + __mul24(K - 1, brs);
+        ap[h] += __mul24(lk, acs);
+        bp[h] += __mul24(lk, brs);
+      }
+      int k = lk;
+      dx4_t acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+      int s = 0;
+      for (; s + 4 <= ks; s += 4) {
+        double av[2][4], bv[2][4];
+        bool kv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          kv[q] = k < K;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            av[h][q] = (kv[q] ? ap[h] : alast[h])[0];
+            bv[h][q] = (kv[q] ? bp[h] : blast[h])[0];
+            ap[h] += acs4;
+            bp[h] += brs4;
+          }
+          k += 4;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h][q], kv[q] ? bv[h][q] : 0.0, acc[h], 0, 0, 0);
+      }
+      for (; s < ks; ++s) {
+        const bool kq = k < K;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double x = (kq ? ap[h] : alast[h])[0];
+          const double y = (kq ? bp[h] : blast[h])[0];
+          acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, kq ? y : 0.0, acc[h], 0, 0, 0);
+          ap[h] += acs4;
+          bp[h] += brs4;
+        }
+        k += 4;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !two) break;
+        if (jv[h]) {
+          const int cj = fmap_at(cc, jj[h]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = (ti[h] << 4) + lk + 4 * e;
+            if (row < M) pc[fmap_at(cr, row) + cj] = acc[h][e];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    return;
+  }
   for (int u = w; u < tiles * groups; u += NWAVE) {
     const int t = u % tiles, part = u / tiles;
     const int ti = t / tn, tj = t - ti * tn;
