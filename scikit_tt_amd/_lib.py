@@ -197,18 +197,28 @@ class DescBatch:
 
     @classmethod
     def from_tts(cls, tts):
-        orders, rows, cols, ranks, ptrs = [], [], [], [], []
+        lazy = [t.__dict__.get('_lazy') if t.__dict__.get('_cores') is None else None for t in tts]
+        if lazy[0] is not None and all(z is not None and z[0] is lazy[0][0] for z in lazy):
+            # views of one flat buffer (e.g. from_host_batch): one base address
+            offs = np.concatenate([z[1] for z in lazy]).astype(np.uint64)
+            ptrs = np.uint64(lazy[0][0].data_ptr()) + np.uint64(8) * offs
+        else:
+            ptrs = np.concatenate([t._core_ptr_array() for t in tts])
+        t0 = tts[0]
+        if all(t.row_dims == t0.row_dims and t.col_dims == t0.col_dims and t.ranks == t0.ranks for t in tts):
+            n = len(tts)  # one structure (the usual batch): dims tiled
+            return cls([t0.order] * n, np.tile(t0.row_dims, n), np.tile(t0.col_dims, n), np.tile(t0.ranks, n), ptrs)
+        orders, rows, cols, ranks = [], [], [], []
         for t in tts:
             orders.append(t.order)
             rows += t.row_dims
             cols += t.col_dims
             ranks += t.ranks
-            ptrs += t._core_ptrs()
         return cls(orders, rows, cols, ranks, ptrs)
 
     def ranks_of(self, s):
         o = int(self.roff[s])
-        return [int(r) for r in self.ranks[o:o + int(self.orders[s]) + 1]]
+        return self.ranks[o:o + int(self.orders[s]) + 1].tolist()
 
 
 class DescPack:

@@ -12,7 +12,8 @@ class SolveInfo:
     """Per-system engine report: status, sweeps, Krylov iterations, eigenvalue."""
 
     __slots__ = ('status', 'sweeps', 'unconverged', 'max_iters', 'krylov_iters', 'local_solves', 'eigenvalue',
-                 'worst_residual', 'flops', 'cycles')
+                 'worst_residual', 'flops', '_r')
+    _CYCLE_NAMES = ('total', 'matvec', 'gram_schmidt', 'tridiag', 'qr', 'stacks', 'setup')
 
     def __init__(self, r):
         self.status = int(r.status)
@@ -24,11 +25,16 @@ class SolveInfo:
         self.eigenvalue = float(r.lambda_)
         self.worst_residual = float(r.worst_residual)
         self.flops = float(r.flops)
-        names = ('total', 'matvec', 'gram_schmidt', 'tridiag', 'qr', 'stacks', 'setup')
-        self.cycles = {k: float(v) for k, v in zip(names, r.cycles)}
+        self._r = r  # keeps the result record for the (rarely read) cycle counters
+
+    @property
+    def cycles(self):
+        """Per-phase cycle counters (non-zero in a -DTTALS_PROFILE=1 build)."""
+        return {k: float(v) for k, v in zip(self._CYCLE_NAMES, self._r.cycles)}
 
     def __repr__(self):
-        return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in self.__slots__)
+        names = [k for k in self.__slots__ if k != '_r'] + ['cycles']
+        return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in names)
 
 
 def check_tts(*groups):
@@ -52,19 +58,35 @@ def outputs(initial_guesses):
     ((buffer, slots), descriptor owner, descriptor pointer).
     """
     dev = initial_guesses[0].device
-    orders, rows, cols, ranks, slots = [], [], [], [], []
-    total = 0
-    for x in initial_guesses:
+    x0 = initial_guesses[0]
+    n = len(initial_guesses)
+
+    def layout(x):
         r = np.asarray(x.ranks, dtype=np.int64)
         sizes = r[:-1] * np.asarray(x.row_dims, dtype=np.int64) * np.asarray(x.col_dims, dtype=np.int64) * r[1:]
-        slots.append(total + np.concatenate(([0], np.cumsum(sizes)[:-1])))
-        total += int(sizes.sum())
-        orders.append(x.order)
-        rows += x.row_dims
-        cols += x.col_dims
-        ranks += x.ranks
+        return np.concatenate(([0], np.cumsum(sizes)[:-1])), int(sizes.sum())
+
+    if all(x.row_dims == x0.row_dims and x.col_dims == x0.col_dims and x.ranks == x0.ranks for x in initial_guesses):
+        offs, per = layout(x0)  # one structure (the usual batch): slots are strided copies
+        slots = np.arange(n, dtype=np.int64)[:, None] * per + offs[None, :]
+        total = n * per
+        orders, rows, cols, ranks = [x0.order] * n, np.tile(x0.row_dims, n), np.tile(x0.col_dims, n), \
+            np.tile(x0.ranks, n)
+        flat = slots.ravel()
+    else:
+        orders, rows, cols, ranks, slots = [], [], [], [], []
+        total = 0
+        for x in initial_guesses:
+            offs, per = layout(x)
+            slots.append(total + offs)
+            total += per
+            orders.append(x.order)
+            rows += x.row_dims
+            cols += x.col_dims
+            ranks += x.ranks
+        flat = np.concatenate(slots)
     buf = torch.empty(max(total, 1), dtype=torch.float64, device=dev)
-    ptrs = buf.data_ptr() + 8 * np.concatenate(slots).astype(np.uint64)
+    ptrs = np.uint64(buf.data_ptr()) + np.uint64(8) * flat.astype(np.uint64)
     b = _lib.DescBatch(orders, rows, cols, ranks, ptrs)
     return (buf, slots), b, b.ptr
 

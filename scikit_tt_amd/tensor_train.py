@@ -72,14 +72,14 @@ def from_host_batch(core_lists):
             off += a.size
     buf = host.to(dev, non_blocking=True)
     torch.cuda.current_stream(dev).synchronize()  # the pinned source must outlive the copy
+    # trains as lazy views: per-core tensors are made only when a core is used
     out, off = [], 0
     for sh in shapes:
-        cores = []
-        for s in sh:
-            n = int(np.prod(s))
-            cores.append(buf[off:off + n].view(s))
-            off += n
-        out.append(TT._from_device(cores))
+        sizes = np.array([int(np.prod(s)) for s in sh], dtype=np.int64)
+        offs = off + np.concatenate(([0], np.cumsum(sizes)[:-1]))
+        off += int(sizes.sum())
+        out.append(TT._from_flat(buf, offs, [s[1] for s in sh], [s[2] for s in sh],
+                                 [s[0] for s in sh] + [sh[-1][3]]))
     return out
 
 
@@ -155,6 +155,8 @@ class TT:
 
     @property
     def device(self):
+        if self.__dict__.get('_cores') is None and self.__dict__.get('_lazy') is not None:
+            return self._lazy[0].device  # without creating the per-core tensors
         return self.device_cores[0].device
 
     def __repr__(self):
@@ -218,11 +220,14 @@ class TT:
 
     def _core_ptrs(self):
         """Device addresses of the cores (without creating per-core tensors)."""
+        return self._core_ptr_array().tolist()
+
+    def _core_ptr_array(self):
+        """Device addresses of the cores as a uint64 array."""
         if self.__dict__.get('_cores') is None and self.__dict__.get('_lazy') is not None:
             buf, offs = self._lazy
-            base = buf.data_ptr()
-            return [base + 8 * int(o) for o in offs]
-        return [c.data_ptr() for c in self.device_cores]
+            return np.uint64(buf.data_ptr()) + np.uint64(8) * np.asarray(offs, dtype=np.uint64)
+        return np.array([c.data_ptr() for c in self.device_cores], dtype=np.uint64)
 
     # -- copies and conversion ---------------------------------------------------
 
