@@ -104,6 +104,22 @@ __device__ __forceinline__ double wave_sum(double v) {
   v += dpp_mov<0x140>(v);
   return (rdlane(v, 0) + rdlane(v, 16)) + (rdlane(v, 32) + rdlane(v, 48));
 }
+// min / max over the wave in every lane (one of the inputs, exactly): DPP
+// stages within each 16-lane row as in wave_sum, then the four row results
+__device__ __forceinline__ double wave_min(double v) {
+  v = fmin(v, dpp_mov<0xB1>(v));
+  v = fmin(v, dpp_mov<0x4E>(v));
+  v = fmin(v, dpp_mov<0x141>(v));
+  v = fmin(v, dpp_mov<0x140>(v));
+  return fmin(fmin(rdlane(v, 0), rdlane(v, 16)), fmin(rdlane(v, 32), rdlane(v, 48)));
+}
+__device__ __forceinline__ double wave_max(double v) {
+  v = fmax(v, dpp_mov<0xB1>(v));
+  v = fmax(v, dpp_mov<0x4E>(v));
+  v = fmax(v, dpp_mov<0x141>(v));
+  v = fmax(v, dpp_mov<0x140>(v));
+  return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
+}
 __device__ __forceinline__ float wave_sumf(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -158,8 +174,7 @@ __device__ double wg_nrm2(const T* x, int n, double* red) {
   // scaled to avoid overflow/underflow of the squares
   double mx = 0.0;
   for (int i = threadIdx.x; i < n; i += NT) mx = fmax(mx, fabs((double)x[i]));
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  mx = wave_max(mx);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) red[w] = mx;

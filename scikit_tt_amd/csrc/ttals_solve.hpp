@@ -372,16 +372,6 @@ __device__ __forceinline__ int sturm_count(const double* al, const double* be, i
 
 
 
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
 
 
 // Sturm count (eigenvalues below x) of a tridiagonal T scaled so that
@@ -428,10 +418,55 @@ __device__ __forceinline__ int sturm_count_lds4(const double* ta_, const double*
   return c;
 }
 
+// p_m(x) / p_m'(x) for p_m = det(T - x I), T and x scaled as for
+// sturm_count_lds4 (rows k < m only: no padding rows).  The derivative runs
+// alongside, p'_{k+1} = (alpha_k - x) p'_k - beta_{k-1}^2 p'_{k-1} - p_k,
+// both rescaled together (the ratio is unchanged).
+__device__ __forceinline__ double newton_ratio_lds4(const double* ta_, const double* tb_, int m, double x) {
+  const TT_LDS double* ta = (const TT_LDS double*)ta_;
+  const TT_LDS double* tb = (const TT_LDS double*)tb_;
+  double p0 = 0.0, p1 = 1.0, d0 = 0.0, d1 = 0.0;
+  for (int k = 0; k < m; k += 4) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      av[q] = ta[k + q];
+      bv[q] = tb[k + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k + q < m) {
+        const double ax = av[q] - x;
+        const double p2 = fma(ax, p1, -bv[q] * p0);
+        const double d2 = fma(ax, d1, -bv[q] * d0) - p1;
+        p0 = p1;
+        p1 = p2;
+        d0 = d1;
+        d1 = d2;
+      }
+    }
+    const double mg = fabs(p1) + fabs(d1);
+    if (mg > 0x1p+600) {
+      p0 *= 0x1p-600;
+      p1 *= 0x1p-600;
+      d0 *= 0x1p-600;
+      d1 *= 0x1p-600;
+    } else if (mg < 0x1p-600 && fabs(p0) + fabs(d0) < 0x1p-600) {
+      p0 *= 0x1p+600;
+      p1 *= 0x1p+600;
+      d0 *= 0x1p+600;
+      d1 *= 0x1p+600;
+    }
+  }
+  return p1 / d1;
+}
+
 // Extreme eigenpair of T (m <= 64; idx = 0 smallest or m-1 largest).
 //  * eigenvalue: multisection over all NT threads (NT+1 subintervals a
-//    round) from the Gershgorin interval down to the rounding level;
-//    invariant count(lo) <= idx < count(hi).  Left in S.scal[1].
+//    round) from the Gershgorin interval (or a hint) to a relative width of
+//    2^-26, invariant count(lo) <= idx < count(hi), then Newton to the
+//    rounding level (multisection all the way if Newton does not settle).
+//    Left in S.scal[1].
 //  * unit eigenvector in s, by a twisted factorization: top-down LDL^T
 //    pivots q+ (wave 0) and bottom-up UDU^T pivots q- (wave 1) run
 //    concurrently; the twist index r minimises
@@ -493,26 +528,58 @@ __device__ TT_COLD bool tri_extreme(const double* al, const double* be, int m_in
       hi = (nle == NT - 1) ? hint_hi : hint_lo + h * (double)nle;
     }
   }
-  for (int round = 0; round < 64; ++round) {
-    if (hi - lo <= 2.2e-16 * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
-    const double h = (hi - lo) * (1.0 / (double)(NT + 1));
-    if (!(lo + h > lo) || !(lo + h * NT < hi)) break;  // spacing below the rounding of lo / hi
-    const double x = lo + h * (double)(threadIdx.x + 1);
-    const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
-    const unsigned long long ball = __ballot(c <= idx);
-    int* cnt = S.cnt + (round & 1) * NWAVE;
-    if (lane == 0) cnt[wv] = __popcll(ball);
-    __syncthreads();
-    int nle = 0;
+  // Multisection to a relative width of 2^-26, then Newton on det(T - x I)
+  // from the bracket end outside the spectrum: every root is real, so the
+  // iterates rise (fall) monotonically to the smallest (largest) eigenvalue,
+  // quadratically once close.  Should Newton not settle within 8 steps (a
+  // near-degenerate pair) or leave the bracket, multisection continues to the
+  // rounding level.  The round counter runs on across both phases (it picks
+  // the alternating count buffer).
+  double th = 0.0;
+  bool have_th = false;
+  int round = 0;
+  for (int phase = 0; phase < 2 && !have_th; ++phase) {
+    const double relw = phase == 0 ? 0x1p-26 : 2.2e-16;
+    for (; round < 128; ++round) {
+      if (hi - lo <= relw * fmax(fabs(lo), fabs(hi)) + 1e-300) break;
+      const double h = (hi - lo) * (1.0 / (double)(NT + 1));
+      if (!(lo + h > lo) || !(lo + h * NT < hi)) break;  // spacing below the rounding of lo / hi
+      const double x = lo + h * (double)(threadIdx.x + 1);
+      const int c = sturm_count_lds4(tw, tw + 64, m4, x * sc);
+      const unsigned long long ball = __ballot(c <= idx);
+      int* cnt = S.cnt + (round & 1) * NWAVE;
+      if (lane == 0) cnt[wv] = __popcll(ball);
+      __syncthreads();
+      int nle = 0;
 #pragma unroll
-    for (int q = 0; q < NWAVE; ++q) nle += cnt[q];
-    // counts are monotone in x: the first nle points lie left of the eigenvalue
-    const double nlo = nle > 0 ? lo + h * (double)nle : lo;
-    const double nhi = nle < NT ? lo + h * (double)(nle + 1) : hi;
-    lo = nlo;
-    hi = nhi;
+      for (int q = 0; q < NWAVE; ++q) nle += cnt[q];
+      // counts are monotone in x: the first nle points lie left of the eigenvalue
+      const double nlo = nle > 0 ? lo + h * (double)nle : lo;
+      const double nhi = nle < NT ? lo + h * (double)(nle + 1) : hi;
+      lo = nlo;
+      hi = nhi;
+    }
+    if (phase == 0 && (idx == 0 || idx == m - 1)) {
+      // every thread runs the same iteration (no exchange needed)
+      double x = (idx == 0 ? lo : hi) * sc;
+      bool ok = false;
+      for (int it = 0; it < 8; ++it) {
+        const double dx = newton_ratio_lds4(tw, tw + 64, m, x);
+        if (!isfinite(dx)) break;
+        x -= dx;
+        if (fabs(dx) <= 0x1p-50 * fabs(x) + 1e-300) {
+          ok = true;
+          break;
+        }
+      }
+      const double t = x / sc;
+      if (ok && t >= lo && t <= hi) {
+        th = t;
+        have_th = true;
+      }
+    }
   }
-  const double th = 0.5 * (lo + hi);
+  if (!have_th) th = 0.5 * (lo + hi);
   double q_l = 0.0, r_l = 0.0;  // this lane's pivot and ratio
   __syncthreads();  // the counts are done with tw
   const TT_LDS double* la = (const TT_LDS double*)al;
