@@ -162,6 +162,28 @@ __device__ __forceinline__ void wg_sum3(double& a, double& b, double& c, double*
   }
 }
 
+// wg_sum3 without the leading barrier: for a `red` that no thread can still
+// be reading, i.e. one whose last readers have passed a barrier since.
+__device__ __forceinline__ void wg_sum3_nb(double& a, double& b, double& c, double* red) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wv] = a;
+    red[NWAVE + wv] = b;
+    red[2 * NWAVE + wv] = c;
+  }
+  __syncthreads();
+  a = b = c = 0.0;
+#pragma unroll
+  for (int q = 0; q < NWAVE; ++q) {
+    a += red[q];
+    b += red[NWAVE + q];
+    c += red[2 * NWAVE + q];
+  }
+}
+
 template <class T>
 __device__ double wg_dot(const T* x, const T* y, int n, double* red) {
   double s = 0.0;

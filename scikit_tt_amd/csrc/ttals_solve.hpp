@@ -40,6 +40,7 @@ constexpr int KMAX = 40;  // Krylov basis size (GMRES restart / Lanczos)
 static_assert(KMAX < 64, "the tridiagonal eigensolver keeps one row of T per lane");
 struct SolveLDS {
   double red[4 * NWAVE];
+  double red2[3 * NWAVE];           // Lanczos pass 1 only (pass 2 uses red)
   double h[KMAX + 2];
   double H[(KMAX + 1) * KMAX];
   double cs[KMAX + 1], sn[KMAX + 1], g[KMAX + 2];
@@ -816,7 +817,9 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           a1 += we * M::ld(cur, e);
           ww += we * we;
         }
-        wg_sum3(a1, ww, dummy, S.red);
+        // the two passes use separate scratch, so each one's last readers have
+        // passed the other's barrier: no leading barrier needed
+        wg_sum3_nb(a1, ww, dummy, S.red2);
         w0 = sqrt(ww);
         // pass 2 fused with the update: c1 = <w', cur>, c0 = <w', prv>, |w'|^2
         double c1 = 0.0, c0 = 0.0, w2 = 0.0;
@@ -827,7 +830,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           c0 += we * M::ld(prv, e);
           w2 += we * we;
         }
-        wg_sum3(c1, c0, w2, S.red);
+        wg_sum3_nb(c1, c0, w2, S.red);
         cc1 = c1;
         cc0 = c0;
         alpha = a1 + c1;
