@@ -116,7 +116,7 @@ int ttals_evp_submit(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A,
 int ttals_query(ttals_ctx* ctx);
 int ttals_wait(ttals_ctx* ctx, ttals_result* res);
 
-/* Size of the on-chip (LDS) arena in KB, 0..62 (default 62): local
+/* Size of the on-chip (LDS) arena in KB, 0..61 (default 61): local
  * problems whose operator and working vectors fit run LDS-resident on the
  * FP64 matrix cores, the others from HBM/L2; 0 forces the global path.
  * Applies to later submissions and pieces on this context.  No counterpart

@@ -181,12 +181,12 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
       if (arena_need_onchip(A.a, A.P, A.n, A.Q, A.b, 3, A.mvar) <= A.arena) {
         onchip_work(W, onchip_operator(op, dyn_lds, A.arena, 3, A.mvar), (int)N, false);
         __syncthreads();
-        r = wg_lanczos<double, true>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+        r = wg_lanczos<double, true>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, LKMAX, lds, S, &lam, &its);
       } else {
-        W.cur = A.V + (long)(KMAX + 1) * N;
-        W.w = A.V + (long)(KMAX + 2) * N;
-        W.r = A.V + (long)(KMAX + 3) * N;
-        r = wg_lanczos<double, false>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, KMAX, lds, S, &lam, &its);
+        W.cur = A.V + (long)(LKMAX + 1) * N;
+        W.w = A.V + (long)(LKMAX + 2) * N;
+        W.r = A.V + (long)(LKMAX + 3) * N;
+        r = wg_lanczos<double, false>(op, A.v, W, A.mode, A.sigma, A.tol, A.maxit, LKMAX, lds, S, &lam, &its);
       }
       if (threadIdx.x == 0) {
         A.res[0] = r;
@@ -443,9 +443,9 @@ struct ttals_ctx {
   std::vector<Pending> pending;
   double* trace = nullptr;   // optional device trace buffer (system 0 of a batch)
   long trace_cap = 0;
-  // on-chip arena of k_als in doubles: 62 KB, which with the ~17 KB of static
-  // LDS keeps two systems resident per CU (160 KB of LDS)
-  long arena = 7936;
+  // on-chip arena of k_als in doubles: 61 KB, which with the ~18.4 KB of
+  // static LDS keeps two systems resident per CU (160 KB of LDS)
+  long arena = 61 * 1024 / 8;
   int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
   int matvec_variant = 0;    // SysDesc::mvar
   std::string err;
@@ -646,7 +646,7 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
     P.w_g = dl.take(8 * P.nmax);
     P.w_t1 = dl.take(8 * P.tmax);
     P.w_t2 = dl.take(8 * P.tmax);
-    P.w_V = dl.take(8 * (size_t)(KMAX + 1) * P.nmax);
+    P.w_V = dl.take(8 * (size_t)(LKMAX + 1) * P.nmax);
     P.w_dense = dl.take(8 * (size_t)(NDENSE * NDENSE + NDENSE));
     P.w_qw = dl.take(8 * P.nmax);
     P.w_qq = dl.take(8 * P.nmax);
@@ -798,7 +798,7 @@ int finish_batch(ttals_ctx* c, ttals_result* res) {
 // scratch for the piece entry points: 2*t + (KMAX+2)*N doubles
 int piece_scratch(ttals_ctx* c, long t, long N, double** t1, double** t2, double** V, double** res) {
   if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
-  const long nv = std::max<long>((KMAX + 4) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
+  const long nv = std::max<long>((LKMAX + 4) * std::max<long>(N, 1), (long)NDENSE * NDENSE + NDENSE);
   const size_t bytes = 8 * (size_t)(2 * t + nv + 16) + 4096;
   int rc = ensure_dws(c, bytes);
   if (rc) return rc;
@@ -841,7 +841,7 @@ int ttals_ctx_create(int device, ttals_ctx** out) {
   c->device = device;
   if (const char* e = std::getenv("TTALS_ARENA_KB")) {
     const long kb = std::atol(e);
-    if (kb >= 0 && kb <= 64) c->arena = kb * 1024 / 8;
+    if (kb >= 0 && kb <= 61) c->arena = kb * 1024 / 8;
   }
   *out = c;
   return 0;
@@ -894,7 +894,7 @@ int ttals_wait(ttals_ctx* c, ttals_result* res) {
 }
 
 int ttals_set_arena_kb(ttals_ctx* c, int kb) {
-  if (!c || kb < 0 || kb > 62) return TTALS_E_ARG;
+  if (!c || kb < 0 || kb > 61) return TTALS_E_ARG;
   if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
   c->arena = (long)kb * 1024 / 8;
   return 0;

@@ -396,10 +396,8 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
-    const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, D.gm_restart, lds, S, lam,
-                                                 &its)
-                         : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, D.gm_restart, lds, S,
-                                                  lam, &its);
+    const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, LKMAX, lds, S, lam, &its)
+                         : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, LKMAX, lds, S, lam, &its);
     // matvecs + two Gram-Schmidt passes against on average half the basis
     st.flops += its * fmv + (double)its * 8.0 * N * 0.5 * (its < D.gm_restart ? its : D.gm_restart);
     if (r == 3) return ST_UNSUPPORTED;

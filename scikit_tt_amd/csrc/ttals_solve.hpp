@@ -36,17 +36,19 @@ __device__ __forceinline__ void apply_op(const LocalOp<T>& op, const T* v, T* y,
     local_matvec<T, ACC>(op.a, op.P, op.n, op.Q, op.b, op.L, op.A, op.R, v, y, op.t1, op.t2, lds);
 }
 
-constexpr int KMAX = 40;  // Krylov basis size (GMRES restart / Lanczos)
-static_assert(KMAX < 64, "the tridiagonal eigensolver keeps one row of T per lane");
+constexpr int KMAX = 40;   // GMRES restart length (its Hessenberg matrix lives in LDS)
+constexpr int LKMAX = 63;  // Lanczos restart length (basis in global memory, T in LDS)
+static_assert(LKMAX < 64, "the tridiagonal eigensolver keeps one row of T per lane");
+static_assert(KMAX <= LKMAX, "the basis and its scratch are sized for the longer restart");
 struct SolveLDS {
   double red[4 * NWAVE];
   double red2[3 * NWAVE];           // Lanczos pass 1 only (pass 2 uses red)
-  double h[KMAX + 2];
+  double h[LKMAX + 2];
   double H[(KMAX + 1) * KMAX];
-  double cs[KMAX + 1], sn[KMAX + 1], g[KMAX + 2];
-  double alpha[KMAX + 1], beta[KMAX + 1], s[KMAX + 1];
+  double cs[LKMAX + 1], sn[KMAX + 1], g[KMAX + 2];
+  double alpha[LKMAX + 1], beta[LKMAX + 1], s[LKMAX + 1];
   double scal[4];
-  double part[(KMAX + 2) * NWAVE];  // per-wave partial dot products
+  double part[(LKMAX + 2) * NWAVE];  // per-wave partial dot products
   long long prof[8];                // cycle counters per phase (thread 0)
   int flag[4];
   int cnt[2 * NWAVE];               // per-wave ballot counts (double-buffered)
@@ -750,8 +752,8 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
   const int N = op.size();
   double* V = W.V;
   double *prv = W.r, *cur = W.cur, *w = W.w;
-  if (kmax > KMAX) kmax = KMAX;
-  const bool full_orth = (N <= KMAX);
+  if (kmax > LKMAX) kmax = LKMAX;
+  const bool full_orth = (N <= LKMAX);
   // start vector into cur (one pass over v, which may be in global memory);
   // every vector loop below gives element e to thread e mod NT
   double vn;

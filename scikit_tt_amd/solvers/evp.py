@@ -10,7 +10,7 @@ The local eigenproblem is solved matrix-free by restarted Lanczos with full
 re-orthogonalisation (operator symmetric slice by slice required):
 'eigh' -> largest eigenvalue (evp.py:434-439); 'eig' -> eigenvalue nearest
 sigma (evp.py:424-432), available when sigma lies below/above the local
-spectrum or the local problem fits the Krylov space (48).  The in-sweep
+spectrum or the local problem fits the Krylov space (63).  The in-sweep
 orthonormalisation is a Householder QR; it spans the same space as the
 reference's leading singular vectors (evp.py:452-487) for full-rank cores.
 """

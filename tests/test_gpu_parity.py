@@ -40,7 +40,7 @@ def empty(*shape):
     return torch.empty(shape, dtype=torch.float64, device='cuda')
 
 
-@pytest.fixture(params=[(62, 0), (62, 1), (0, 0)], ids=['onchip', 'onchip3', 'global'])
+@pytest.fixture(params=[(61, 0), (61, 1), (0, 0)], ids=['onchip', 'onchip3', 'global'])
 def arena(request, env):
     """Run a piece on the engine's on-chip (LDS, matrix-core) path where the
     problem fits -- two-stage matvec (operator core merged with the right
@@ -50,7 +50,7 @@ def arena(request, env):
     assert lib.ttals_set_arena_kb(ctx, kb) == 0
     assert lib.ttals_set_matvec_variant(ctx, variant) == 0
     yield kb
-    assert lib.ttals_set_arena_kb(ctx, 62) == 0
+    assert lib.ttals_set_arena_kb(ctx, 61) == 0
     assert lib.ttals_set_matvec_variant(ctx, 0) == 0
 
 
