@@ -956,6 +956,44 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           tri_eigvec(S.alpha, S.beta, m, th, S.s, S.cs);
           have_vec = true;
           res = 0.0;
+        } else if (m == 2) {
+          // 2x2 T (every local solve's first check): closed form.  The
+          // eigenvalue of larger magnitude from the mean and the half-gap, the
+          // other from the determinant divided by it (no cancellation); the
+          // eigenvector from whichever row of T - th I gives the longer null
+          // vector, its larger component made positive (the twisted
+          // factorization's convention).
+          const double a = S.alpha[0], c = S.alpha[1], b = S.beta[0];
+          const double sm = a + c, rt = hypot(a - c, 2.0 * b);
+          const double acmx = fabs(a) > fabs(c) ? a : c, acmn = fabs(a) > fabs(c) ? c : a;
+          double rt1, rt2;
+          if (sm != 0.0) {
+            rt1 = 0.5 * (sm + copysign(rt, sm));
+            rt2 = (acmx / rt1) * acmn - (b / rt1) * b;
+          } else {
+            rt1 = 0.5 * rt;
+            rt2 = -0.5 * rt;
+          }
+          th = (idx == 0) ? fmin(rt1, rt2) : fmax(rt1, rt2);
+          double s0 = b, s1 = th - a;
+          const double u0 = th - c, u1 = b;
+          if (u0 * u0 + u1 * u1 > s0 * s0 + s1 * s1) {
+            s0 = u0;
+            s1 = u1;
+          }
+          double nn = sqrt(s0 * s0 + s1 * s1);
+          if (!(nn > 0.0)) {  // a == c, b == 0: any vector
+            s0 = 1.0;
+            s1 = 0.0;
+            nn = 1.0;
+          }
+          const double sg = (fabs(s0) >= fabs(s1) ? s0 : s1) < 0.0 ? -1.0 : 1.0;
+          if (threadIdx.x == 0) {
+            S.s[0] = sg * s0 / nn;
+            S.s[1] = sg * s1 / nn;
+          }
+          have_vec = true;
+          res = (brk || full) ? 0.0 : bn * fabs(s1 / nn);
         } else {
           // extreme eigenvalue: multisection to full precision, eigenvector
           // by twisted factorization, residual estimate bn |s_m|
