@@ -845,10 +845,11 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           cc1 = cc0 = 0.0;
           __syncthreads();
         }
-        if (!dgks && threadIdx.x == 0) {
-          S.alpha[j] = alpha;
-          S.beta[j] = 0.0;
-        }
+        // every thread holds the same alpha (fixed-order sums) and writes it:
+        // each one's later reads of row j then need no barrier.  (Each entry
+        // is written once, with its final value: a provisional value could
+        // land after another wave's final one.)
+        if (!dgks) S.alpha[j] = alpha;
       }
       if (dgks) {
         // w = A cur - (alpha cur + beta prv) (local pass already applied when !full_orth)
@@ -861,12 +862,10 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           alpha += S.H[j];
         }
         bn = nb;
-        if (threadIdx.x == 0) {
-          S.alpha[j] = alpha;
-          S.beta[j] = 0.0;
-        }
+        S.alpha[j] = alpha;
       }
-      __syncthreads();
+      // no barrier: w is next read elementwise by its own threads (the
+      // normalisation), row j of T by threads that wrote it themselves
       ph_add(S, PH_ORTH, t0);
       ++*iters;
       m = j + 1;
@@ -878,6 +877,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       if (brk && !full && m < kk) {
         // invariant subspace: continue with a fresh direction orthogonal to
         // the basis (T stays tridiagonal with a zero coupling)
+        S.beta[j] = 0.0;
         for (int e = threadIdx.x; e < N; e += NT)
           M::st(w, e, (double)((((unsigned)e + 1u) * 2654435761u + ((unsigned)j + 1u) * 40503u) % 1009u) / 1009.0 -
                           0.5);
@@ -898,8 +898,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         __syncthreads();
         continue;
       }
-      if (threadIdx.x == 0) S.beta[j] = brk ? 0.0 : bn;
-      __syncthreads();
+      S.beta[j] = brk ? 0.0 : bn;  // by every thread, as alpha
       // convergence checks at m = 2, then every 4 steps, spaced out while the
       // residual estimate is still far from the tolerance
       const bool check = full || brk || (m == kk) || m == next_check || *iters >= maxit;
@@ -988,10 +987,10 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
             nn = 1.0;
           }
           const double sg = (fabs(s0) >= fabs(s1) ? s0 : s1) < 0.0 ? -1.0 : 1.0;
-          if (threadIdx.x == 0) {
-            S.s[0] = sg * s0 / nn;
-            S.s[1] = sg * s1 / nn;
-          }
+          // written by every thread (identical values): a converged check
+          // goes straight to the Ritz vector, which reads s without a barrier
+          S.s[0] = sg * s0 / nn;
+          S.s[1] = sg * s1 / nn;
           have_vec = true;
           res = (brk || full) ? 0.0 : bn * fabs(s1 / nn);
         } else {

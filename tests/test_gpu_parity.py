@@ -452,6 +452,29 @@ def test_evp_adaptive_tolerance_matches_fixed(env):
     assert again[0][0] == fixed[0][0]
 
 
+def test_evp_batch_rerun_bit_identical(env):
+    """The engine is deterministic: the same batch twice gives bit-identical
+    eigenvalues, ranks and cores (a data race between the waves of a system
+    shows up here as run-to-run differences)."""
+    from scikit_tt_amd.tensor_train import from_host_batch
+    import scikit_tt_amd.solvers.evp as evp
+    lists = []
+    for s in range(128):
+        H, y0, J, g = W.tfim_chain(seed=100 + s, d=16, rank=12)
+        lists += [H, W.right_orthonormal(y0)]
+    tts = from_host_batch(lists)
+    ops, xs = tts[0::2], tts[1::2]
+    kw = dict(repeats=4, conv_eps=0.0, solver='eig', sigma=-100.0)
+    first = evp.als_batch(ops, xs, **kw)
+    for rep in range(2):
+        again = evp.als_batch(ops, xs, **kw)
+        for (l1, t1, _), (l2, t2, _) in zip(first, again):
+            assert l1 == l2
+            assert t1.ranks == t2.ranks
+            for c1, c2 in zip(t1.device_cores, t2.device_cores):
+                assert torch.equal(c1, c2)
+
+
 def test_sle_laplace_large_full_size(env):
     """C3 at its real size (d=20, n=32, TT rank 32, N = 32768 per interior
     local problem, global-memory path): two sweeps reach rel.res 2e-9
