@@ -526,14 +526,20 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
+  // no integer divisions below (each expands to a long instruction sequence):
+  // tile rows by an FP32-reciprocal quotient (exact for these small counts),
+  // the K-split cases by shifts
+  static_assert(NWAVE == 4, "the K-split cases assume four waves");
   const int tn = (N + 15) >> 4, tiles = ((M + 15) >> 4) * tn;
+  const float itn = 1.0f / (float)tn;
   const int ks = (K + 3) >> 2;
-  int groups = 1;
-  if (2 * tiles <= NWAVE && NWAVE % tiles == 0) {  // then every wave gets exactly one (tile, part)
-    groups = NWAVE / tiles;
-    if ((groups - 1) * tiles * 256 > red_cap) groups = 1;
+  int gl = 0;  // log2 of the K groups: tiles == 1 -> 4 groups, tiles == 2 -> 2 (every wave one (tile, part))
+  if (tiles <= 2) {
+    gl = tiles == 1 ? 2 : 1;
+    if (((1 << gl) - 1) * tiles * 256 > red_cap) gl = 0;
   }
-  const int per = (ks + groups - 1) / groups;
+  const int groups = 1 << gl;
+  const int per = (ks + groups - 1) >> gl;
   const int acs4 = acs << 2, brs4 = brs << 2;
   if (groups == 1 && tiles > NWAVE) {
     // more tiles than waves: a wave runs tiles u and u + NWAVE together, their
@@ -547,7 +553,7 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
       bool jv[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        ti[h] = tt[h] / tn;
+        ti[h] = (int)(((float)tt[h] + 0.5f) * itn);
         const int tj = tt[h] - ti[h] * tn;
         const int i = (ti[h] << 4) + li;
         jj[h] = (tj << 4) + li;
@@ -613,8 +619,10 @@ __device__ void mf_gemm_lin(int M, int N, int K, const double* A_, int ars, int 
     return;
   }
   for (int u = w; u < tiles * groups; u += NWAVE) {
-    const int t = u % tiles, part = u / tiles;
-    const int ti = t / tn, tj = t - ti * tn;
+    // groups > 1 only with tiles 1 or 2
+    const int t = gl == 0 ? u : (tiles == 1 ? 0 : (u & 1));
+    const int part = gl == 0 ? 0 : (tiles == 1 ? u : (u >> 1));
+    const int ti = (int)(((float)t + 0.5f) * itn), tj = t - ti * tn;
     const int i = (ti << 4) + li, j = (tj << 4) + li;
     const bool iv = i < M, jv = j < N;
     const int s0 = part * per, s1 = min(ks, s0 + per);
