@@ -485,6 +485,9 @@ __device__ __forceinline__ int fmap_at(const FMap& f, int i) {
   return __mul24(q, f.hi) + __mul24(i - __mul24(q, f.div), f.lo);
 }
 
+// floor(i / div) for 0 <= i < 2^22 with the FP32 reciprocal of div (see FMap)
+__device__ __forceinline__ int fquot(int i, float inv) { return (int)(((float)i + 0.5f) * inv); }
+
 // Output maps of the first two stages (computed once per local problem).
 struct MvPlan {
   FMap c1r, c1c, c2r, c2c;
@@ -497,10 +500,11 @@ __device__ __forceinline__ MvPlan mv_plan(int a, int P, int n, int Q, int b) {
 // The operator core (P, m, n, Q) copied into LDS as A'(P, n, m, Q).
 __device__ __forceinline__ void wg_copy_opcore_pnmq(const double* src, double* dst, int P, int n, int Q) {
   const int tot = P * n * n * Q;
+  const float iQ = 1.0f / (float)Q, in = 1.0f / (float)n;
   for (int e = threadIdx.x; e < tot; e += NT) {
-    const int q = e % Q, r1 = e / Q;
-    const int nn = r1 % n, r2 = r1 / n;
-    const int mm = r2 % n, p = r2 / n;
+    const int r1 = fquot(e, iQ), q = e - r1 * Q;
+    const int r2 = fquot(r1, in), nn = r1 - r2 * n;
+    const int p = fquot(r2, in), mm = r2 - p * n;
     dst[((p * n + nn) * n + mm) * Q + q] = src[e];
   }
 }
@@ -769,20 +773,22 @@ __device__ __attribute__((noinline)) void stack_left_mf(int a, int P, int n, int
 // stage-2 layout of the mirrored problem (P and Q swapped).
 __device__ __forceinline__ void wg_copy_opcore_qnmp(const double* src, double* dst, int P, int n, int Q) {
   const int tot = P * n * n * Q;
+  const float iQ = 1.0f / (float)Q, in = 1.0f / (float)n;
   for (int e = threadIdx.x; e < tot; e += NT) {
-    const int q = e % Q, r1 = e / Q;
-    const int nn = r1 % n, r2 = r1 / n;
-    const int mm = r2 % n, p = r2 / n;
+    const int r1 = fquot(e, iQ), q = e - r1 * Q;
+    const int r2 = fquot(r1, in), nn = r1 - r2 * n;
+    const int p = fquot(r2, in), mm = r2 - p * n;
     dst[((q * n + nn) * n + mm) * P + p] = src[e];
   }
 }
 
-// core (a, n, b) copied into LDS transposed to (b, n, a)
+// core (a, n, b) copied into LDS transposed to (b, n, a) (a n b < 2^22)
 __device__ __forceinline__ void wg_copy_core_t(const double* src, double* dst, int a, int n, int b) {
   const int tot = a * n * b;
+  const float ib = 1.0f / (float)b, in = 1.0f / (float)n;
   for (int e = threadIdx.x; e < tot; e += NT) {
-    const int bb = e % b, r = e / b;
-    const int nn = r % n, aa = r / n;
+    const int r = fquot(e, ib), bb = e - r * b;
+    const int aa = fquot(r, in), nn = r - aa * n;
     dst[(bb * n + nn) * a + aa] = src[e];
   }
 }
@@ -803,9 +809,6 @@ __host__ __device__ __forceinline__ int m2_red_need(int a, int nb) {
   const int tiles = ((a + 15) >> 4) * ((nb + 15) >> 4);
   return (2 * tiles <= NWAVE && NWAVE % tiles == 0) ? (NWAVE / tiles - 1) * tiles * 256 : 0;
 }
-
-// floor(i / div) for 0 <= i < 2^22 with the FP32 reciprocal of div (see FMap)
-__device__ __forceinline__ int fquot(int i, float inv) { return (int)(((float)i + 0.5f) * inv); }
 
 // M2 from the operator core A (P, m, n, Q) and the right stack R (b, Q, b') in
 // global memory, both staged through `tmp` (b Q b + P n n Q doubles).  Ends
@@ -1098,8 +1101,9 @@ __device__ void wg_qr_small_t(double* W, int m, int c, double* tau, double* Qm, 
       if (jj >= k && jj < c && i < m) W[i * c + jj] = a[q][u];
     }
   __syncthreads();
+  const float ic = 1.0f / (float)c;
   for (int e = threadIdx.x; e < k * c; e += NT) {
-    const int i = e / c, jj = e - (e / c) * c;
+    const int i = fquot(e, ic), jj = e - i * c;
     Rm[e] = (jj >= i) ? W[i * c + jj] : 0.0;
   }
   // explicit Q = H_0 ... H_{k-1} I by backward accumulation; H_j leaves the

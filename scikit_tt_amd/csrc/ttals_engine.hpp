@@ -90,8 +90,14 @@ __device__ int qr_factor(const double* v, int m, int c, bool transposed, const Q
   const long mc = (long)m * c;
   if (!transposed) {
     for (long e = threadIdx.x; e < mc; e += NT) W[e] = v[e];
-  } else {
+  } else if (mc < (1L << 22)) {
     // W[i][j] = v[j][i], v is (c x m)
+    const float ic = 1.0f / (float)c;
+    for (int e = threadIdx.x; e < (int)mc; e += NT) {
+      const int i = fquot(e, ic), j = e - i * c;
+      W[e] = v[(long)j * m + i];
+    }
+  } else {
     for (long e = threadIdx.x; e < mc; e += NT) {
       const int i = (int)(e / c), j = (int)(e % c);
       W[e] = v[(long)j * m + i];
@@ -439,9 +445,18 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     const int k = qr_factor(D.g, n * b, a, true, B, S);
     st.flops += fl_qr(n * b, a) + 2.0 * D.rx[i - 1] * D.n[i - 1] * k * a;
     // core i (k, n*b) = Q^T, Q is (n*b x k)
-    for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
-      const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
-      D.xc[i][e] = B.Q[(long)c * k + r];
+    if ((long)k * n * b < (1L << 22)) {
+      const int nb = n * b;
+      const float inb = 1.0f / (float)nb;
+      for (int e = threadIdx.x; e < k * nb; e += NT) {
+        const int r = fquot(e, inb), c = e - r * nb;
+        D.xc[i][e] = B.Q[(long)c * k + r];
+      }
+    } else {
+      for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
+        const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
+        D.xc[i][e] = B.Q[(long)c * k + r];
+      }
     }
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
     // g[(c0 n0), k] = core_{i-1}[(c0 n0), a] * R[k, a]^T
