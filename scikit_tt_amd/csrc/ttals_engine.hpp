@@ -82,6 +82,15 @@ __device__ __forceinline__ QRBuf qr_buffers(const SysDesc& D, int m, int c, doub
   return QRBuf{D.qw, D.qq, D.rf, D.tau};
 }
 
+// Room for `len` more doubles in the arena after the on-chip QR buffers of an
+// (m x c) factorisation, or nullptr (QR buffers in global memory, or no room).
+__device__ __forceinline__ double* qr_stage(const QRBuf& B, int m, int c, long len, double* arena, long cap) {
+  if (B.W != arena) return nullptr;
+  const long k = m < c ? m : c;
+  const long used = (long)m * c + (long)m * k + k * c + k;
+  return used + len <= cap ? arena + used : nullptr;
+}
+
 // v is (m x c) row-major (forward) or (c x m) row-major transposed
 // (backward).  Householder QR of the (m x c) view: explicit Q (m x k) into
 // B.Q, the clean upper-triangular R (k x c) into B.R.  Returns k.
@@ -432,10 +441,20 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     const QRBuf B = qr_buffers(D, a * n, b, reinterpret_cast<double*>(lds), arena);
     const int k = qr_factor(D.g, a * n, b, false, B, S);
     wg_copy(B.Q, D.xc[i], a * n * k);
-    const int n1 = D.n[i + 1], c1 = D.rx[i + 2];
+    const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
     st.flops += fl_qr(a * n, b) + 2.0 * k * n1 * c1 * b;
-    ds_gemm<double, ACC>(k, n1 * c1, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], n1 * c1),
-                         mrm<double>(D.g, n1 * c1));
+    // warm start g = R core_{i+1}: on the matrix cores when R is on-chip and
+    // the next core, the product and the K-split scratch fit beside it
+    double* xs = qr_stage(B, a * n, b, (long)b * nc + (long)k * nc + 768, reinterpret_cast<double*>(lds), arena);
+    if (xs) {
+      double* gs = xs + (long)b * nc;
+      wg_copy(D.xc[i + 1], xs, b * nc);
+      __syncthreads();
+      mf_gemm_lin(k, nc, b, B.R, b, 1, xs, nc, 1, gs, fmap(1, nc, 0), fmap(1, 1, 0), gs + (long)k * nc, 768);
+      wg_copy(gs, D.g, k * nc);
+    } else {
+      ds_gemm<double, ACC>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc));
+    }
     if (threadIdx.x == 0) D.rx[i + 1] = k;
     __syncthreads();
   } else if (i > 0) {
@@ -458,9 +477,18 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
         D.xc[i][e] = B.Q[(long)c * k + r];
       }
     }
-    const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
-    // g[(c0 n0), k] = core_{i-1}[(c0 n0), a] * R[k, a]^T
-    ds_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+    const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
+    // g[(c0 n0), k] = core_{i-1}[(c0 n0), a] * R[k, a]^T (matrix cores as above)
+    double* xs = qr_stage(B, n * b, a, (long)cn * a + (long)cn * k + 768, reinterpret_cast<double*>(lds), arena);
+    if (xs) {
+      double* gs = xs + (long)cn * a;
+      wg_copy(D.xc[i - 1], xs, cn * a);
+      __syncthreads();
+      mf_gemm_lin(cn, k, a, xs, a, 1, B.R, 1, a, gs, fmap(1, k, 0), fmap(1, 1, 0), gs + (long)cn * k, 768);
+      wg_copy(gs, D.g, cn * k);
+    } else {
+      ds_gemm<double, ACC>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+    }
     if (threadIdx.x == 0) D.rx[i] = k;
     __syncthreads();
   } else {
