@@ -31,7 +31,7 @@ __all__ = ['Server', 'footprint_bytes']
 
 _SLE_SOLVERS = {'solve': 0, 'lu': 1}
 _EVP_SOLVERS = {'eig': 0, 'eigh': 1}
-_KMAX = 40
+_KBASIS = 64  # Krylov basis vectors the engine plans per system (Lanczos restart 63, + 1)
 _NDENSE = 96
 
 
@@ -39,7 +39,7 @@ def footprint_bytes(kind, A, x0, b=None):
     """Device bytes one system needs in a launch: stacks, scratch, Krylov basis, output cores.
 
     Mirrors the engine's planner (interface stacks on every bond, two
-    contraction intermediates of N*max(R, p), the (KMAX+1) x N basis, QR and
+    contraction intermediates of N*max(R, p), the 64 x N Krylov basis, QR and
     LU scratch) plus the output cores.
     """
     d = x0.order
@@ -51,7 +51,7 @@ def footprint_bytes(kind, A, x0, b=None):
     nmax = max(N)
     tmax = max(N[i] * max(R[i], R[i + 1], p[i], p[i + 1]) for i in range(d))
     rmax = max(r)
-    scratch = 4 * nmax + 2 * tmax + (_KMAX + 1) * nmax + _NDENSE * (_NDENSE + 1) + rmax * (rmax + 1)
+    scratch = 4 * nmax + 2 * tmax + _KBASIS * nmax + _NDENSE * (_NDENSE + 1) + rmax * (rmax + 1)
     cores = sum(N)
     best = cores if kind == 'evp' else 0
     return 8 * (2 * stacks + scratch + 2 * cores + best) + 4096

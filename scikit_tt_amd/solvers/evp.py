@@ -6,8 +6,10 @@ Same signature, defaults and return convention ``(eigenvalue, TT,
 iterations)``, the best-so-far pair by |lambda - sigma| (evp.py:151-155)
 and the same convergence rule (evp.py:158-165).
 
-The local eigenproblem is solved matrix-free by restarted Lanczos with full
-re-orthogonalisation (operator symmetric slice by slice required):
+The local eigenproblem is solved matrix-free by restarted Lanczos (restart
+length 63; each step re-orthogonalised in two passes against the two previous
+vectors, fully against the basis on severe cancellation; operator symmetric
+slice by slice required):
 'eigh' -> largest eigenvalue (evp.py:434-439); 'eig' -> eigenvalue nearest
 sigma (evp.py:424-432), available when sigma lies below/above the local
 spectrum or the local problem fits the Krylov space (63).  The in-sweep

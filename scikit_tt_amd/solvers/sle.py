@@ -8,7 +8,7 @@ solves many independent systems of uneven order and rank in one launch.
 
 Local solves: LU with partial pivoting (as dgesv, sle.py:506) for local size
 <= 96, matrix-free CG (operator symmetric slice by slice; falls back to
-GMRES if p'Mp <= 0) or GMRES(48) above, to relative residual ``tol``.
+GMRES if p'Mp <= 0) or GMRES(40) above, to relative residual ``tol``.
 The initial guess is right-orthonormalised on the device before the right
 stacks are built; ALS is gauge invariant, so this changes nothing in exact
 arithmetic and keeps the stacks well scaled.
