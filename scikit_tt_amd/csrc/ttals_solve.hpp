@@ -1024,9 +1024,11 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         // check reaches the goal (80 % of the way), 2..10 steps ahead
         int step = 4;
         if (have_hint && res < res_prev && m > m_prev && res > 0.0) {
-          const double rate = log(res_prev / res) / (double)(m - m_prev);
-          const double need = log(res / goal) / rate;
-          step = (int)fmin(10.0, fmax(2.0, floor(0.8 * need)));
+          // steps to go = (m - m_prev) log(res / goal) / log(res_prev / res); an
+          // estimate, so FP32 logarithms (overflow to inf only clamps to 10)
+          const float lr = __log2f((float)(res_prev / res));
+          const float need = __log2f((float)(res / goal)) * (float)(m - m_prev) / lr;
+          step = (int)fminf(10.0f, fmaxf(2.0f, floorf(0.8f * need)));
         } else if (res > 1e6 * goal && m >= 10) {
           step = 8;
         }
