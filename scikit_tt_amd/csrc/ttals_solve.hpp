@@ -346,9 +346,10 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
 }
 
 // ---------------------------------------------------------------------------
-// Eigenvalues of a symmetric tridiagonal matrix (alpha, beta), m <= KMAX.
-// Sturm-count multisection by one wave (64 shifts per round), then inverse
-// iteration for the eigenvector.
+// Eigenvalues of a symmetric tridiagonal matrix (alpha, beta), m <= LKMAX.
+// tri_eigval / tri_eigvec (interior eigenvalues): Sturm-count multisection by
+// one wave (64 shifts per round), then inverse iteration for the eigenvector.
+// tri_extreme (extreme eigenpairs, every convergence check): below.
 
 // Sturm count #{eigenvalues < x} from the LDL^T pivots; the division by the
 // previous pivot uses the hardware reciprocal refined by two Newton steps
