@@ -171,6 +171,34 @@ def test_local_eig(env, arena, N_shape, mode):
     assert np.linalg.norm(M @ x - lam.value * x) <= 1e-8 * np.abs(w).max()
 
 
+def test_local_eig_restarts(env, arena):
+    """A lowest eigenvalue with a small relative gap needs several Lanczos
+    restart cycles (63 steps each); the restarted solve converges to it.
+    The local operator is a Kronecker product of diagonal stacks and core,
+    so its spectrum is known exactly."""
+    lib, ctx, _ = env
+    a, P, n, Q, b = 4, 1, 8, 1, 16
+    L = np.zeros((a, 1, a))
+    L[:, 0, :] = np.diag([1.0, 1.3, 1.7, 2.0])
+    Ac = np.zeros((1, n, n, 1))
+    Ac[0, :, :, 0] = np.diag(np.linspace(1.0, 3.0, n))
+    R = np.zeros((b, 1, b))
+    R[:, 0, :] = np.diag(np.concatenate(([1.0, 1.05], np.linspace(1.2, 4.0, b - 2))))
+    M = sle_als.micro_matrix(L, Ac, R)
+    w = np.linalg.eigvalsh(M)
+    v = dev(np.random.default_rng(3).standard_normal(M.shape[0]))
+    lam, its = ctypes.c_double(), ctypes.c_int()
+    keep = [dev(L), dev(Ac), dev(R)]
+    rc = lib.ttals_local_eig(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], ptr(v), 0, 0.0,
+                             1e-12, 4000, ctypes.byref(lam), ctypes.byref(its))
+    assert rc == 0
+    assert its.value > 63  # at least one restart
+    assert abs(lam.value - w[0]) <= 1e-10 * np.abs(w).max()
+    x = v.cpu().numpy()
+    assert abs(np.linalg.norm(x) - 1) < 1e-12
+    assert np.linalg.norm(M @ x - lam.value * x) <= 1e-8 * np.abs(w).max()
+
+
 def test_local_eig_nearest_interior_small(env):
     """Local problem inside the Krylov space: the eigenvalue nearest an interior sigma (evp.py:429)."""
     lib, ctx, _ = env
