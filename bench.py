@@ -149,8 +149,8 @@ def main():
                          'reference for hardware byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU only: every box here has one MI355X; the multi-GPU layout is '
-              'described in DESIGN.md (independent systems shard across GPUs without communication).',
+        print('bench.py measures one GPU only: every box here has one MI355X. Across GPUs the '
+              'independent systems of a batch would shard one slice per GPU with no communication.',
               file=sys.stderr)
         sys.exit(2)
 
