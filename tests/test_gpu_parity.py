@@ -342,6 +342,49 @@ def test_sle_laplace_small_vs_oracle(env):
         assert tt_ops.norm(tt_ops.sub(x.cores[:], xr)) / tt_ops.norm(xr) < 1e-7
 
 
+def _short_train(d):
+    """A symmetric positive definite operator of order d, x0 and b (order 1:
+    one dense 5x5 core; orders 2, 3: the Laplacian with n=4, rank 2)."""
+    if d > 1:
+        return W.laplace_small(d=d, n=4, rank=2)
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((5, 5))
+    M = M @ M.T + 5.0 * np.eye(5)
+    return [M.reshape(1, 5, 5, 1)], [rng.standard_normal((1, 5, 1, 1))], [np.ones((1, 5, 1, 1))]
+
+
+@pytest.mark.parametrize('d', [1, 2, 3])
+def test_short_trains_vs_oracle(env, d):
+    """Orders 1-3 (one core: no forward step, one local solve per sweep):
+    sle and evp ('eig' nearest sigma, 'eigh' largest) against the oracle."""
+    import scikit_tt_amd.solvers.sle as sle
+    import scikit_tt_amd.solvers.evp as evp
+    A, x0, b = _short_train(d)
+    for reps in (1, 2):
+        x = sle.als(_tt(A), _tt(x0), _tt(b), repeats=reps)
+        xr = sle_als.als(A, x0, b, repeats=reps)
+        assert x.ranks == tt_ops.ranks_of(xr)
+        # compared as full tensors: for one core, x - y of the reference writes
+        # both operands into the same 1x1 rank block (tensor_train.py:313-333)
+        # and returns -y, which the device addition reproduces
+        xd, xrd = x.matricize().ravel(), tt_ops.matricize(xr).ravel()
+        assert np.linalg.norm(xd - xrd) / np.linalg.norm(xrd) < 1e-10
+        np.testing.assert_allclose((x - _tt(xr)).matricize().ravel(),
+                                   tt_ops.matricize(tt_ops.sub(x.cores[:], xr)).ravel(), rtol=1e-14, atol=1e-15)
+    dense = tt_ops.matricize(A)
+    for solver, sigma in (('eig', -100.0), ('eigh', 1.0)):
+        lam, y, it = evp.als(_tt(A), _tt(x0), repeats=3, conv_eps=0, solver=solver, sigma=sigma)
+        lr, yr, itr = evp_als.als(A, x0, repeats=3, conv_eps=0, solver=solver, sigma=sigma)
+        assert it == itr
+        assert abs(lam - lr) < 1e-11 * abs(lr), (solver, lam, lr)
+        # eigenvectors agree up to sign
+        ov = abs(float(y.matricize().ravel() @ tt_ops.matricize(yr).ravel()))
+        assert abs(ov - 1.0) < 1e-9, (solver, ov)
+        if d == 1:  # one core: the exact extreme eigenvalue
+            ev = np.linalg.eigvalsh(dense)
+            assert abs(lam - (ev[0] if solver == 'eig' else ev[-1])) < 1e-12 * abs(lam)
+
+
 def test_evp_tfim_chain_full_size(env):
     """TFIM chain d=32, rank 16, 6 sweeps: lowest eigenvalue vs the free-fermion closed form."""
     import scikit_tt_amd.solvers.evp as evp
@@ -360,10 +403,11 @@ def test_evp_tfim_chain_full_size(env):
 
 
 def test_batch_equals_alone(env):
-    """Each system's result is the same alone and in a mixed batch (one workgroup per system)."""
+    """Each system's result is the same alone and in a mixed batch of uneven orders (one workgroup per system)."""
     import scikit_tt_amd.solvers.sle as sle
     systems = [W.shifted_batch_instance(s, d_range=(4, 8), n_choices=(2, 4), r_range=(2, 6)) for s in range(6)]
     systems.append(W.laplace_small(d=6, n=5, rank=3))
+    systems += [_short_train(1), _short_train(2)]  # ragged orders down to one core
     As = [_tt(A) for A, _, _ in systems]
     Xs = [_tt(x) for _, x, _ in systems]
     Bs = [_tt(b) for _, _, b in systems]
