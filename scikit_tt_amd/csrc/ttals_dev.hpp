@@ -931,9 +931,89 @@ __device__ void micro_dense(int a, int P, int n, int Q, int b, const T* L, const
 // Householder QR of a row-major m x c matrix W (ld = c), in place.
 // On exit W holds R on/above the diagonal and the reflectors below it
 // (unit leading entry implied); tau[0..k) with k = min(m, c).
+// Applies H = I - t v v^T (v: rows j..m-1 of column 0 of `vcol`, leading
+// dimension ldv, v_j = 1 implied) to columns c0..c1-1 of the row-major matrix
+// X (leading dimension ldx), rows j..m-1.  X and vcol may be the same matrix
+// (v's column lies outside c0..c1-1).  Waves own disjoint columns: no barrier
+// inside.
+//  - wide (>= 48 columns): each wave takes blocks of 16 consecutive columns
+//    with its four 16-lane rows on interleaved matrix rows, so a load reads
+//    four contiguous 128-byte row segments; four rows per lane are loaded
+//    before any is used or stored (the stores may alias later loads, which
+//    would otherwise serialise every row on its load latency); the four
+//    partial dot products are combined by two lane exchanges, in the same
+//    order in every lane;
+//  - narrow: the 64 lanes of a wave walk the rows, four columns per wave at
+//    a time, reduced over the wave.
+__device__ __forceinline__ void hh_apply(double* X, int ldx, int m, int j, const double* vcol, int ldv, int c0,
+                                         int c1, double t) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (c1 - c0 >= 48) {
+    const int cl = lane & 15, g = lane >> 4;
+    const int nblk = (c1 - c0 + 15) >> 4;
+    for (int blk = w; blk < nblk; blk += NWAVE) {
+      const int col = c0 + (blk << 4) + cl;
+      const bool on = col < c1;
+      const int cc = on ? col : c0;  // in-bounds column for lanes past the end
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int i = j + g;
+      for (; i + 12 < m; i += 16) {
+        const double v0 = (i == j) ? 1.0 : vcol[(long)i * ldv], v1 = vcol[(long)(i + 4) * ldv];
+        const double v2 = vcol[(long)(i + 8) * ldv], v3 = vcol[(long)(i + 12) * ldv];
+        const double x0 = X[(long)i * ldx + cc], x1 = X[(long)(i + 4) * ldx + cc];
+        const double x2 = X[(long)(i + 8) * ldx + cc], x3 = X[(long)(i + 12) * ldx + cc];
+        s0 += v0 * x0;
+        s1 += v1 * x1;
+        s2 += v2 * x2;
+        s3 += v3 * x3;
+      }
+      for (; i < m; i += 4) s0 += ((i == j) ? 1.0 : vcol[(long)i * ldv]) * X[(long)i * ldx + cc];
+      double sm = (s0 + s1) + (s2 + s3);
+      sm += __shfl_xor(sm, 16, 64);
+      sm += __shfl_xor(sm, 32, 64);
+      sm *= t;
+      if (on) {
+        i = j + g;
+        for (; i + 12 < m; i += 16) {
+          const double v0 = (i == j) ? 1.0 : vcol[(long)i * ldv], v1 = vcol[(long)(i + 4) * ldv];
+          const double v2 = vcol[(long)(i + 8) * ldv], v3 = vcol[(long)(i + 12) * ldv];
+          const double x0 = X[(long)i * ldx + col], x1 = X[(long)(i + 4) * ldx + col];
+          const double x2 = X[(long)(i + 8) * ldx + col], x3 = X[(long)(i + 12) * ldx + col];
+          X[(long)i * ldx + col] = x0 - v0 * sm;
+          X[(long)(i + 4) * ldx + col] = x1 - v1 * sm;
+          X[(long)(i + 8) * ldx + col] = x2 - v2 * sm;
+          X[(long)(i + 12) * ldx + col] = x3 - v3 * sm;
+        }
+        for (; i < m; i += 4) X[(long)i * ldx + col] -= ((i == j) ? 1.0 : vcol[(long)i * ldv]) * sm;
+      }
+    }
+    return;
+  }
+  for (int q0 = c0 + w; q0 < c1; q0 += 4 * NWAVE) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = j + lane; i < m; i += 64) {
+      const double v = (i == j) ? 1.0 : vcol[(long)i * ldv];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = q0 + q * NWAVE;
+        if (col < c1) s[q] += v * X[(long)i * ldx + col];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
+    for (int i = j + lane; i < m; i += 64) {
+      const double v = (i == j) ? 1.0 : vcol[(long)i * ldv];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = q0 + q * NWAVE;
+        if (col < c1) X[(long)i * ldx + col] -= v * s[q];
+      }
+    }
+  }
+}
+
 __device__ void wg_householder(double* W, int m, int c, double* tau, double* red) {
   const int k = m < c ? m : c;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = 0; j < k; ++j) {
     double s2 = 0.0;
     for (int i = j + 1 + threadIdx.x; i < m; i += NT) {
@@ -956,31 +1036,7 @@ __device__ void wg_householder(double* W, int m, int c, double* tau, double* red
     }
     if (threadIdx.x == 0) tau[j] = t;
     __syncthreads();
-    if (t != 0.0) {
-      // apply H_j to columns j+1..c-1: wave w takes columns j+1+w, +NWAVE, ...,
-      // four at a time so that their reductions overlap
-      for (int c0 = j + 1 + w; c0 < c; c0 += 4 * NWAVE) {
-        double s[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = j + lane; i < m; i += 64) {
-          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int col = c0 + q * NWAVE;
-            if (col < c) s[q] += v * W[(long)i * c + col];
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
-        for (int i = j + lane; i < m; i += 64) {
-          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int col = c0 + q * NWAVE;
-            if (col < c) W[(long)i * c + col] -= v * s[q];
-          }
-        }
-      }
-    }
+    if (t != 0.0) hh_apply(W, c, m, j, W + j, c, j + 1, c, t);
     __syncthreads();
   }
 }
@@ -1180,37 +1236,13 @@ __device__ void wg_qr_small(double* W, int m, int c, double* tau, double* Qm, do
 // Explicit Q (m x k, row-major, ld = k) from the reflectors in W.
 __device__ void wg_form_q(const double* W, int m, int c, const double* tau, double* Qm) {
   const int k = m < c ? m : c;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (long e = threadIdx.x; e < (long)m * k; e += NT) {
-    const int i = (int)(e / k), j = (int)(e % k);
-    Qm[e] = (i == j) ? 1.0 : 0.0;
-  }
+  for (long e = threadIdx.x; e < (long)m * k; e += NT) Qm[e] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += NT) Qm[(long)i * k + i] = 1.0;
   __syncthreads();
   for (int j = k - 1; j >= 0; --j) {
     const double t = tau[j];
-    if (t != 0.0) {
-      for (int c0 = j + w; c0 < k; c0 += 4 * NWAVE) {
-        double s[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = j + lane; i < m; i += 64) {
-          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int col = c0 + q * NWAVE;
-            if (col < k) s[q] += v * Qm[(long)i * k + col];
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
-        for (int i = j + lane; i < m; i += 64) {
-          const double v = (i == j) ? 1.0 : W[(long)i * c + j];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int col = c0 + q * NWAVE;
-            if (col < k) Qm[(long)i * k + col] -= v * s[q];
-          }
-        }
-      }
-    }
+    if (t != 0.0) hh_apply(Qm, k, m, j, W + j, c, j, k, t);
     __syncthreads();
   }
 }
