@@ -1,7 +1,7 @@
 """Benchmark: TT-ALS solves/s on one MI355X.
 
 Workload (one "step" = one pass of the hot path over one batch): a batch of
-B independent transverse-field Ising chains, d = 32 sites, n = 2, MPO rank
+B (default 2048) independent transverse-field Ising chains, d = 32 sites, n = 2, MPO rank
 3, couplings J_i, g_i ~ U[0.5, 1.5] (seed = instance index), random N(0,1)
 initial TT of rank 16 right-orthonormalised and normalised, evp.als lowest
 eigenpair ('eig', sigma = -100), 6 sweeps, solved in ONE engine launch.
@@ -134,7 +134,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=512)
+    ap.add_argument('--batch', type=int, default=2048)
     ap.add_argument('--sites', type=int, default=32)
     ap.add_argument('--rank', type=int, default=16)
     ap.add_argument('--sweeps', type=int, default=6)
@@ -148,6 +148,8 @@ def main():
                     help='run one plain device copy of this many bytes before the timed steps (a known-size '
                          'reference for hardware byte counters)')
     args = ap.parse_args()
+    if args.plain < 1:
+        ap.error('--plain must be at least 1 (the plain drop-in rate is part of the result)')
     if args.gpus != 1:
         print('bench.py measures one GPU only: every box here has one MI355X. Across GPUs the '
               'independent systems of a batch would shard one slice per GPU with no communication.',
