@@ -120,11 +120,6 @@ __device__ __forceinline__ double wave_max(double v) {
   v = fmax(v, dpp_mov<0x140>(v));
   return fmax(fmax(rdlane(v, 0), rdlane(v, 16)), fmax(rdlane(v, 32), rdlane(v, 48)));
 }
-__device__ __forceinline__ float wave_sumf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // Sum over the workgroup; every thread gets the result.  `red` is LDS
 // scratch of at least NWAVE doubles.
@@ -185,13 +180,6 @@ __device__ __forceinline__ void wg_sum3_nb(double& a, double& b, double& c, doub
 }
 
 template <class T>
-__device__ double wg_dot(const T* x, const T* y, int n, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) s += (double)x[i] * (double)y[i];
-  return wg_sum(s, red);
-}
-
-template <class T>
 __device__ double wg_nrm2(const T* x, int n, double* red) {
   // scaled to avoid overflow/underflow of the squares
   double mx = 0.0;
@@ -214,11 +202,6 @@ __device__ double wg_nrm2(const T* x, int n, double* red) {
   return mx * sqrt(wg_sum(s, red));
 }
 
-// y := a*x + b*y  (elementwise, no barrier)
-template <class T>
-__device__ __forceinline__ void wg_axpby(double a, const T* x, double b, T* y, int n) {
-  for (int i = threadIdx.x; i < n; i += NT) y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
-}
 template <class T, class U>
 __device__ __forceinline__ void wg_copy(const T* x, U* y, int n) {
   for (int i = threadIdx.x; i < n; i += NT) y[i] = (U)x[i];
