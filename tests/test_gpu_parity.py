@@ -217,8 +217,11 @@ def test_local_eig_nearest_interior_small(env):
 
 
 @pytest.mark.parametrize('m,c', [(12, 5), (5, 12), (200, 48), (7, 7), (1, 9), (9, 1), (32, 16), (64, 64),
-                                 (64, 40), (30, 64), (65, 16)])
+                                 (64, 40), (30, 64), (65, 16), (200, 80), (130, 130), (100, 300), (520, 70)])
 def test_qr(env, m, c):
+    """Small panels (m, c <= 64) take the DPP-row QR; larger ones the
+    Householder sweep, whose trailing updates run column-blocked when 48 or
+    more columns remain (the last four shapes) and row-parallel below."""
     lib, ctx, _ = env
     Y = np.random.default_rng(m * c).standard_normal((m, c))
     k = min(m, c)
