@@ -927,8 +927,8 @@ __device__ void micro_dense(int a, int P, int n, int Q, int b, const T* L, const
 //    partial dot products are combined by two lane exchanges, in the same
 //    order in every lane;
 //  - narrow: the 64 lanes of a wave walk the rows, four columns per wave at
-//    a time, reduced over the wave; the update loads two rows per lane
-//    before storing either.
+//    a time, reduced over the wave; both passes load two rows per lane
+//    before using either.
 __device__ __forceinline__ void hh_apply(double* X, int ldx, int m, int j, const double* vcol, int ldv, int c0,
                                          int c1, double t) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -975,7 +975,23 @@ __device__ __forceinline__ void hh_apply(double* X, int ldx, int m, int j, const
   }
   for (int q0 = c0 + w; q0 < c1; q0 += 4 * NWAVE) {
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = j + lane; i < m; i += 64) {
+    int i = j + lane;
+    for (; i + 64 < m; i += 128) {
+      const double v0 = (i == j) ? 1.0 : vcol[(long)i * ldv], v1 = vcol[(long)(i + 64) * ldv];
+      double x0[4], x1[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = q0 + q * NWAVE;
+        x0[q] = col < c1 ? X[(long)i * ldx + col] : 0.0;
+        x1[q] = col < c1 ? X[(long)(i + 64) * ldx + col] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        s[q] += v0 * x0[q];  // row order kept: the same sums as one row at a time
+        s[q] += v1 * x1[q];
+      }
+    }
+    if (i < m) {
       const double v = (i == j) ? 1.0 : vcol[(long)i * ldv];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -985,7 +1001,7 @@ __device__ __forceinline__ void hh_apply(double* X, int ldx, int m, int j, const
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) s[q] = wave_sum(s[q]) * t;
-    int i = j + lane;
+    i = j + lane;
     for (; i + 64 < m; i += 128) {
       const double v0 = (i == j) ? 1.0 : vcol[(long)i * ldv], v1 = vcol[(long)(i + 64) * ldv];
       double x0[4], x1[4];
