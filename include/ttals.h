@@ -40,6 +40,8 @@ extern "C" {
 #define TTALS_S_NOCONV 2      /* some local Krylov solve hit its iteration cap */
 #define TTALS_S_UNSUPPORTED 3 /* evp: sigma inside the spectrum of a large local problem */
 #define TTALS_S_NONFINITE 4   /* NaN/Inf in a local solution */
+#define TTALS_S_NONSYM 5      /* evp: operator not symmetric (NotImplementedError; the reference's
+                                 'eig' would use a general eigensolver, scikit_tt/solvers/evp.py:424-432) */
 
 typedef struct ttals_ctx ttals_ctx;
 
@@ -92,6 +94,22 @@ int ttals_set_trace(ttals_ctx* ctx, double* buf, long long cap);
 int ttals_sle_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
                         const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
                         ttals_result* res);
+
+/* Replaces scikit_tt.solvers.sle.als (scikit_tt/solvers/sle.py:10-95) for
+ * large systems: each system is solved by a GANG of `gang` workgroups (<= 0:
+ * automatic, one per row of the widest bond up to 32) that split every
+ * phase -- the local matvec over the row-side rank index, the interface
+ * stacks over their first index, the Krylov vector operations over the
+ * same rows -- and meet at gang barriers; all gangs are co-resident (one
+ * cooperative launch).  precision 0: FP64 throughout; 1: every Krylov
+ * iteration in FP32 on an FP32 copy of the local operator, refined with the
+ * FP64 local residual f - M v until it meets `tol` (cores, stacks, QR and
+ * the residual FP64).  Local problems are solved by CG (operator symmetric
+ * slice by slice; GMRES when p'Mp <= 0) or GMRES(40), never by a dense LU.
+ * Synchronous; TTALS_E_ARG when the gangs do not fit the device at once. */
+int ttals_sle_als_gang(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                       const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
+                       int precision, int gang, ttals_result* res);
 
 /* Replaces scikit_tt.solvers.evp.als (scikit_tt/solvers/evp.py:17-179) with
  * number_ev=1, operator_gevp=None, previous=[] for `count` systems in ONE
@@ -200,6 +218,17 @@ int ttals_core_embed(void* stream, int r0, int mn, int r1, const double* src, do
 /* scikit_tt/tensor_train.py:1334-1430 (p=2): Euclidean/Frobenius norm of a
  * TT (operator cores are vectorised).  Synchronous; result on the host. */
 int ttals_tt_norm(ttals_ctx* ctx, void* stream, const ttals_tt* t, double* out);
+
+/* Residual norms of `count` systems in ONE launch (one workgroup per
+ * system): res_norm[s] = ||A_s x_s - b_s|| and rhs_norm[s] = ||b_s|| (host
+ * arrays), the quantities of the reference's rel.res check
+ * (A.dot(x) - b).norm() / b.norm() (tests/test_sle.py:52-53; TT.dot
+ * tensor_train.py:422-524, TT.__sub__ 346-366, TT.norm 1334-1430) and of
+ * residual_error (tensor_train.py:2035-2074).  A x - b is never stored: its
+ * block cores are generated into a right-to-left Householder QR sweep.
+ * Synchronous. */
+int ttals_residual_norm_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x,
+                              const ttals_tt* b, double* res_norm, double* rhs_norm);
 
 /* scikit_tt/tensor_train.py:1092-1205 (dir 0, ortho_left over cores
  * start..end) and 1207-1310 (dir 1, ortho_right over cores start..end

@@ -258,24 +258,6 @@ def residual_error(op, x, b):
     return err
 
 
-def _rank_list(order, ranks):
-    if not isinstance(ranks, list):
-        ranks = [1] + [ranks] * (order - 1) + [1]
-    return ranks
-
-
-def zeros(row_dims, col_dims, ranks=1):
-    """tensor_train.py:1806-1835"""
-    r = _rank_list(len(row_dims), ranks)
-    return [np.zeros((r[i], row_dims[i], col_dims[i], r[i + 1])) for i in range(len(row_dims))]
-
-
-def ones(row_dims, col_dims, ranks=1):
-    """tensor_train.py:1838-1867"""
-    r = _rank_list(len(row_dims), ranks)
-    return [np.ones((r[i], row_dims[i], col_dims[i], r[i + 1])) for i in range(len(row_dims))]
-
-
 def eye(dims):
     """tensor_train.py:1870-1893"""
     return [np.eye(n).reshape(1, n, n, 1) for n in dims]

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libttals.so')
 
 E_ARG, E_SHAPE, E_HIP, E_OOM, E_UNSUPPORTED = -1, -2, -3, -4, -5
-S_SINGULAR, S_NOCONV, S_UNSUPPORTED, S_NONFINITE = 1, 2, 3, 4
+S_SINGULAR, S_NOCONV, S_UNSUPPORTED, S_NONFINITE, S_NONSYM = 1, 2, 3, 4, 5
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -52,6 +52,7 @@ SIGNATURES = {
     'ttals_last_error': (ctypes.c_char_p, [_vp]),
     'ttals_sle_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _Rp]),
     'ttals_evp_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i, _Rp]),
+    'ttals_sle_als_gang': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _i, _i, _Rp]),
     'ttals_sle_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i]),
     'ttals_evp_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i]),
     'ttals_query': (_i, [_vp]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     'ttals_tt_norm': (_i, [_vp, _vp, _TTp, c_double_p]),
     'ttals_tt_ortho': (_i, [_vp, _vp, _TTp, _i, _i, _i, _d, c_int_p]),
     'ttals_set_trace': (_i, [_vp, _vp, ctypes.c_longlong]),
+    'ttals_residual_norm_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, c_double_p, c_double_p]),
 }
 
 _lock = threading.Lock()

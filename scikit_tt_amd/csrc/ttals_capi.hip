@@ -9,6 +9,8 @@
 #include <vector>
 
 #include "ttals_engine.hpp"
+#include "ttals_resid.hpp"
+#include "ttals_gang.hpp"
 #include "../../include/ttals.h"
 
 using namespace ttals;
@@ -519,7 +521,7 @@ struct SysPlan {
   // metadata (host staging offsets)
   size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_x0c, m_Loff, m_Roff, m_Lboff, m_Rboff, m_desc;
   // device workspace offsets (doubles region)
-  size_t w_L, w_R, w_Lb, w_Rb, w_f, w_g, w_t1, w_t2, w_V, w_dense, w_qw, w_qq, w_tau, w_rf, w_best, w_out;
+  size_t w_L, w_R, w_Lb, w_Rb, w_f, w_g, w_t1, w_t2, w_V, w_dense, w_qw, w_qq, w_tau, w_rf, w_best, w_out, w_sym;
   std::vector<long> Loff, Roff, Lboff, Rboff;
 };
 
@@ -602,168 +604,321 @@ int plan_system(ttals_ctx* c, int kind, const ttals_tt& A, const ttals_tt& x0, c
 template <class T>
 T* hptr(ttals_ctx* c, size_t off) { return reinterpret_cast<T*>(c->hst + off); }
 
-// Builds descriptors for `count` systems, runs k_als (one workgroup per
-// system), copies ranks and scalars back and synchronises the stream.
-int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals_tt* A, const ttals_tt* x0,
-                 const ttals_tt* b, const ttals_tt* xo, int repeats, int solver, double tol, int maxit, double sigma,
-                 double conv_eps) {
-  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is still pending on this context (call ttals_wait)");
-  if (count < 1 || !A || !x0 || !xo) return fail(c, TTALS_E_ARG, "bad batch arguments");
-  if (kind == 0 && !b) return fail(c, TTALS_E_ARG, "missing right-hand side");
-  if (repeats < 0) return fail(c, TTALS_E_ARG, "repeats must be >= 0");
-  if (!(tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
-  if (maxit < 1) return fail(c, TTALS_E_ARG, "maxit must be >= 1");
+// Sizes of one engine system, read straight from its descriptors (no
+// per-system heap storage: the batch planner makes two passes instead).
+struct SysSizes {
+  int d = 0, ramax = 1;
+  long nmax = 1, tmax = 1, rmax = 1;
+  size_t nL = 0, nR = 0, nLb = 0, nRb = 0, xsz = 0;
+};
+
+// Host-metadata and device-workspace offsets of one system.
+struct SysOff {
+  size_t m_n, m_rx, m_ra, m_rb, m_brx, m_xc, m_ac, m_bc, m_bestc, m_x0c, m_Loff, m_Roff, m_Lboff, m_Rboff;
+  size_t w_L, w_R, w_Lb, w_Rb, w_f, w_g, w_t1, w_t2, w_V, w_dense, w_qw, w_qq, w_tau, w_rf, w_best, w_out, w_sym;
+};
+
+// Checks system s (shapes as plan_system, plus the output descriptor's
+// dims and rank capacity) and computes its sizes.
+int size_system(ttals_ctx* c, int kind, const ttals_tt& A, const ttals_tt& x0, const ttals_tt* b,
+                const ttals_tt& xo, int s, SysSizes& z) {
+  int rc;
+  if ((rc = check_tt(c, A, false, "operator")) || (rc = check_tt(c, x0, true, "initial_guess")) ||
+      (b && (rc = check_tt(c, *b, true, "right_hand_side"))))
+    return fail(c, rc, "system " + std::to_string(s) + ": " + c->err);
+  const int d = A.order;
+  if (x0.order != d || (b && b->order != d)) return fail(c, TTALS_E_SHAPE, "orders of the TTs differ");
+  if (x0.ranks[0] != 1 || x0.ranks[d] != 1 || A.ranks[0] != 1 || A.ranks[d] != 1 ||
+      (b && (b->ranks[0] != 1 || b->ranks[d] != 1)))
+    return fail(c, TTALS_E_SHAPE, "outer TT ranks must be 1");
+  if (check_tt(c, xo, true, "x_out") || xo.order != d)
+    return fail(c, TTALS_E_ARG, "system " + std::to_string(s) + ": bad output descriptor");
+  z = SysSizes();
+  z.d = d;
+  for (int i = 0; i < d; ++i) {
+    if (A.row_dims[i] != A.col_dims[i])
+      return fail(c, TTALS_E_SHAPE, "operator must be square core by core (row_dims == col_dims)");
+    if (x0.row_dims[i] != A.col_dims[i]) return fail(c, TTALS_E_SHAPE, "initial guess dims != operator col dims");
+    if (b && b->row_dims[i] != A.row_dims[i]) return fail(c, TTALS_E_SHAPE, "rhs dims != operator row dims");
+    // the engine writes x0-sized cores into the output buffers
+    if (xo.row_dims[i] != A.row_dims[i] || xo.ranks[i] < x0.ranks[i] || xo.ranks[i + 1] < x0.ranks[i + 1])
+      return fail(c, TTALS_E_SHAPE, "system " + std::to_string(s) + ": output core " + std::to_string(i) +
+                                        " has other dims or less rank capacity than the initial guess");
+    const long a = x0.ranks[i], bb = x0.ranks[i + 1], n = A.row_dims[i];
+    const long p0 = b ? b->ranks[i] : 1, p1 = b ? b->ranks[i + 1] : 1;
+    const long N = a * n * bb;
+    z.nmax = std::max(z.nmax, N);
+    const long mx = std::max(std::max((long)A.ranks[i], (long)A.ranks[i + 1]), std::max(p0, p1));
+    z.tmax = std::max(z.tmax, N * mx);
+    z.rmax = std::max(z.rmax, std::max(a, bb));
+    // stacks sized by the initial ranks (ranks never grow): L_i on bond i
+    // (r_i, R_i, r_i), Lb_i (p_i, r_i); R_i on bond i+1, Rb_i (p_{i+1}, r_{i+1})
+    z.nL += (size_t)(a * A.ranks[i] * a);
+    z.nR += (size_t)(bb * A.ranks[i + 1] * bb);
+    z.nLb += (size_t)(p0 * a);
+    z.nRb += (size_t)(p1 * bb);
+    z.xsz += (size_t)N;
+  }
+  for (int i = 0; i <= d; ++i) z.ramax = std::max(z.ramax, A.ranks[i]);
+  return 0;
+}
+
+void take_offsets(int kind, const SysSizes& z, Layout& hl, Layout& dl, SysOff& o) {
+  const int d = z.d;
+  o.m_n = hl.take(sizeof(int) * d);
+  o.m_rx = hl.take(sizeof(int) * (d + 1));
+  o.m_ra = hl.take(sizeof(int) * (d + 1));
+  o.m_rb = hl.take(sizeof(int) * (d + 1));
+  o.m_brx = hl.take(sizeof(int) * (d + 1));
+  o.m_xc = hl.take(sizeof(void*) * d);
+  o.m_ac = hl.take(sizeof(void*) * d);
+  o.m_bc = hl.take(sizeof(void*) * d);
+  o.m_bestc = hl.take(sizeof(void*) * d);
+  o.m_x0c = hl.take(sizeof(void*) * d);
+  o.m_Loff = hl.take(sizeof(long) * d);
+  o.m_Roff = hl.take(sizeof(long) * d);
+  o.m_Lboff = hl.take(sizeof(long) * d);
+  o.m_Rboff = hl.take(sizeof(long) * d);
+  o.w_L = dl.take(8 * std::max<size_t>(z.nL, 1));
+  o.w_R = dl.take(8 * std::max<size_t>(z.nR, 1));
+  o.w_Lb = dl.take(8 * std::max<size_t>(z.nLb, 1));
+  o.w_Rb = dl.take(8 * std::max<size_t>(z.nRb, 1));
+  o.w_f = dl.take(8 * z.nmax);
+  o.w_g = dl.take(8 * z.nmax);
+  o.w_t1 = dl.take(8 * z.tmax);
+  o.w_t2 = dl.take(8 * z.tmax);
+  o.w_V = dl.take(8 * (size_t)(LKMAX + 1) * z.nmax);
+  o.w_dense = dl.take(8 * (size_t)(NDENSE * NDENSE + NDENSE));
+  o.w_qw = dl.take(8 * z.nmax);
+  o.w_qq = dl.take(8 * z.nmax);
+  o.w_tau = dl.take(8 * z.rmax);
+  o.w_rf = dl.take(8 * z.rmax * z.rmax);
+  o.w_best = dl.take(kind == 1 ? 8 * z.xsz : 8);
+  o.w_out = dl.take(8 * 16);
+  o.w_sym = dl.take(kind == 1 ? 8 * 4 * (size_t)z.ramax * z.ramax : 8);
+}
+
+// Where a batch's metadata, workspace and results live.
+struct Placement {
+  char* host;       // staging: metadata, then the compact results
+  char* dev;        // workspace, then the metadata mirror
+  size_t host_cap;  // 0: the context's grow-only buffers (allocated here)
+  size_t dev_cap;
+};
+
+struct BatchArgs {
+  int kind, count, repeats, solver, maxit;
+  double tol, sigma, conv_eps;
+  const ttals_tt *A, *x0, *b, *xo;
+};
+
+// Plans and enqueues one k_als launch for a batch (descriptors, one metadata
+// copy, the launch, the result copy).  On return *r_res is the staging offset
+// of the compact results (res_stride doubles per system).
+int enqueue_batch(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, Placement& pl, size_t* r_res,
+                  size_t* res_stride_out, bool timed) {
+  const int kind = B.kind, count = B.count;
+  if (count < 1 || !B.A || !B.x0 || !B.xo) return fail(c, TTALS_E_ARG, "bad batch arguments");
+  if (kind == 0 && !B.b) return fail(c, TTALS_E_ARG, "missing right-hand side");
+  if (B.repeats < 0) return fail(c, TTALS_E_ARG, "repeats must be >= 0");
+  if (!(B.tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
+  if (B.maxit < 1) return fail(c, TTALS_E_ARG, "maxit must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
-  std::vector<SysPlan> plans(count);
+  // pass 1: checks and totals
   Layout hl, dl;  // host metadata layout (mirrored on device), device workspace layout
   const size_t m_descs = hl.take(sizeof(SysDesc) * count);
+  int dmax = 0;
+  SysSizes z;
+  SysOff o;
   for (int s = 0; s < count; ++s) {
-    SysPlan& P = plans[s];
-    int rc = plan_system(c, kind, A[s], x0[s], kind == 0 ? &b[s] : nullptr, P);
-    if (rc) return fail(c, rc, "system " + std::to_string(s) + ": " + c->err);
-    if (check_tt(c, xo[s], true, "x_out") || xo[s].order != P.d)
-      return fail(c, TTALS_E_ARG, "system " + std::to_string(s) + ": bad output descriptor");
-    const int d = P.d;
-    P.m_n = hl.take(sizeof(int) * d);
-    P.m_rx = hl.take(sizeof(int) * (d + 1));
-    P.m_ra = hl.take(sizeof(int) * (d + 1));
-    P.m_rb = hl.take(sizeof(int) * (d + 1));
-    P.m_brx = hl.take(sizeof(int) * (d + 1));
-    P.m_xc = hl.take(sizeof(void*) * d);
-    P.m_ac = hl.take(sizeof(void*) * d);
-    P.m_bc = hl.take(sizeof(void*) * d);
-    P.m_bestc = hl.take(sizeof(void*) * d);
-    P.m_x0c = hl.take(sizeof(void*) * d);
-    P.m_Loff = hl.take(sizeof(long) * d);
-    P.m_Roff = hl.take(sizeof(long) * d);
-    P.m_Lboff = hl.take(sizeof(long) * d);
-    P.m_Rboff = hl.take(sizeof(long) * d);
-    P.w_L = dl.take(8 * std::max<size_t>(P.nL, 1));
-    P.w_R = dl.take(8 * std::max<size_t>(P.nR, 1));
-    P.w_Lb = dl.take(8 * std::max<size_t>(P.nLb, 1));
-    P.w_Rb = dl.take(8 * std::max<size_t>(P.nRb, 1));
-    P.w_f = dl.take(8 * P.nmax);
-    P.w_g = dl.take(8 * P.nmax);
-    P.w_t1 = dl.take(8 * P.tmax);
-    P.w_t2 = dl.take(8 * P.tmax);
-    P.w_V = dl.take(8 * (size_t)(LKMAX + 1) * P.nmax);
-    P.w_dense = dl.take(8 * (size_t)(NDENSE * NDENSE + NDENSE));
-    P.w_qw = dl.take(8 * P.nmax);
-    P.w_qq = dl.take(8 * P.nmax);
-    P.w_tau = dl.take(8 * P.rmax);
-    P.w_rf = dl.take(8 * P.rmax * P.rmax);
-    P.w_best = dl.take(kind == 1 ? 8 * P.xsz_total : 8);
-    P.w_out = dl.take(8 * 16);
+    int rc = size_system(c, kind, B.A[s], B.x0[s], kind == 0 ? &B.b[s] : nullptr, B.xo[s], s, z);
+    if (rc) return rc;
+    take_offsets(kind, z, hl, dl, o);
+    dmax = std::max(dmax, z.d);
   }
   const size_t meta_bytes = hl.bytes;
   // compact results: one row of 16 scalars + (dmax+1) ranks per system, on
   // the device after the workspace and in the staging buffer after the
   // metadata
-  int dmax = 0;
-  for (int s = 0; s < count; ++s) dmax = std::max(dmax, plans[s].d);
   const size_t res_stride = 16 + (size_t)dmax + 1;
   const size_t w_res = dl.take(8 * res_stride * count);
   Layout rl;
   rl.bytes = meta_bytes;
-  const size_t r_res = rl.take(8 * res_stride * count);
-  int rc;
-  if ((rc = ensure_hst(c, rl.bytes))) return rc;
-  if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
-  char* dbase = reinterpret_cast<char*>(c->dws);
+  *r_res = rl.take(8 * res_stride * count);
+  *res_stride_out = res_stride;
+  if (pl.host_cap == 0) {
+    int rc;
+    if ((rc = ensure_hst(c, rl.bytes))) return rc;
+    if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
+    pl.host = c->hst;
+    pl.dev = reinterpret_cast<char*>(c->dws);
+  } else if (rl.bytes > pl.host_cap || dl.bytes + meta_bytes > pl.dev_cap) {
+    return fail(c, TTALS_E_ARG, "batch needs " + std::to_string(dl.bytes + meta_bytes) + " bytes of device scratch and " +
+                                    std::to_string(rl.bytes) + " of host staging; the fixed buffers hold " +
+                                    std::to_string(pl.dev_cap) + " and " + std::to_string(pl.host_cap));
+  }
+  char* hbase = pl.host;
+  char* dbase = pl.dev;
   char* dmeta = dbase + dl.bytes;  // metadata mirrored after the workspace
   auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
   auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
-  SysDesc* hdesc = hptr<SysDesc>(c, m_descs);
+  auto hp = [&](size_t off) { return reinterpret_cast<void*>(hbase + off); };
+  SysDesc* hdesc = reinterpret_cast<SysDesc*>(hbase + m_descs);
+  // pass 2: the same offsets again, descriptors written
+  Layout hl2, dl2;
+  (void)hl2.take(sizeof(SysDesc) * count);
   for (int s = 0; s < count; ++s) {
-    SysPlan& P = plans[s];
-    const int d = P.d;
-    std::memcpy(hptr<int>(c, P.m_n), P.n.data(), sizeof(int) * d);
-    std::memcpy(hptr<int>(c, P.m_rx), P.rx.data(), sizeof(int) * (d + 1));
-    std::memcpy(hptr<int>(c, P.m_ra), P.ra.data(), sizeof(int) * (d + 1));
-    std::memcpy(hptr<int>(c, P.m_rb), P.rb.data(), sizeof(int) * (d + 1));
-    std::memcpy(hptr<int>(c, P.m_brx), P.rx.data(), sizeof(int) * (d + 1));
-    double** xcp = hptr<double*>(c, P.m_xc);
-    const double** acp = hptr<const double*>(c, P.m_ac);
-    const double** bcp = hptr<const double*>(c, P.m_bc);
-    double** bestp = hptr<double*>(c, P.m_bestc);
-    const double** x0p = hptr<const double*>(c, P.m_x0c);
-    size_t boff = 0;
-    for (int i = 0; i < d; ++i) {
-      x0p[i] = x0[s].cores[i];
-      xcp[i] = xo[s].cores[i];
-      acp[i] = A[s].cores[i];
-      bcp[i] = kind == 0 ? b[s].cores[i] : nullptr;
-      bestp[i] = wptr(P.w_best) + (kind == 1 ? boff : 0);
-      boff += (size_t)P.rx[i] * P.n[i] * P.rx[i + 1];
+    const ttals_tt& A = B.A[s];
+    const ttals_tt& x0 = B.x0[s];
+    const ttals_tt* b = kind == 0 ? &B.b[s] : nullptr;
+    (void)size_system(c, kind, A, x0, b, B.xo[s], s, z);
+    take_offsets(kind, z, hl2, dl2, o);
+    const int d = z.d;
+    int* hn = (int*)hp(o.m_n);
+    int* hrx = (int*)hp(o.m_rx);
+    int* hra = (int*)hp(o.m_ra);
+    int* hrb = (int*)hp(o.m_rb);
+    int* hbrx = (int*)hp(o.m_brx);
+    for (int i = 0; i < d; ++i) hn[i] = A.row_dims[i];
+    for (int i = 0; i <= d; ++i) {
+      hrx[i] = hbrx[i] = x0.ranks[i];
+      hra[i] = A.ranks[i];
+      hrb[i] = b ? b->ranks[i] : 1;
     }
-    std::memcpy(hptr<long>(c, P.m_Loff), P.Loff.data(), sizeof(long) * d);
-    std::memcpy(hptr<long>(c, P.m_Roff), P.Roff.data(), sizeof(long) * d);
-    std::memcpy(hptr<long>(c, P.m_Lboff), P.Lboff.data(), sizeof(long) * d);
-    std::memcpy(hptr<long>(c, P.m_Rboff), P.Rboff.data(), sizeof(long) * d);
+    double** xcp = (double**)hp(o.m_xc);
+    const double** acp = (const double**)hp(o.m_ac);
+    const double** bcp = (const double**)hp(o.m_bc);
+    double** bestp = (double**)hp(o.m_bestc);
+    const double** x0p = (const double**)hp(o.m_x0c);
+    long* Lo = (long*)hp(o.m_Loff);
+    long* Ro = (long*)hp(o.m_Roff);
+    long* Lbo = (long*)hp(o.m_Lboff);
+    long* Rbo = (long*)hp(o.m_Rboff);
+    size_t boff = 0;
+    long lo = 0, ro = 0, lbo = 0, rbo = 0;
+    for (int i = 0; i < d; ++i) {
+      const long a = x0.ranks[i], bb = x0.ranks[i + 1], n = A.row_dims[i];
+      x0p[i] = x0.cores[i];
+      xcp[i] = B.xo[s].cores[i];
+      acp[i] = A.cores[i];
+      bcp[i] = b ? b->cores[i] : nullptr;
+      bestp[i] = wptr(o.w_best) + (kind == 1 ? boff : 0);
+      boff += (size_t)(a * n * bb);
+      Lo[i] = lo;
+      lo += a * A.ranks[i] * a;
+      Ro[i] = ro;
+      ro += bb * A.ranks[i + 1] * bb;
+      Lbo[i] = lbo;
+      lbo += (long)hrb[i] * a;
+      Rbo[i] = rbo;
+      rbo += (long)hrb[i + 1] * bb;
+    }
     SysDesc& D = hdesc[s];
     std::memset(&D, 0, sizeof(D));
     D.d = d;
     D.kind = kind;
-    D.solver = solver;
-    D.repeats = repeats;
+    D.solver = B.solver;
+    D.repeats = B.repeats;
     D.ortho_x0 = (kind == 0) ? 1 : 0;
     D.sym = -1;
     D.gm_restart = KMAX;
     D.adapt = (kind == 1) ? c->tol_policy : 0;
     D.mvar = c->matvec_variant;
-    D.maxit = maxit;
-    D.tol = tol;
-    D.sigma = sigma;
-    D.conv_eps = conv_eps;
-    D.n = (const int*)dptr(P.m_n);
-    D.rx = (int*)dptr(P.m_rx);
-    D.ra = (const int*)dptr(P.m_ra);
-    D.rb = (const int*)dptr(P.m_rb);
-    D.best_rx = (int*)dptr(P.m_brx);
-    D.xc = (double* const*)dptr(P.m_xc);
-    D.ac = (const double* const*)dptr(P.m_ac);
-    D.bc = (const double* const*)dptr(P.m_bc);
-    D.bestc = (double* const*)dptr(P.m_bestc);
-    D.Ls = wptr(P.w_L);
-    D.Rs = wptr(P.w_R);
-    D.Lbs = wptr(P.w_Lb);
-    D.Rbs = wptr(P.w_Rb);
-    D.Loff = (const long*)dptr(P.m_Loff);
-    D.Roff = (const long*)dptr(P.m_Roff);
-    D.Lboff = (const long*)dptr(P.m_Lboff);
-    D.Rboff = (const long*)dptr(P.m_Rboff);
-    D.f = wptr(P.w_f);
-    D.g = wptr(P.w_g);
-    D.t1 = wptr(P.w_t1);
-    D.t2 = wptr(P.w_t2);
-    D.V = wptr(P.w_V);
-    D.dense = wptr(P.w_dense);
-    D.qw = wptr(P.w_qw);
-    D.qq = wptr(P.w_qq);
-    D.tau = wptr(P.w_tau);
-    D.rf = wptr(P.w_rf);
-    D.out = wptr(P.w_out);
-    D.x0c = (const double* const*)dptr(P.m_x0c);
+    D.maxit = B.maxit;
+    D.tol = B.tol;
+    D.sigma = B.sigma;
+    D.conv_eps = B.conv_eps;
+    D.n = (const int*)dptr(o.m_n);
+    D.rx = (int*)dptr(o.m_rx);
+    D.ra = (const int*)dptr(o.m_ra);
+    D.rb = (const int*)dptr(o.m_rb);
+    D.best_rx = (int*)dptr(o.m_brx);
+    D.xc = (double* const*)dptr(o.m_xc);
+    D.ac = (const double* const*)dptr(o.m_ac);
+    D.bc = (const double* const*)dptr(o.m_bc);
+    D.bestc = (double* const*)dptr(o.m_bestc);
+    D.Ls = wptr(o.w_L);
+    D.Rs = wptr(o.w_R);
+    D.Lbs = wptr(o.w_Lb);
+    D.Rbs = wptr(o.w_Rb);
+    D.Loff = (const long*)dptr(o.m_Loff);
+    D.Roff = (const long*)dptr(o.m_Roff);
+    D.Lboff = (const long*)dptr(o.m_Lboff);
+    D.Rboff = (const long*)dptr(o.m_Rboff);
+    D.f = wptr(o.w_f);
+    D.g = wptr(o.w_g);
+    D.t1 = wptr(o.w_t1);
+    D.t2 = wptr(o.w_t2);
+    D.V = wptr(o.w_V);
+    D.dense = wptr(o.w_dense);
+    D.qw = wptr(o.w_qw);
+    D.qq = wptr(o.w_qq);
+    D.tau = wptr(o.w_tau);
+    D.rf = wptr(o.w_rf);
+    D.out = wptr(o.w_out);
+    D.x0c = (const double* const*)dptr(o.m_x0c);
     D.res = wptr(w_res) + res_stride * s;
     D.trace = (c->trace && s == 0) ? c->trace : nullptr;
     D.trace_cap = c->trace_cap;
+    D.symw = wptr(o.w_sym);
   }
-  HIPCHK(c, hipMemcpyAsync(dmeta, c->hst, meta_bytes, hipMemcpyHostToDevice, stream));
+  HIPCHK(c, hipMemcpyAsync(dmeta, hbase, meta_bytes, hipMemcpyHostToDevice, stream));
   const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
-  if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
-  if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
-  HIPCHK(c, hipEventRecord(c->k0, stream));
+  if (timed) {
+    if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
+    if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
+    HIPCHK(c, hipEventRecord(c->k0, stream));
+  }
   hipLaunchKernelGGL(k_als<double>, dim3(count), dim3(NT), (size_t)arena * 8, stream,
                      (const SysDesc*)dptr(m_descs), arena);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(c->k1, stream));
-  c->timed = true;
-  HIPCHK(c, hipMemcpyAsync(c->hst + r_res, wptr(w_res), 8 * res_stride * count, hipMemcpyDeviceToHost, stream));
+  if (timed) {
+    HIPCHK(c, hipEventRecord(c->k1, stream));
+    c->timed = true;
+  }
+  HIPCHK(c, hipMemcpyAsync(hbase + *r_res, wptr(w_res), 8 * res_stride * count, hipMemcpyDeviceToHost, stream));
+  return 0;
+}
+
+// Unpacks one system's compact result row; returns its status.
+int unpack_result(const double* o, int d, int* ranks, ttals_result& R) {
+  for (int i = 0; i <= d; ++i) ranks[i] = (int)o[16 + i];
+  R.lambda = o[0];
+  R.sweeps = (int)o[1];
+  R.status = (int)o[2];
+  R.krylov_iters = (long long)o[3];
+  R.unconverged = (int)o[4];
+  R.local_solves = (long long)o[5];
+  R.max_iters = (int)o[6];
+  R.worst_residual = o[7];
+  R.flops = o[8];
+  for (int k = 0; k < 7; ++k) R.cycles[k] = o[9 + k];
+  if (R.status == ST_OK && R.unconverged > 0) R.status = ST_NOCONV;
+  return R.status;
+}
+
+// Worst status of a batch: the first non-NOCONV failure, else NOCONV, else 0.
+int worst_status(int worst, int st) {
+  if (st != ST_OK && (worst == 0 || (st != ST_NOCONV && worst == ST_NOCONV))) return st;
+  return worst;
+}
+
+// Builds descriptors for `count` systems and enqueues k_als (one workgroup
+// per system) and the result copy on the context's buffers; ttals_wait
+// (finish_batch) unpacks.
+int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const ttals_tt* A, const ttals_tt* x0,
+                 const ttals_tt* b, const ttals_tt* xo, int repeats, int solver, double tol, int maxit, double sigma,
+                 double conv_eps) {
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is still pending on this context (call ttals_wait)");
+  BatchArgs B{kind, count, repeats, solver, maxit, tol, sigma, conv_eps, A, x0, b, xo};
+  Placement pl{nullptr, nullptr, 0, 0};
+  size_t r_res = 0, stride = 0;
+  int rc = enqueue_batch(c, stream, B, pl, &r_res, &stride, true);
+  if (rc) return rc;
   if (!c->done) HIPCHK(c, hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
   HIPCHK(c, hipEventRecord(c->done, stream));
   c->pending.clear();
-  for (int s = 0; s < count; ++s)
-    c->pending.push_back(Pending{r_res + 8 * res_stride * s, plans[s].d, xo[s].ranks});
+  for (int s = 0; s < count; ++s) c->pending.push_back(Pending{r_res + 8 * stride * s, A[s].order, xo[s].ranks});
   c->inflight = true;
   return 0;
 }
@@ -776,21 +931,7 @@ int finish_batch(ttals_ctx* c, ttals_result* res) {
   int worst = 0;
   for (size_t s = 0; s < c->pending.size(); ++s) {
     const Pending& pd = c->pending[s];
-    const double* o = hptr<double>(c, pd.r_res);
-    for (int i = 0; i <= pd.d; ++i) pd.ranks[i] = (int)o[16 + i];
-    ttals_result& R = res[s];
-    R.lambda = o[0];
-    R.sweeps = (int)o[1];
-    R.status = (int)o[2];
-    R.krylov_iters = (long long)o[3];
-    R.unconverged = (int)o[4];
-    R.local_solves = (long long)o[5];
-    R.max_iters = (int)o[6];
-    R.worst_residual = o[7];
-    R.flops = o[8];
-    for (int k = 0; k < 7; ++k) R.cycles[k] = o[9 + k];
-    if (R.status == ST_OK && R.unconverged > 0) R.status = ST_NOCONV;
-    if (R.status != ST_OK && (worst == 0 || (R.status != ST_NOCONV && worst == ST_NOCONV))) worst = R.status;
+    worst = worst_status(worst, unpack_result(hptr<double>(c, pd.r_res), pd.d, pd.ranks, res[s]));
   }
   return worst;
 }
@@ -822,6 +963,248 @@ bool dims_ok(std::initializer_list<int> v) {
   for (int x : v)
     if (x < 1) return false;
   return true;
+}
+
+// Workgroups that can be resident at once for the gang kernel (all CUs).
+int gang_capacity(ttals_ctx* c, int* cap) {
+  int nb = 0, cus = 0;
+  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_gang), NT, 0));
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  *cap = nb * cus;
+  return 0;
+}
+
+// Automatic gang size: one workgroup per row a' of the widest bond, at most 32.
+int gang_auto_size(const ttals_tt& x0) {
+  int r = 1;
+  for (int i = 0; i <= x0.order; ++i) r = std::max(r, x0.ranks[i]);
+  return std::max(1, std::min(32, r));
+}
+
+int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, const ttals_tt* x0, const ttals_tt* b,
+               const ttals_tt* xo, int repeats, int solver, double tol, int maxit, int prec, int gang,
+               ttals_result* res) {
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is still pending on this context (call ttals_wait)");
+  if (count < 1 || !A || !x0 || !b || !xo || !res) return fail(c, TTALS_E_ARG, "bad batch arguments");
+  if (repeats < 0 || !(tol > 0.0) || maxit < 1) return fail(c, TTALS_E_ARG, "bad repeats/tol/maxit");
+  if (prec != 0 && prec != 1) return fail(c, TTALS_E_ARG, "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+  if (gang > 256) return fail(c, TTALS_E_ARG, "gang size must be <= 256");
+  HIPCHK(c, hipSetDevice(c->device));
+  int cap = 0, rc;
+  if ((rc = gang_capacity(c, &cap))) return rc;
+  // pass 1: checks, gang sizes, totals
+  Layout hl, dl;
+  const size_t m_descs = hl.take(sizeof(GangDesc) * count);
+  int total = 0, dmax = 0;
+  SysSizes z;
+  for (int s = 0; s < count; ++s) {
+    if ((rc = size_system(c, 0, A[s], x0[s], &b[s], xo[s], s, z))) return rc;
+    dmax = std::max(dmax, z.d);
+    total += gang > 0 ? gang : gang_auto_size(x0[s]);
+  }
+  if (total > cap)
+    return fail(c, TTALS_E_ARG, "the gangs need " + std::to_string(total) + " resident workgroups; the device holds " +
+                                    std::to_string(cap));
+  const size_t m_blocks = hl.take(sizeof(int) * total);
+  struct GOff {
+    SysOff o;
+    size_t w_r, w_p, w_q, w_Vg, w_xt, w_At, w_L32, w_A32, w_R32, w_r32, w_p32, w_q32, w_x32, w_V32, w_t1, w_t2,
+        w_part, w_sync, w_res;
+    long tmax;
+    int G;
+  };
+  auto plan = [&](int s, Layout& h, Layout& dd, GOff& go) {
+    (void)size_system(c, 0, A[s], x0[s], &b[s], xo[s], s, z);
+    take_offsets(0, z, h, dd, go.o);
+    const int d = z.d;
+    long amax = 1, kmax = 1, lmax = 1, rmax2 = 1;
+    for (int i = 0; i < d; ++i) {
+      const long a = x0[s].ranks[i], bb = x0[s].ranks[i + 1], n = A[s].row_dims[i];
+      kmax = std::max(kmax, (long)A[s].ranks[i] * n * n * A[s].ranks[i + 1]);
+      lmax = std::max(lmax, a * A[s].ranks[i] * a);
+      rmax2 = std::max(rmax2, bb * A[s].ranks[i + 1] * bb);
+      amax = std::max(amax, a);
+    }
+    const int G = gang > 0 ? gang : gang_auto_size(x0[s]);
+    go.G = G;
+    go.tmax = z.tmax;
+    const size_t nm = (size_t)z.nmax;
+    go.w_r = dd.take(8 * nm);
+    go.w_p = dd.take(8 * nm);
+    go.w_q = dd.take(8 * nm);
+    go.w_Vg = dd.take(8 * (size_t)(KMAX + 1) * nm);
+    go.w_xt = dd.take(8 * nm);
+    go.w_At = dd.take(8 * (size_t)kmax);
+    const bool mixed = prec == 1;
+    go.w_L32 = dd.take(mixed ? 4 * (size_t)lmax : 4);
+    go.w_A32 = dd.take(mixed ? 4 * (size_t)kmax : 4);
+    go.w_R32 = dd.take(mixed ? 4 * (size_t)rmax2 : 4);
+    go.w_r32 = dd.take(mixed ? 4 * nm : 4);
+    go.w_p32 = dd.take(mixed ? 4 * nm : 4);
+    go.w_q32 = dd.take(mixed ? 4 * nm : 4);
+    go.w_x32 = dd.take(mixed ? 4 * nm : 4);
+    go.w_V32 = dd.take(mixed ? 4 * (size_t)(KMAX + 1) * nm : 4);
+    go.w_t1 = dd.take(8 * (size_t)G * z.tmax);
+    go.w_t2 = dd.take(8 * (size_t)G * z.tmax);
+    go.w_part = dd.take(8 * (size_t)G * GANG_LD);
+    go.w_sync = dd.take(sizeof(GangSync));
+    go.w_res = dd.take(8 * (16 + (size_t)d + 1));
+  };
+  {
+    Layout h1 = hl, d1 = dl;
+    GOff go;
+    for (int s = 0; s < count; ++s) plan(s, h1, d1, go);
+    hl = h1;
+    dl = d1;
+  }
+  const size_t meta_bytes = hl.bytes;
+  Layout rl;
+  rl.bytes = meta_bytes;
+  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t r_res = rl.take(8 * res_stride * count);
+  if ((rc = ensure_hst(c, rl.bytes))) return rc;
+  if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
+  char* hbase = c->hst;
+  char* dbase = reinterpret_cast<char*>(c->dws);
+  char* dmeta = dbase + dl.bytes;
+  auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
+  auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
+  auto fptr = [&](size_t off) { return reinterpret_cast<float*>(dbase + off); };
+  auto hp = [&](size_t off) { return reinterpret_cast<void*>(hbase + off); };
+  GangDesc* hdesc = reinterpret_cast<GangDesc*>(hbase + m_descs);
+  int* hblocks = reinterpret_cast<int*>(hbase + m_blocks);
+  // pass 2: descriptors (same offsets again)
+  Layout hl2, dl2;
+  (void)hl2.take(sizeof(GangDesc) * count);
+  (void)hl2.take(sizeof(int) * total);
+  int first = 0;
+  std::vector<size_t> res_off(count), sync_off(count);
+  for (int s = 0; s < count; ++s) {
+    GOff go;
+    plan(s, hl2, dl2, go);
+    const SysOff& o = go.o;
+    const ttals_tt& Ai = A[s];
+    const ttals_tt& xi = x0[s];
+    const ttals_tt& bi = b[s];
+    const int d = Ai.order;
+    int* hn = (int*)hp(o.m_n);
+    int* hrx = (int*)hp(o.m_rx);
+    int* hra = (int*)hp(o.m_ra);
+    int* hrb = (int*)hp(o.m_rb);
+    for (int i = 0; i < d; ++i) hn[i] = Ai.row_dims[i];
+    for (int i = 0; i <= d; ++i) {
+      hrx[i] = xi.ranks[i];
+      hra[i] = Ai.ranks[i];
+      hrb[i] = bi.ranks[i];
+    }
+    double** xcp = (double**)hp(o.m_xc);
+    const double** acp = (const double**)hp(o.m_ac);
+    const double** bcp = (const double**)hp(o.m_bc);
+    const double** x0p = (const double**)hp(o.m_x0c);
+    long* Lo = (long*)hp(o.m_Loff);
+    long* Ro = (long*)hp(o.m_Roff);
+    long* Lbo = (long*)hp(o.m_Lboff);
+    long* Rbo = (long*)hp(o.m_Rboff);
+    long lo = 0, ro = 0, lbo = 0, rbo = 0;
+    for (int i = 0; i < d; ++i) {
+      const long a = xi.ranks[i], bb = xi.ranks[i + 1];
+      x0p[i] = xi.cores[i];
+      xcp[i] = xo[s].cores[i];
+      acp[i] = Ai.cores[i];
+      bcp[i] = bi.cores[i];
+      Lo[i] = lo;
+      lo += a * Ai.ranks[i] * a;
+      Ro[i] = ro;
+      ro += bb * Ai.ranks[i + 1] * bb;
+      Lbo[i] = lbo;
+      lbo += (long)hrb[i] * a;
+      Rbo[i] = rbo;
+      rbo += (long)hrb[i + 1] * bb;
+    }
+    GangDesc& D = hdesc[s];
+    std::memset(&D, 0, sizeof(D));
+    D.d = d;
+    D.solver = solver;
+    D.repeats = repeats;
+    D.maxit = maxit;
+    D.prec = prec;
+    D.G = go.G;
+    D.first = first;
+    D.maxit_inner = maxit;
+    D.gm_restart = KMAX;
+    D.tol = tol;
+    D.tol_inner = 1e-5;
+    D.n = (const int*)dptr(o.m_n);
+    D.rx = (int*)dptr(o.m_rx);
+    D.ra = (const int*)dptr(o.m_ra);
+    D.rb = (const int*)dptr(o.m_rb);
+    D.xc = (double* const*)dptr(o.m_xc);
+    D.ac = (const double* const*)dptr(o.m_ac);
+    D.bc = (const double* const*)dptr(o.m_bc);
+    D.x0c = (const double* const*)dptr(o.m_x0c);
+    D.Ls = wptr(o.w_L);
+    D.Rs = wptr(o.w_R);
+    D.Lbs = wptr(o.w_Lb);
+    D.Rbs = wptr(o.w_Rb);
+    D.Loff = (const long*)dptr(o.m_Loff);
+    D.Roff = (const long*)dptr(o.m_Roff);
+    D.Lboff = (const long*)dptr(o.m_Lboff);
+    D.Rboff = (const long*)dptr(o.m_Rboff);
+    D.f = wptr(o.w_f);
+    D.g = wptr(o.w_g);
+    D.r = wptr(go.w_r);
+    D.p = wptr(go.w_p);
+    D.q = wptr(go.w_q);
+    D.V = wptr(go.w_Vg);
+    D.xt = wptr(go.w_xt);
+    D.At = wptr(go.w_At);
+    D.L32 = fptr(go.w_L32);
+    D.A32 = fptr(go.w_A32);
+    D.R32 = fptr(go.w_R32);
+    D.r32 = fptr(go.w_r32);
+    D.p32 = fptr(go.w_p32);
+    D.q32 = fptr(go.w_q32);
+    D.x32 = fptr(go.w_x32);
+    D.V32 = fptr(go.w_V32);
+    D.t1 = wptr(go.w_t1);
+    D.t2 = wptr(go.w_t2);
+    D.tmax = go.tmax;
+    D.part = wptr(go.w_part);
+    D.qw = wptr(o.w_qw);
+    D.qq = wptr(o.w_qq);
+    D.tau = wptr(o.w_tau);
+    D.rf = wptr(o.w_rf);
+    D.sync = reinterpret_cast<GangSync*>(dbase + go.w_sync);
+    D.out = wptr(o.w_out);
+    D.res = wptr(go.w_res);
+    res_off[s] = go.w_res;
+    sync_off[s] = go.w_sync;
+    for (int k = 0; k < go.G; ++k) hblocks[first + k] = s;
+    first += go.G;
+  }
+  HIPCHK(c, hipMemcpyAsync(dmeta, hbase, meta_bytes, hipMemcpyHostToDevice, stream));
+  for (int s = 0; s < count; ++s) HIPCHK(c, hipMemsetAsync(dbase + sync_off[s], 0, sizeof(GangSync), stream));
+  if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
+  if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
+  const GangDesc* ddesc = (const GangDesc*)dptr(m_descs);
+  const int* dblocks = (const int*)dptr(m_blocks);
+  void* args[] = {(void*)&ddesc, (void*)&dblocks};
+  HIPCHK(c, hipEventRecord(c->k0, stream));
+  HIPCHK(c, hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_gang), dim3(total), dim3(NT), args, 0, stream));
+  HIPCHK(c, hipEventRecord(c->k1, stream));
+  c->timed = true;
+  for (int s = 0; s < count; ++s)
+    HIPCHK(c, hipMemcpyAsync(hbase + r_res + 8 * res_stride * s, dbase + res_off[s], 8 * (16 + A[s].order + 1),
+                             hipMemcpyDeviceToHost, stream));
+  HIPCHK(c, hipStreamSynchronize(stream));
+  int worst = 0;
+  for (int s = 0; s < count; ++s) {
+    const int st = unpack_result(reinterpret_cast<const double*>(hbase + r_res + 8 * res_stride * s), A[s].order,
+                                 xo[s].ranks, res[s]);
+    if (st == ST_ABORT) return fail(c, TTALS_E_HIP, "gang barrier timed out (system " + std::to_string(s) + ")");
+    worst = worst_status(worst, st);
+  }
+  return worst;
 }
 
 }  // namespace
@@ -930,6 +1313,15 @@ int ttals_sle_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A
   if (!res) return c ? fail(c, TTALS_E_ARG, "null result array") : TTALS_E_ARG;
   int rc = ttals_sle_submit(c, stream, count, A, x0, b, x_out, repeats, solver, tol, maxit);
   return rc ? rc : finish_batch(c, res);
+}
+
+int ttals_sle_als_gang(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                       const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
+                       int precision, int gang, ttals_result* res) {
+  if (!c) return TTALS_E_ARG;
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "sle solver must be 'solve' or 'lu'");
+  return gang_solve(c, (hipStream_t)stream, count, A, x0, b, x_out, repeats, solver, tol, maxit, precision, gang,
+                    res);
 }
 
 int ttals_evp_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
@@ -1153,6 +1545,109 @@ int ttals_tt_norm(ttals_ctx* c, void* stream, const ttals_tt* t, double* out) {
   HIPCHK(c, hipMemcpyAsync(&h, na.out, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
   *out = h;
+  return 0;
+}
+
+int ttals_residual_norm_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x,
+                              const ttals_tt* b, double* res_norm, double* rhs_norm) {
+  if (!c) return TTALS_E_ARG;
+  if (count < 1 || !A || !x || !b || !res_norm || !rhs_norm) return fail(c, TTALS_E_ARG, "bad batch arguments");
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t st = (hipStream_t)stream;
+  // plans: shapes checked like a solve (square operator, matching dims, outer ranks 1)
+  struct RPlan {
+    SysPlan sp;
+    size_t m_n, m_ra, m_rx, m_rb, m_ac, m_xc, m_bc, w_W, w_U, w_R, w_tau, w_out;
+  };
+  std::vector<RPlan> plans(count);
+  Layout hl, dl;
+  const size_t m_descs = hl.take(sizeof(ResidDesc) * count);
+  for (int s = 0; s < count; ++s) {
+    RPlan& R = plans[s];
+    int rc = plan_system(c, 0, A[s], x[s], &b[s], R.sp);
+    if (rc) return fail(c, rc, "system " + std::to_string(s) + ": " + c->err);
+    const SysPlan& P = R.sp;
+    const int d = P.d;
+    // residual ranks rho_i and the QR ranks k_i (k_d = 1, k_i = min(n_i k_{i+1}, rho_i))
+    std::vector<long> rho(d + 1), k(d + 1);
+    for (int i = 0; i <= d; ++i) rho[i] = (i == 0 || i == d) ? 1 : (long)P.ra[i] * P.rx[i] + P.rb[i];
+    k[d] = 1;
+    long wmax = 1, umax = 1, rmax = 1;
+    for (int i = d - 1; i >= 0; --i) {
+      const long mk = (long)P.n[i] * k[i + 1];
+      wmax = std::max(wmax, mk * std::max(rho[i], (long)P.rb[i]));
+      umax = std::max(umax, std::max((long)P.rx[i] * P.n[i] * P.ra[i + 1] * k[i + 1], mk));
+      k[i] = std::min(mk, rho[i]);
+      rmax = std::max(rmax, rho[i]);
+    }
+    rmax = std::max(rmax, rho[d]);
+    R.m_n = hl.take(sizeof(int) * d);
+    R.m_ra = hl.take(sizeof(int) * (d + 1));
+    R.m_rx = hl.take(sizeof(int) * (d + 1));
+    R.m_rb = hl.take(sizeof(int) * (d + 1));
+    R.m_ac = hl.take(sizeof(void*) * d);
+    R.m_xc = hl.take(sizeof(void*) * d);
+    R.m_bc = hl.take(sizeof(void*) * d);
+    R.w_W = dl.take(8 * (size_t)wmax);
+    R.w_U = dl.take(8 * (size_t)umax);
+    R.w_R = dl.take(8 * (size_t)rmax * rmax);
+    R.w_tau = dl.take(8 * (size_t)rmax);
+    R.w_out = dl.take(8 * 2);
+  }
+  const size_t meta_bytes = hl.bytes;
+  Layout rl;
+  rl.bytes = meta_bytes;
+  const size_t r_out = rl.take(8 * 2 * (size_t)count);
+  int rc;
+  if ((rc = ensure_hst(c, rl.bytes))) return rc;
+  if ((rc = ensure_dws(c, dl.bytes + meta_bytes + 8 * 2 * (size_t)count))) return rc;
+  char* dbase = reinterpret_cast<char*>(c->dws);
+  char* dmeta = dbase + dl.bytes;
+  double* dres = reinterpret_cast<double*>(dmeta + meta_bytes);  // compact results
+  ResidDesc* hd = hptr<ResidDesc>(c, m_descs);
+  for (int s = 0; s < count; ++s) {
+    const RPlan& R = plans[s];
+    const SysPlan& P = R.sp;
+    const int d = P.d;
+    std::memcpy(hptr<int>(c, R.m_n), P.n.data(), sizeof(int) * d);
+    std::memcpy(hptr<int>(c, R.m_ra), P.ra.data(), sizeof(int) * (d + 1));
+    std::memcpy(hptr<int>(c, R.m_rx), P.rx.data(), sizeof(int) * (d + 1));
+    std::memcpy(hptr<int>(c, R.m_rb), P.rb.data(), sizeof(int) * (d + 1));
+    const double** ap = hptr<const double*>(c, R.m_ac);
+    const double** xp = hptr<const double*>(c, R.m_xc);
+    const double** bp = hptr<const double*>(c, R.m_bc);
+    for (int i = 0; i < d; ++i) {
+      ap[i] = A[s].cores[i];
+      xp[i] = x[s].cores[i];
+      bp[i] = b[s].cores[i];
+    }
+    ResidDesc& D = hd[s];
+    std::memset(&D, 0, sizeof(D));
+    D.d = d;
+    D.n = reinterpret_cast<const int*>(dmeta + R.m_n);
+    D.ra = reinterpret_cast<const int*>(dmeta + R.m_ra);
+    D.rx = reinterpret_cast<const int*>(dmeta + R.m_rx);
+    D.rb = reinterpret_cast<const int*>(dmeta + R.m_rb);
+    D.ac = reinterpret_cast<const double* const*>(dmeta + R.m_ac);
+    D.xc = reinterpret_cast<const double* const*>(dmeta + R.m_xc);
+    D.bc = reinterpret_cast<const double* const*>(dmeta + R.m_bc);
+    D.W = reinterpret_cast<double*>(dbase + R.w_W);
+    D.U = reinterpret_cast<double*>(dbase + R.w_U);
+    D.Rf = reinterpret_cast<double*>(dbase + R.w_R);
+    D.tau = reinterpret_cast<double*>(dbase + R.w_tau);
+    D.out = dres + 2 * s;
+  }
+  HIPCHK(c, hipMemcpyAsync(dmeta, c->hst, meta_bytes, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_residual_norm, dim3(count), dim3(NT), 0, st, (const ResidDesc*)dmeta);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->hst + r_out, dres, 8 * 2 * (size_t)count, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  const double* h = hptr<double>(c, r_out);
+  for (int s = 0; s < count; ++s) {
+    res_norm[s] = h[2 * s];
+    rhs_norm[s] = h[2 * s + 1];
+  }
   return 0;
 }
 

@@ -15,7 +15,13 @@ enum Status : int {
   ST_NOCONV = 2,        // a local Krylov solve hit its iteration cap (warning)
   ST_UNSUPPORTED = 3,   // sigma inside the spectrum of a large local problem
   ST_NONFINITE = 4,     // NaN/Inf in a local solution
+  ST_NONSYM = 5,        // evp: the operator is not symmetric (only symmetric local eigensolvers here)
 };
+
+// Relative asymmetry ||A - A^T||_F / ||A||_F above which evp rejects an
+// operator.  Computed from Gram transfer matrices, whose rounding puts the
+// value of an exactly symmetric operator near sqrt(eps) ~ 1e-8.
+constexpr double SYM_TOL = 1e-6;
 
 struct SysDesc {
   int d, kind, solver, repeats;
@@ -56,6 +62,7 @@ struct SysDesc {
   double* res;                // compact result row: out[0..15], then the d+1 final ranks
   double* trace;              // optional: 6 doubles per local solve (nullptr = off)
   long trace_cap;             // capacity in local solves
+  double* symw;               // evp: 4 * (max operator rank)^2 doubles for the symmetry test
 };
 
 // --- small helpers ---------------------------------------------------------
@@ -350,6 +357,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
   int its = 0;
   const double fmv = fl_op(a, P, n, Q, b);
   const bool dense = (D.kind == 0 && N <= NDENSE);
+  if (threadIdx.x == 0) S.vfl = 0.0;  // vector flops of this local solve, counted by the solvers
   if (D.kind == 0) {
     st.flops += fl_rhs(a, D.rb[i], n, D.rb[i + 1], b);
     micro_rhs<double, ACC>(a, D.rb[i], n, D.rb[i + 1], b, D.Lbs + D.Lboff[i], D.bc[i], D.Rbs + D.Rboff[i], D.f,
@@ -405,16 +413,17 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
         wg_copy(W.x, D.g, N);
         __syncthreads();
       }
-      // matvecs + vector work (Gram-Schmidt counted as two passes)
-      const double kb = sym ? 10.0 : 8.0 * 0.5 * KMAX;
-      st.flops += (its + 2) * fmv + (double)its * kb * N;
+      // matvecs (Krylov steps, the initial and the final residual) + the
+      // vector work the solver counted as it ran (Gram-Schmidt passes as made)
+      st.flops += (its + 2) * fmv + S.vfl + 6.0 * N;
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
     const int r = onchip ? wg_lanczos<ACC, true>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, LKMAX, lds, S, lam, &its)
                          : wg_lanczos<ACC, false>(op, D.g, W, mode, D.sigma, st.tol, D.maxit, LKMAX, lds, S, lam, &its);
-    // matvecs + two Gram-Schmidt passes against on average half the basis
-    st.flops += its * fmv + (double)its * 8.0 * N * 0.5 * (its < D.gm_restart ? its : D.gm_restart);
+    // matvecs + the vector work the solver counted as it ran (local passes
+    // every step, full Gram-Schmidt only where it was made, Ritz vectors)
+    st.flops += its * fmv + S.vfl;
     if (r == 3) return ST_UNSUPPORTED;
     if (r == 4) return ST_NONFINITE;
     if (r != 0) ++st.nunconv;
@@ -537,6 +546,70 @@ __device__ bool operator_symmetric(const SysDesc& D, SolveLDS& S) {
   return wg_sum(bad, S.red) == 0.0;
 }
 
+// ||A - A^T||_F / ||A||_F of the whole TT operator, for operators whose
+// slices are not symmetric one by one (the full operator can still be).
+// ||A - A^T||^2 = 2 <A, A> - 2 <A, A^T>; both inner products are chains of
+// transfer matrices E'[q,q'] = sum E[p,p'] A[p,m,n,q] B[p',m,n,q'] with
+// B = A or B[p',m,n,q'] = A[p',n,m,q'], rescaled after every core (the ratio
+// is all that is needed).  D.symw holds the two pairs of matrices.
+__device__ double operator_asymmetry(const SysDesc& D, SolveLDS& S) {
+  int rmax = 1;
+  for (int i = 0; i <= D.d; ++i) rmax = D.ra[i] > rmax ? D.ra[i] : rmax;
+  const long r2 = (long)rmax * rmax;
+  double* Ea = D.symw;            // <A, A> chain
+  double* Et = D.symw + r2;       // <A, A^T> chain
+  double* Fa = D.symw + 2 * r2;
+  double* Ft = D.symw + 3 * r2;
+  if (threadIdx.x == 0) {
+    Ea[0] = 1.0;
+    Et[0] = 1.0;
+  }
+  __syncthreads();
+  for (int i = 0; i < D.d; ++i) {
+    const int P = D.ra[i], n = D.n[i], Q = D.ra[i + 1];
+    const double* A = D.ac[i];
+    const long nn = (long)n * n;
+    double mx = 0.0;
+    for (int e = threadIdx.x; e < Q * Q; e += NT) {
+      const int q = e / Q, q2 = e - q * Q;
+      double sa = 0.0, st = 0.0;
+      for (int p = 0; p < P; ++p)
+        for (int p2 = 0; p2 < P; ++p2) {
+          const double ea = Ea[p * P + p2], et = Et[p * P + p2];
+          double ga = 0.0, gt = 0.0;
+          for (int m = 0; m < n; ++m)
+            for (int c = 0; c < n; ++c) {
+              const double x = A[((long)p * nn + (long)m * n + c) * Q + q];
+              ga += x * A[((long)p2 * nn + (long)m * n + c) * Q + q2];
+              gt += x * A[((long)p2 * nn + (long)c * n + m) * Q + q2];
+            }
+          sa += ea * ga;
+          st += et * gt;
+        }
+      Fa[e] = sa;
+      Ft[e] = st;
+      mx = fmax(mx, fabs(sa));
+    }
+    // rescale both chains by the same factor (the largest |<A,A>| entry)
+    mx = wave_max(mx);
+    __syncthreads();  // every F entry written, S.red free
+    if ((threadIdx.x & 63) == 0) S.red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    double gm = 0.0;
+    for (int w = 0; w < NWAVE; ++w) gm = fmax(gm, S.red[w]);
+    const double sc = gm > 0.0 ? 1.0 / gm : 1.0;
+    for (int e = threadIdx.x; e < Q * Q; e += NT) {
+      Ea[e] = Fa[e] * sc;
+      Et[e] = Ft[e] * sc;
+    }
+    __syncthreads();
+  }
+  const double aa = Ea[0], at = Et[0];
+  __syncthreads();
+  if (!(aa > 0.0)) return 0.0;  // the zero operator is symmetric
+  return sqrt(fmax(0.0, 2.0 * (1.0 - at / aa)));
+}
+
 template <class ACC>
 __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   const int d = D.d;
@@ -555,6 +628,12 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   // operator symmetric slice by slice (A[P,m,n,Q] == A[P,n,m,Q]): enables CG
   bool sym = D.sym > 0;
   if (D.sym < 0) sym = operator_symmetric(D, S);
+  // evp's local eigensolver (Lanczos) needs a symmetric operator: one whose
+  // slices are not symmetric one by one is checked as a whole and rejected
+  // when it is not symmetric (the reference's 'eig' would call dgeev)
+  if (D.kind == 1 && !sym) {
+    if (operator_asymmetry(D, S) > SYM_TOL) status = ST_NONSYM;
+  }
 
   if (D.kind == 0 && D.ortho_x0) ortho_right_x<ACC>(D, lds, arena, S);
 
@@ -566,7 +645,8 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     set_one(D.Rbs + D.Rboff[d - 1]);
   }
   __syncthreads();
-  for (int i = d - 2; i >= 0; --i) st.flops += build_right<ACC>(i, D, lds, arena);
+  if (status == ST_OK)
+    for (int i = d - 2; i >= 0; --i) st.flops += build_right<ACC>(i, D, lds, arena);
 
   bool converged = false;
   double lam1 = 0.0, lam2 = 0.0;  // eigenvalue after the last and the second-to-last sweep

@@ -50,6 +50,7 @@ struct SolveLDS {
   double scal[4];
   double part[(LKMAX + 2) * NWAVE];  // per-wave partial dot products
   long long prof[8];                // cycle counters per phase (thread 0)
+  double vfl;                       // vector flops of the current local solve (thread 0 only)
   int flag[4];
   int cnt[2 * NWAVE];               // per-wave ballot counts (double-buffered)
 };
@@ -149,9 +150,11 @@ __device__ __forceinline__ void wg_multiaxpy(const double* V, long ldv, int cnt,
 // orthogonal to the basis.  Accumulates the coefficients in hsum; returns
 // ||w|| after orthogonalisation and the norm before it in *w0.
 __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, double* h, double* hsum,
-                          double* part, double* red, double* w0) {
+                          double* part, double* red, double* w0, double* vfl) {
   double prev = 0.0, nrm = 0.0;
   for (int pass = 0; pass < 4; ++pass) {
+    // dots (and |w|^2 on the first pass), the update and the norm
+    if (threadIdx.x == 0) *vfl += 2.0 * N * (2 * cnt + (pass == 0 ? 1 : 0) + 1);
     wg_multidot(V, ldv, cnt, w, N, h, part, pass == 0);
     if (pass == 0) {
       prev = sqrt(fmax(h[cnt], 0.0));
@@ -189,8 +192,10 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
     __syncthreads();
     return 0;
   }
+  if (threadIdx.x == 0) S.vfl += 2.0 * N;
   for (int restart = 0; restart < 3; ++restart) {
     apply_op<double, ACC>(op, x, q, lds);
+    if (threadIdx.x == 0) S.vfl += 6.0 * N;  // this residual and the true-residual check below
     double rr = 0.0;
     for (int e = threadIdx.x; e < N; e += NT) {
       const double re = M::ld(f, e) - M::ld(q, e);
@@ -221,6 +226,7 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
       }
       wg_sum3(rn, d1, d2, S.red);
       ++*iters;
+      if (threadIdx.x == 0) S.vfl += 10.0 * N;  // two dots, two updates, the direction
       if (sqrt(rn) <= tol * fn) break;
       const double be = rn / rr;
       rr = rn;
@@ -265,6 +271,7 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     for (int e = threadIdx.x; e < N; e += NT) w[e] = f[e] - w[e];
     __syncthreads();
     const double beta = wg_nrm2(w, N, S.red);
+    if (threadIdx.x == 0) S.vfl += 4.0 * N;
     if (beta <= tol * fn) return 0;
     if (*iters >= maxit) return 2;
     const double ib = 1.0 / beta;
@@ -283,8 +290,9 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     for (int j = 0; j < m; ++j) {
       apply_op<double, ACC>(op, cur, w, lds);
       double w0;
-      double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.part, S.red, &w0);
+      double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.part, S.red, &w0, &S.vfl);
       if (!(hn > 1e-14 * w0)) hn = 0.0;  // (lucky) breakdown: Krylov space invariant
+      if (threadIdx.x == 0) S.vfl += (double)N;
       if (hn > 0.0) {
         const double ih = 1.0 / hn;
         for (int e = threadIdx.x; e < N; e += NT) {
@@ -333,6 +341,7 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
       for (int i = 0; i < jend; ++i) t += S.h[i] * V[(long)i * N + e];
       x[e] += t;
     }
+    if (threadIdx.x == 0) S.vfl += 2.0 * N * jend + N;
     __syncthreads();
     if (res <= tol * fn) {
       // confirm with the true residual once
@@ -834,6 +843,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
           w2 += we * we;
         }
         wg_sum3_nb(c1, c0, w2, S.red);
+        if (threadIdx.x == 0) S.vfl += 14.0 * N;  // pass 1: two dots; pass 2: update and three dots
         cc1 = c1;
         cc0 = c0;
         alpha = a1 + c1;
@@ -855,7 +865,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       if (dgks) {
         // w = A cur - (alpha cur + beta prv) (local pass already applied when !full_orth)
         double h0;
-        const double nb = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.part, S.red, &h0);
+        const double nb = wg_orth(V, N, j + 1, w, N, S.h, S.H, S.part, S.red, &h0, &S.vfl);
         if (full_orth) {
           w0 = h0;
           alpha = S.H[j];
@@ -884,7 +894,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
                           0.5);
         __syncthreads();
         double r0;
-        bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.part, S.red, &r0);
+        bn = wg_orth(V, N, j + 1, w, N, S.h, S.cs, S.part, S.red, &r0, &S.vfl);
         const double ib = bn > 0.0 ? 1.0 / bn : 0.0;
         double* t = prv;
         prv = cur;
@@ -1053,6 +1063,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
         V[(long)(j + 1) * N + e] = x;
       }
       beta_prev = bn;
+      if (threadIdx.x == 0) S.vfl += 5.0 * N;  // lazy correction and normalisation
       __syncthreads();
     }
     if (!have_vec) {
@@ -1083,6 +1094,7 @@ __device__ int wg_lanczos(const LocalOp<double>& op, double* v, const Work& W, i
       wg_sum3(ss, d1, d2, S.red);
     }
     vn = sqrt(ss);
+    if (threadIdx.x == 0) S.vfl += 2.0 * N * m + 3.0 * N;  // Ritz vector, its norm and scaling
     if (!isfinite(vn) || !(vn > 0.0)) {
       status = 4;
       break;

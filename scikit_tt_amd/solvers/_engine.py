@@ -1,4 +1,6 @@
 """Shared host plumbing for the fused ALS engine calls (libttals.so)."""
+import collections.abc
+import ctypes
 import warnings
 
 import numpy as np
@@ -35,6 +37,79 @@ class SolveInfo:
     def __repr__(self):
         names = [k for k in self.__slots__ if k != '_r'] + ['cycles']
         return 'SolveInfo(%s)' % ', '.join('%s=%r' % (k, getattr(self, k)) for k in names)
+
+
+RESULT_DTYPE = np.dtype([('status', '<i4'), ('sweeps', '<i4'), ('unconverged', '<i4'), ('max_iters', '<i4'),
+                         ('krylov_iters', '<i8'), ('local_solves', '<i8'), ('eigenvalue', '<f8'),
+                         ('worst_residual', '<f8'), ('flops', '<f8'), ('cycles', '<f8', (7,))])
+assert RESULT_DTYPE.itemsize == ctypes.sizeof(_lib.Result)
+
+
+class Infos(collections.abc.Sequence):
+    """Per-system reports of one batched call: ``infos[s]`` is a SolveInfo
+    (made on access); ``infos.array`` views every report as one numpy record
+    array (fields status, sweeps, ..., eigenvalue, flops), without
+    per-system Python objects."""
+
+    def __init__(self, res):
+        self._res = res
+        self.array = np.frombuffer(res, dtype=RESULT_DTYPE)
+
+    def __len__(self):
+        return len(self._res)
+
+    def __getitem__(self, s):
+        if isinstance(s, slice):
+            return [SolveInfo(self._res[k]) for k in range(*s.indices(len(self)))]
+        if s < 0:
+            s += len(self)
+        return SolveInfo(self._res[s])
+
+
+class Solutions(collections.abc.Sequence):
+    """Solution trains of one batched call, each made on access (views of
+    one device buffer): a call allocates O(1) Python objects however many
+    systems it solves."""
+
+    def __init__(self, cores, batch, initial_guesses):
+        self._cores, self._batch, self._x0 = cores, batch, initial_guesses
+
+    def __len__(self):
+        return len(self._x0)
+
+    def __getitem__(self, s):
+        if isinstance(s, slice):
+            return [self[k] for k in range(*s.indices(len(self)))]
+        if s < 0:
+            s += len(self)
+        return finish(self._cores, s, self._batch, self._x0[s])
+
+
+class EigenResults(collections.abc.Sequence):
+    """(eigenvalue, TT, iterations) per system of one batched evp call, made
+    on access; ``eigenvalues`` is a numpy array of all of them."""
+
+    def __init__(self, sols, infos):
+        self._sols, self._infos = sols, infos
+        self.eigenvalues = infos.array['eigenvalue'].copy()
+
+    def __len__(self):
+        return len(self._sols)
+
+    def __getitem__(self, s):
+        if isinstance(s, slice):
+            return [self[k] for k in range(*s.indices(len(self)))]
+        if s < 0:
+            s += len(self)
+        return (float(self.eigenvalues[s]), self._sols[s], int(self._infos.array['sweeps'][s]))
+
+
+def raise_first(infos, what, count):
+    """raise_status for the first system whose status is not 0 (vectorised scan)."""
+    bad = np.nonzero(infos.array['status'] != 0)[0]
+    if len(bad):
+        s = int(bad[0])
+        raise_status(infos[s], '%s system %d' % (what, s) if count > 1 else what)
 
 
 def check_tts(*groups):
@@ -108,5 +183,8 @@ def raise_status(info, what):
     if st == _lib.S_UNSUPPORTED:
         raise NotImplementedError('%s: the eigenvalue nearest sigma lies inside the spectrum of a local problem '
                                   'larger than the Krylov space' % what)
+    if st == _lib.S_NONSYM:
+        raise NotImplementedError('%s: the operator is not symmetric; local eigensolves of non-symmetric '
+                                  'operators (the reference\'s dgeev path) are not implemented' % what)
     if st == _lib.S_NOCONV:
         warnings.warn('%s: %d local Krylov solve(s) stopped at maxit' % (what, info.unconverged), RuntimeWarning)

@@ -20,7 +20,7 @@ from .. import _lib
 from ..tensor_train import TT, _stream
 from . import _engine
 
-__all__ = ['als', 'als_batch']
+__all__ = ['als', 'als_batch', 'power_method']
 
 _SOLVERS = {'eig': 0, 'eigh': 1}
 ADAPTIVE_CAP_EXP = 9  # adaptive local tolerance never looser than 1e-9
@@ -48,8 +48,9 @@ def als_batch(operators, initial_guesses, repeats=1, conv_eps=1e-10, solver='eig
     min(1e-9, 0.1 sqrt(|dlambda|/|lambda|)), never below ``tol``.  On the
     512-chain TFIM benchmark every eigenvalue stays within 5e-12 of the
     fixed-tolerance result with 1.8x fewer Krylov steps.
-    Returns a list of (eigenvalue, TT, iterations) tuples (and the list of
-    SolveInfo if ``return_info``).
+    Returns a sequence of (eigenvalue, TT, iterations) tuples (and a
+    sequence of SolveInfo if ``return_info``), each made on access;
+    ``.eigenvalues`` holds every eigenvalue as a numpy array.
     """
     _check_options([], None, 1, solver, True)
     count = len(operators)
@@ -70,14 +71,10 @@ def als_batch(operators, initial_guesses, repeats=1, conv_eps=1e-10, solver='eig
     finally:
         lib.ttals_set_local_tol_policy(ctx, 0)
     _lib.check(rc, ctx, 'evp.als')
-    infos = [_engine.SolveInfo(r) for r in res]
+    infos = _engine.Infos(res)
     if not return_info:
-        for s, info in enumerate(infos):
-            _engine.raise_status(info, 'evp.als system %d' % s if count > 1 else 'evp.als')
-    out = []
-    for s in range(count):
-        t = _engine.finish(cores, s, opacks, initial_guesses[s])
-        out.append((infos[s].eigenvalue, t, infos[s].sweeps))
+        _engine.raise_first(infos, 'evp.als', count)
+    out = _engine.EigenResults(_engine.Solutions(cores, opacks, initial_guesses), infos)
     return (out, infos) if return_info else out
 
 
@@ -92,3 +89,31 @@ def als(operator, initial_guess, previous=[], shift=0, operator_gevp=None, numbe
     _engine.check_tts([operator, initial_guess])
     return als_batch([operator], [initial_guess], repeats=repeats, conv_eps=conv_eps, solver=solver,
                      sigma=sigma)[0]
+
+
+def power_method(operator, initial_guess, operator_gevp=None, repeats=10, sigma=0.999):
+    """Inverse power iteration (scikit_tt/solvers/evp.py:182-250) on the device.
+
+    Each iteration is an ``sle.als`` solve with the shifted operator
+    A - sigma I (or A - sigma B for a generalised problem), the new iterate
+    normalised, and the eigenvalue taken as the Rayleigh quotient
+    x^T A x (divided by x^T B x when B is given).  Returns (eigenvalue, TT).
+    """
+    from .. import tensor_train as tt
+    from . import sle
+    if operator_gevp is None:
+        operator_shift = operator - sigma * tt.eye(operator.row_dims)
+    else:
+        operator_shift = operator - sigma * operator_gevp
+    eigenvalue = 0
+    eigentensor = initial_guess
+    for _ in range(repeats):
+        if operator_gevp is None:
+            eigentensor = sle.als(operator_shift, eigentensor, eigentensor)
+        else:
+            eigentensor = sle.als(operator_shift, eigentensor, operator_gevp.dot(eigentensor))
+        eigentensor = (1 / eigentensor.norm()) * eigentensor
+        eigenvalue = eigentensor.transpose().dot(operator).dot(eigentensor)
+        if operator_gevp is not None:
+            eigenvalue *= 1 / (eigentensor.transpose().dot(operator_gevp).dot(eigentensor))
+    return eigenvalue, eigentensor

@@ -2,8 +2,9 @@
 
 ``TT`` mirrors the attributes of scikit_tt/tensor_train.py:10-267 (``order``,
 ``row_dims``, ``col_dims``, ``ranks``, ``cores``) but keeps every core as a
-float64 HIP device tensor.  ``t.cores[i]`` returns a host numpy copy and
-``t.cores[i] = arr`` uploads; the device tensors are ``t.device_cores``.
+float64 HIP device tensor.  ``t.cores[i]`` returns a read-only host numpy
+copy and ``t.cores[i] = arr`` uploads; the device tensors are
+``t.device_cores``.
 
 All arithmetic on the ALS path runs in libttals.so (hand-written HIP):
 ``dot``/``@`` (core products), ``+``/``-``/``*`` (block embedding), ``norm``
@@ -18,7 +19,7 @@ import torch
 
 from . import _lib
 
-__all__ = ['TT', 'zeros', 'ones', 'eye', 'rand', 'residual_error', 'device', 'from_host_batch']
+__all__ = ['TT', 'zeros', 'ones', 'eye', 'rand', 'residual_error', 'residual_norms', 'device', 'from_host_batch']
 
 
 def device():
@@ -83,8 +84,16 @@ def from_host_batch(core_lists):
     return out
 
 
+def _host_readonly(c):
+    a = c.detach().cpu().numpy()
+    a.setflags(write=False)
+    return a
+
+
 class _CoreList:
-    """List-like view of the cores: reads give host numpy copies, writes upload."""
+    """List-like view of the cores: reads give READ-ONLY host numpy copies
+    (an in-place edit such as ``t.cores[i][...] = v`` raises instead of being
+    lost on a copy); assigning ``t.cores[i] = arr`` uploads."""
 
     def __init__(self, tt):
         self._tt = tt
@@ -94,8 +103,8 @@ class _CoreList:
 
     def __getitem__(self, i):
         if isinstance(i, slice):
-            return [c.detach().cpu().numpy() for c in self._tt.device_cores[i]]
-        return self._tt.device_cores[i].detach().cpu().numpy()
+            return [_host_readonly(c) for c in self._tt.device_cores[i]]
+        return _host_readonly(self._tt.device_cores[i])
 
     def __setitem__(self, i, value):
         t = self._tt
@@ -374,7 +383,9 @@ class TT:
                                                                                max_rank=max_rank)
 
     def norm(self, p=2):
-        """Norm (tensor_train.py:1334-1430); p=2 runs on the device."""
+        """Norm (tensor_train.py:1334-1430): p=2 the Euclidean/Frobenius norm
+        (device QR sweep), p=1 the Manhattan norm / maximum column sum of a
+        non-negative train (device core products)."""
         if p == 2:
             ctx = _lib.context(self.device.index)
             pack = self._desc()
@@ -383,7 +394,16 @@ class TT:
                                                  ctypes.byref(out)), ctx, 'tt_norm')
             return float(out.value)
         if p == 1:
-            raise NotImplementedError('p=1 norm is not part of the ALS path in scikit_tt_amd')
+            # Manhattan norm (tensor) / maximum column sum (operator), entries
+            # assumed non-negative as in the reference: the row modes are
+            # summed by a core product with all-ones row cores (1, 1, m, 1)
+            t = self.transpose() if all(m == 1 for m in self.row_dims) else self
+            dev = t.device
+            ones = TT._from_device([torch.ones((1, 1, m, 1), dtype=torch.float64, device=dev) for m in t.row_dims])
+            s = ones.dot(t)
+            if isinstance(s, float):
+                return s
+            return float(np.max(s.matricize()))
         raise ValueError('p must be 1 or 2.')
 
     def transpose(self):
@@ -472,7 +492,34 @@ def rand(row_dims, col_dims, ranks=1):
 def residual_error(operator, lhs, rhs):
     """||operator @ lhs - rhs|| (tensor_train.py:2035-2074).
 
-    Same quantity as the reference's core-wise SVD compression, computed as
-    the device QR-sweep norm of the explicit difference.
+    Same quantity as the reference's core-wise SVD compression; computed on
+    the device by one right-to-left Householder sweep over the block cores
+    of operator @ lhs - rhs, generated on the fly (never stored).
     """
-    return (operator.dot(lhs) - rhs).norm()
+    return float(residual_norms([operator], [lhs], [rhs])[0][0])
+
+
+def residual_norms(operators, lhss, rhss):
+    """Residual norms of many systems in ONE launch.
+
+    Returns two float64 arrays: ``||A_s x_s - b_s||`` and ``||b_s||``, the
+    numerator and denominator of the reference's rel.res check
+    ``(A.dot(x) - b).norm() / b.norm()`` (tests/test_sle.py:52-53).
+    """
+    count = len(operators)
+    if count == 0 or len(lhss) != count or len(rhss) != count:
+        raise ValueError('operators, solutions and right-hand sides must be non-empty lists of equal length')
+    for t in list(operators) + list(lhss) + list(rhss):
+        if not isinstance(t, TT):
+            raise TypeError('Unsupported argument.')
+    dev = operators[0].device
+    dA = _lib.DescBatch.from_tts(operators)
+    dX = _lib.DescBatch.from_tts(lhss)
+    dB = _lib.DescBatch.from_tts(rhss)
+    res = np.zeros(count)
+    bn = np.zeros(count)
+    ctx = _lib.context(dev.index)
+    _lib.check(_lib.load().ttals_residual_norm_batch(
+        ctx, _stream(dev), count, dA.ptr, dX.ptr, dB.ptr, res.ctypes.data_as(_lib.c_double_p),
+        bn.ctypes.data_as(_lib.c_double_p)), ctx, 'residual_norm_batch')
+    return res, bn

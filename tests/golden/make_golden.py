@@ -99,5 +99,98 @@ def main(ref_root):
     print(out)
 
 
+def c2_instance(ref_root):
+    """evp.als on the benchmark's TFIM instance 0 (d=32, n=2, MPO rank 3,
+    rank-16 N(0,1) start right-orthonormalised and normalised), 6 sweeps,
+    'eig', sigma=-100, conv_eps=0: the full-size eigenproblem configuration."""
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    from scikit_tt.tensor_train import TT
+    import scikit_tt.solvers.evp as revp
+    from scikit_tt_amd import workloads as W
+
+    H, y0, J, g = W.tfim_chain(seed=0, d=32, rank=16)
+    y0t = TT(y0).ortho_right()
+    y0t = (1.0 / y0t.norm()) * y0t
+    lam, y, it = revp.als(TT(H), y0t, repeats=6, conv_eps=0, solver='eig', sigma=-100.0)
+    s = {}
+    _save_tt(s, 'A', H)
+    _save_tt(s, 'x0', y0t.cores)
+    _save_tt(s, 'x', y.cores)
+    s['ranks'] = np.array(y.ranks)
+    s['eigenvalue'] = np.array(float(np.real(lam)))
+    s['iterations'] = np.array(it)
+    s['repeats'] = np.array(6)
+    s['conv_eps'] = np.array(0.0)
+    s['sigma'] = np.array(-100.0)
+    s['solver'] = np.array('eig')
+    s['J'] = J
+    s['g'] = g
+    np.savez_compressed(os.path.join(HERE, 'evp_tfim_d32_c2.npz'), **s)
+    print('evp_tfim_d32_c2', float(np.real(lam)), int(it))
+
+
+def ode_cases(ref_root):
+    """Implicit integrators on the reference's own ODE test operator
+    (signaling cascade, 2 species, quantised to 12 modes of size 2,
+    tests/test_ode.py:176-192) with fewer steps: implicit Euler (20 unit
+    steps), trapezoidal rule (mixed steps) and the adaptive method; the
+    operator, start values and the returned trains are stored."""
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    import scikit_tt.tensor_train as rtt
+    import scikit_tt.models as mdl
+    import scikit_tt.solvers.ode as rode
+    op = mdl.signaling_cascade(2).tt2qtt([[2] * 6] * 2, [[2] * 6] * 2)
+    iv = rtt.unit(op.row_dims, [0] * op.order)
+    ig = rtt.ones(op.row_dims, [1] * op.order, ranks=4).ortho_right()
+    s = {}
+    _save_tt(s, 'A', op.cores)
+    _save_tt(s, 'iv', iv.cores)
+    _save_tt(s, 'ig', ig.cores)
+    ie = rode.implicit_euler(op, iv, ig, [1.0] * 20, progress=False)
+    _save_tt(s, 'ie_last', ie[-1].cores)
+    s['ie_errors'] = np.array(rode.errors_impl_euler(op, ie, [1.0] * 20))
+    steps = [0.01] * 5 + [0.5] * 5
+    tr = rode.trapezoidal_rule(op, iv, ig, steps, progress=False)
+    _save_tt(s, 'tr_last', tr[-1].cores)
+    s['tr_errors'] = np.array(rode.errors_trapezoidal(op, tr, steps))
+    ad, times = rode.adaptive_step_size(op, iv, ig, 30, step_size_first=1, second_method='two_step_Euler',
+                                        progress=False)
+    _save_tt(s, 'ad_last', ad[-1].cores)
+    s['ad_times'] = np.array(times)
+    np.savez_compressed(os.path.join(HERE, 'ode_signaling_cascade.npz'), **s)
+    print('ode', s['ie_errors'][-1], s['tr_errors'][-1], len(times))
+
+
+def power_case(ref_root):
+    """evp.power_method on a TFIM chain (d=6, rank 4 start), sigma below the
+    spectrum: the inverse iteration converges to the ground state."""
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    from scikit_tt.tensor_train import TT
+    import scikit_tt.solvers.evp as revp
+    from scikit_tt_amd import workloads as W
+    H, y0, J, g = W.tfim_chain(seed=5, d=6, rank=4)
+    y0t = TT(y0).ortho_right()
+    y0t = (1.0 / y0t.norm()) * y0t
+    lam, y = revp.power_method(TT(H), y0t, repeats=8, sigma=-9.0)
+    s = {}
+    _save_tt(s, 'A', H)
+    _save_tt(s, 'x0', y0t.cores)
+    _save_tt(s, 'x', y.cores)
+    s['eigenvalue'] = np.array(float(lam))
+    s['sigma'] = np.array(-9.0)
+    s['repeats'] = np.array(8)
+    s['exact'] = np.array(W.tfim_ground_energy(J, g))
+    np.savez_compressed(os.path.join(HERE, 'evp_power_tfim_d6.npz'), **s)
+    print('power', float(lam), W.tfim_ground_energy(J, g))
+
+
+CASES = {'base': main, 'c2': c2_instance, 'ode': ode_cases, 'power': power_case}
+
 if __name__ == '__main__':
-    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
+    # usage: make_golden.py [reference root] [case ...]   (cases: base, c2, ode, power; default: all)
+    root = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
+    for name in (sys.argv[2:] or list(CASES)):
+        CASES[name](root)

@@ -1,0 +1,96 @@
+"""GPU: the multi-workgroup (gang) engine against the fused one-workgroup
+engine and the oracle, FP64 and mixed precision (FP32 Krylov iterations with
+FP64 refinement of the local residual).
+
+Both engines solve every local problem to the relative residual 1e-12 by
+different Krylov paths (the gang engine uses Krylov also where the fused
+engine factors small problems densely), so iterates agree to ~1e-9, not bit
+for bit; ranks are identical.  The mixed path must agree with the FP64 path
+to the same level, since it refines to the same local tolerance.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import tt_ops, sle_als
+from scikit_tt_amd import workloads as W
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def env():
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    from scikit_tt_amd import _lib
+    torch.cuda.set_device(0)
+    _lib.load()
+
+
+def _tt(cores):
+    from scikit_tt_amd.tensor_train import TT
+    return TT([np.asarray(c) for c in cores])
+
+
+def _reldiff(x, y):
+    return tt_ops.norm(tt_ops.sub(x, y)) / tt_ops.norm(y)
+
+
+@pytest.mark.parametrize('gang', [1, 3, 0])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_gang_laplace_vs_oracle(env, gang, precision):
+    """Symmetric operator (CG path), uneven gang splits of the rank rows."""
+    import scikit_tt_amd.solvers.sle as sle
+    print('case laplace d=6 n=5 r=5 gang=%d %s' % (gang, precision), flush=True)
+    A, x0, b = W.laplace_small(seed=4, d=6, n=5, rank=5)
+    (x,), (info,) = sle.als_gang([_tt(A)], [_tt(x0)], [_tt(b)], repeats=2, precision=precision, gang=gang,
+                                 return_info=True)
+    assert info.status == 0, info
+    xr = sle_als.als(A, x0, b, repeats=2)
+    assert x.ranks == tt_ops.ranks_of(xr)
+    assert _reldiff(x.cores[:], xr) < 1e-8
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_gang_nonsymmetric_batch_vs_fused(env, precision):
+    """Non-symmetric operators (GMRES path), several gangs in one launch."""
+    import scikit_tt_amd.solvers.sle as sle
+    print('case shifted batch %s' % precision, flush=True)
+    systems = [W.shifted_batch_instance(s, d_range=(4, 7), n_choices=(3, 4), r_range=(3, 9)) for s in range(4)]
+    As = [_tt(A) for A, _, _ in systems]
+    Xs = [_tt(x) for _, x, _ in systems]
+    Bs = [_tt(b) for _, _, b in systems]
+    gang_sols, infos = sle.als_gang(As, Xs, Bs, repeats=2, precision=precision, gang=2, return_info=True)
+    fused = sle.als_batch(As, Xs, Bs, repeats=2)
+    assert (infos.array['status'] == 0).all(), infos.array['status']
+    for k in range(len(systems)):
+        assert gang_sols[k].ranks == fused[k].ranks
+        assert _reldiff(gang_sols[k].cores[:], fused[k].cores[:]) < 1e-8, k
+
+
+def test_gang_full_size_laplace_mixed_vs_fp64(env):
+    """The large Laplacian configuration (d=20, n=32, rank 32, rank-4 N(0,1)
+    rhs): FP64 and mixed precision on one gang each, rel.res of both from the
+    FP64 residual kernel, the mixed result's distance to the FP64 one."""
+    import scikit_tt_amd.solvers.sle as sle
+    from scikit_tt_amd.tensor_train import residual_norms
+    A, x0, b = W.laplace_large()
+    tA, tx, tb = _tt(A), _tt(x0), _tt(b)
+    out = {}
+    for prec in ('fp64', 'mixed'):
+        print('case laplace_large %s' % prec, flush=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        (x,), (info,) = sle.als_gang([tA], [tx], [tb], repeats=2, precision=prec, return_info=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        res, bn = residual_norms([tA], [x], [tb])
+        out[prec] = (x, res[0] / bn[0], dt, info)
+        print('  %s: 2 sweeps %.3f s, rel.res %.3e, krylov %d' % (prec, dt, res[0] / bn[0], info.krylov_iters),
+              flush=True)
+    assert out['fp64'][1] < 2e-9
+    assert out['mixed'][1] < 1.1 * out['fp64'][1] + 1e-12
+    d = _reldiff(out['mixed'][0].cores[:], out['fp64'][0].cores[:])
+    print('  mixed vs fp64 iterate: %.3e' % d, flush=True)
+    assert d < 1e-7

@@ -583,3 +583,71 @@ def test_sle_uneven_batch_full_size(env):
         (alone,), _ = sle.als_batch([As[s]], [xs[s]], [bs[s]], repeats=2, return_info=True)
         for c1, c2 in zip(alone.device_cores, sols[s].device_cores):
             assert torch.equal(c1, c2)
+
+
+def _nonsym_slices_symmetric_operator(rng, d, n, scale=0.3):
+    """A = sum over k of B_k (x) B_k^T-chains: a rank-2 TT operator whose
+    slices are not symmetric one by one but whose whole matrix is symmetric
+    (core i = [[C_i, 0], [0, C_i^T]] with first/last cores closing the two
+    paths), plus a diagonal shift so the spectrum has a clear bottom."""
+    cores = []
+    for i in range(d):
+        C = scale * rng.standard_normal((n, n))
+        if i == 0:
+            c = np.zeros((1, n, n, 3))
+            c[0, :, :, 0], c[0, :, :, 1], c[0, :, :, 2] = C, C.T, np.eye(n)
+        elif i == d - 1:
+            c = np.zeros((3, n, n, 1))
+            c[0, :, :, 0], c[1, :, :, 0], c[2, :, :, 0] = C, C.T, np.diag(np.arange(n, dtype=float))
+        else:
+            c = np.zeros((3, n, n, 3))
+            c[0, :, :, 0], c[1, :, :, 1], c[2, :, :, 2] = C, C.T, np.eye(n)
+        cores.append(c)
+    return cores
+
+
+def test_evp_nonsymmetric_operator_raises(env):
+    """evp.als with an operator that is not symmetric raises NotImplementedError
+    (the reference's 'eig' would run dgeev, scikit_tt/solvers/evp.py:424-432)
+    instead of returning the eigenpair of a symmetric solver."""
+    import scikit_tt_amd.solvers.evp as evp
+    rng = np.random.default_rng(11)
+    A = W.random_tt(rng, [3] * 4, [1, 2, 2, 2, 1], col_dims=[3] * 4)
+    x0 = W.right_orthonormal(W.random_tt(rng, [3] * 4, 2))
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), repeats=2, conv_eps=0, solver='eig', sigma=-100)
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), repeats=2, conv_eps=0, solver='eigh')
+
+
+def test_evp_symmetric_operator_with_nonsymmetric_slices(env):
+    """The symmetry test looks at the whole operator: slices that are not
+    symmetric one by one but add up to a symmetric matrix are accepted, and
+    the eigenvalue matches the oracle's (dense local eigensolves)."""
+    import scikit_tt_amd.solvers.evp as evp
+    rng = np.random.default_rng(12)
+    A = _nonsym_slices_symmetric_operator(rng, 4, 3)
+    M = tt_ops.matricize(A)
+    assert np.abs(M - M.T).max() < 1e-12 * np.abs(M).max()
+    assert any(np.abs(c - c.transpose(0, 2, 1, 3)).max() > 0.1 for c in A)
+    x0 = W.right_orthonormal(W.random_tt(rng, [3] * 4, 3))
+    lam, y, it = evp.als(_tt(A), _tt(x0), repeats=3, conv_eps=0, solver='eig', sigma=-100)
+    lr, yr, itr = evp_als.als(A, x0, repeats=3, conv_eps=0, solver='eig', sigma=-100)
+    assert it == itr
+    assert abs(lam - lr) < 1e-10 * max(1.0, abs(lr)), (lam, lr)
+
+
+def test_output_descriptor_capacity_checked(env):
+    """The C ABI rejects output buffers smaller than the initial guess (no
+    out-of-bounds device writes for direct C callers)."""
+    lib, ctx, _lib = env
+    from scikit_tt_amd.tensor_train import TT
+    A, x0, b = W.laplace_small(d=4, n=3, rank=3)
+    tA, tx, tb = _tt(A), _tt(x0), _tt(b)
+    small = W.random_tt(np.random.default_rng(1), [3] * 4, 2)
+    ts = _tt(small)
+    res = (_lib.Result * 1)()
+    packs = [t._desc() for t in (tA, tx, tb, ts)]  # the descriptors' arrays stay alive through the call
+    rc = lib.ttals_sle_als_batch(ctx, None, 1, *[ctypes.byref(p.desc) for p in packs], 1, 0, 1e-12, 100, res)
+    assert rc == _lib.E_SHAPE
+    assert b'capacity' in lib.ttals_last_error(ctx)

@@ -283,6 +283,21 @@ def test_evp_golden(golden, name):
     assert min(d1, d2) < 1e-8
 
 
+def test_evp_golden_full_size_c2(golden):
+    """The full-size eigenproblem instance (d=32, rank 16, 6 sweeps): the
+    oracle reproduces the reference's eigenvalue and iteration count, and
+    the eigenvalue lies above the free-fermion ground energy."""
+    g = golden('evp_tfim_d32_c2')
+    lam, y, it = evp_als.als(g['tt']('A'), g['tt']('x0'), repeats=int(g['repeats']), conv_eps=float(g['conv_eps']),
+                             solver=str(g['solver']), sigma=float(g['sigma']))
+    ref = float(g['eigenvalue'])
+    assert it == int(g['iterations'])
+    assert abs(lam - ref) <= 1e-12 * abs(ref), (lam, ref)
+    assert tt_ops.ranks_of(y) == list(g['ranks'])
+    E0 = W.tfim_ground_energy(g['J'], g['g'])
+    assert E0 - 1e-12 * abs(E0) <= ref < E0 + 1e-7 * abs(E0)
+
+
 def test_tfim_closed_form_vs_dense():
     for d in (2, 4, 6, 8):
         rng = np.random.default_rng(20 + d)
