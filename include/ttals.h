@@ -95,6 +95,23 @@ int ttals_sle_als_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt*
                         const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
                         ttals_result* res);
 
+/* Allocation-free form of ttals_sle_als_batch: same arguments, results and
+ * kernels, but every device buffer the engine needs beyond the caller's
+ * inputs and outputs (interface stacks, Krylov basis, QR and LU scratch,
+ * descriptors, result rows) lives in a static device array of 64 MiB per GPU
+ * (`scratch`, held by each GPU the library runs on until the process ends)
+ * and the host staging in a static 4 MiB host buffer.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree, hipHostMalloc or event creation.  A batch needing more than the
+ * static sizes is rejected with TTALS_E_ARG (message: the bytes needed).
+ * Calls are serialised by a process-wide lock (they share the arrays); the
+ * arrays keep the last call's contents, and the engine reads nothing of them
+ * it has not written in the same call.  Synchronous. */
+int ttals_sle_als_batch_static(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                               const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol,
+                               int maxit, ttals_result* res);
+
 /* Replaces scikit_tt.solvers.sle.als (scikit_tt/solvers/sle.py:10-95) for
  * large systems: each system is solved by a GANG of `gang` workgroups (<= 0:
  * automatic, one per row of the widest bond up to 32) that split every

@@ -53,6 +53,7 @@ SIGNATURES = {
     'ttals_sle_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _Rp]),
     'ttals_evp_als_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i, _Rp]),
     'ttals_sle_als_gang': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _i, _i, _Rp]),
+    'ttals_sle_als_batch_static': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i, _Rp]),
     'ttals_sle_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _TTp, _i, _i, _d, _i]),
     'ttals_evp_submit': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, _i, _d, _i, _d, _d, _i]),
     'ttals_query': (_i, [_vp]),

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -424,6 +425,18 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point
+// (ttals_sle_als_batch_static): every GPU the library runs on holds one copy
+// of this array, at its own address, until the process ends.  Host code
+// never passes `scratch` itself to a kernel or a copy (on the host the name
+// is only a stand-in): ttals_ctx_create resolves each device's address once
+// with hipGetSymbolAddress, and that address is what the entry point uses.
+constexpr size_t TTALS_STATIC_DEV_BYTES = (size_t)64 << 20;   // device scratch per GPU
+constexpr size_t TTALS_STATIC_HOST_BYTES = (size_t)4 << 20;   // host staging (metadata and results)
+constexpr int TTALS_MAX_DEVICES = 64;
+static __device__ __align__(256) char scratch[TTALS_STATIC_DEV_BYTES];
+
+// ---------------------------------------------------------------------------
 // host side
 
 struct Pending {
@@ -454,6 +467,12 @@ struct ttals_ctx {
 };
 
 namespace {
+
+// per-device address of `scratch`, resolved once per device; the host
+// staging of the static entry point; the lock that serialises its calls
+std::mutex g_static_mu;
+void* g_scratch_dev[TTALS_MAX_DEVICES] = {};
+alignas(256) char g_static_host[TTALS_STATIC_HOST_BYTES];
 
 int fail(ttals_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -1220,6 +1239,22 @@ int ttals_ctx_create(int device, ttals_ctx** out) {
   if (!out) return TTALS_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return TTALS_E_ARG;
+  if (device >= TTALS_MAX_DEVICES) return TTALS_E_ARG;
+  {
+    // the static scratch's address on this device, once per device, with
+    // the device current (the caller's current device is restored)
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_scratch_dev[device]) {
+      int prev = 0;
+      if (hipGetDevice(&prev) != hipSuccess) return TTALS_E_HIP;
+      if (hipSetDevice(device) != hipSuccess) return TTALS_E_HIP;
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      (void)hipSetDevice(prev);
+      if (e != hipSuccess || !p) return TTALS_E_HIP;
+      g_scratch_dev[device] = p;
+    }
+  }
   ttals_ctx* c = new ttals_ctx();
   c->device = device;
   if (const char* e = std::getenv("TTALS_ARENA_KB")) {
@@ -1313,6 +1348,30 @@ int ttals_sle_als_batch(ttals_ctx* c, void* stream, int count, const ttals_tt* A
   if (!res) return c ? fail(c, TTALS_E_ARG, "null result array") : TTALS_E_ARG;
   int rc = ttals_sle_submit(c, stream, count, A, x0, b, x_out, repeats, solver, tol, maxit);
   return rc ? rc : finish_batch(c, res);
+}
+
+int ttals_sle_als_batch_static(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
+                               const ttals_tt* b, const ttals_tt* x_out, int repeats, int solver, double tol, int maxit,
+                               ttals_result* res) {
+  if (!c) return TTALS_E_ARG;
+  if (!res) return fail(c, TTALS_E_ARG, "null result array");
+  if (solver != 0 && solver != 1) return fail(c, TTALS_E_UNSUPPORTED, "sle solver must be 'solve' or 'lu'");
+  if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is still pending on this context (call ttals_wait)");
+  void* dev = g_scratch_dev[c->device];
+  if (!dev) return fail(c, TTALS_E_HIP, "static scratch address not resolved for this device");
+  // one call at a time: calls share the device scratch and the host staging
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  BatchArgs B{0, count, repeats, solver, maxit, tol, 0.0, 0.0, A, x0, b, x_out};
+  Placement pl{g_static_host, reinterpret_cast<char*>(dev), TTALS_STATIC_HOST_BYTES, TTALS_STATIC_DEV_BYTES};
+  size_t r_res = 0, stride = 0;
+  int rc = enqueue_batch(c, (hipStream_t)stream, B, pl, &r_res, &stride, false);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize((hipStream_t)stream));
+  int worst = 0;
+  for (int s = 0; s < count; ++s)
+    worst = worst_status(worst, unpack_result(reinterpret_cast<const double*>(g_static_host + r_res + 8 * stride * s),
+                                              A[s].order, x_out[s].ranks, res[s]));
+  return worst;
 }
 
 int ttals_sle_als_gang(ttals_ctx* c, void* stream, int count, const ttals_tt* A, const ttals_tt* x0,
