@@ -247,6 +247,22 @@ int ttals_tt_norm(ttals_ctx* ctx, void* stream, const ttals_tt* t, double* out);
 int ttals_residual_norm_batch(ttals_ctx* ctx, void* stream, int count, const ttals_tt* A, const ttals_tt* x,
                               const ttals_tt* b, double* res_norm, double* rhs_norm);
 
+/* scikit_tt/utils.py:111-158 (truncated_svd) and the two-site split of
+ * sle.mals (scikit_tt/solvers/sle.py:603-645): SVD of the row-major (m x c)
+ * device matrix Y, keeping s/s0 > threshold (rel != 0) or s > threshold
+ * (rel == 0) when threshold != 0, then at most max_rank (<= 0: no cap).
+ * U (m x k), s (k), Vt (k x c) row-major device buffers with room for
+ * k = min(m, c); k (host) receives the kept rank.  Householder QR of the
+ * tall orientation, then a one-sided Jacobi SVD of R.  Synchronous. */
+int ttals_truncated_svd(ttals_ctx* ctx, void* stream, int m, int c, const double* Y, double threshold, int rel,
+                        int max_rank, double* U, double* s, double* Vt, int* k);
+
+/* Supercore of two neighbouring cores (the two-site operator / rhs / vector
+ * of sle.mals, scikit_tt/solvers/sle.py:376-388, 424-428):
+ * out[r1, (m1,m2), (n1,n2), r2] = sum_s A[r1,m1,n1,s] B[s,m2,n2,r2]. */
+int ttals_core_merge(void* stream, int r1, int m1, int n1, int s, int m2, int n2, int r2, const double* A,
+                     const double* B, double* out);
+
 /* scikit_tt/tensor_train.py:1092-1205 (dir 0, ortho_left over cores
  * start..end) and 1207-1310 (dir 1, ortho_right over cores start..end
  * descending), in place: the cores keep their capacity and t->ranks receives

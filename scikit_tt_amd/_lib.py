@@ -79,6 +79,8 @@ SIGNATURES = {
     'ttals_tt_ortho': (_i, [_vp, _vp, _TTp, _i, _i, _i, _d, c_int_p]),
     'ttals_set_trace': (_i, [_vp, _vp, ctypes.c_longlong]),
     'ttals_residual_norm_batch': (_i, [_vp, _vp, _i, _TTp, _TTp, _TTp, c_double_p, c_double_p]),
+    'ttals_truncated_svd': (_i, [_vp, _vp, _i, _i, _vp, _d, _i, _i, _vp, _vp, _vp, c_int_p]),
+    'ttals_core_merge': (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
 }
 
 _lock = threading.Lock()

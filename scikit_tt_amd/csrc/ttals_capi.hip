@@ -424,6 +424,94 @@ __global__ __launch_bounds__(NT) void k_tt_ortho(OrthoArgs A) {
   }
 }
 
+// Truncated SVD of a row-major m x c matrix Y (one workgroup): Householder
+// QR of the tall orientation (Y, or Y^T when m < c), one-sided Jacobi SVD of
+// R^T, singular triplets kept by s/s0 > thr (rel) or s > thr (absolute) when
+// thr != 0, then at most maxr (the rule of scikit_tt/utils.py:145-156).
+// U (m x k), s (k), Vt (k x c) row-major; k written to *kout.
+struct SvdArgs {
+  int m, c, maxr, rel;
+  double thr;
+  const double* Y;
+  double* U;
+  double* s;
+  double* Vt;
+  double* W;    // mt x ct
+  double* Qm;   // mt x ct
+  double* B;    // ct x ct
+  double* Vm;   // ct x ct
+  double* L;    // mt x ct
+  double* sig;  // ct
+  double* tau;  // ct
+  int* perm;    // ct
+  int* kout;
+};
+__global__ __launch_bounds__(NT) void k_svd_trunc(SvdArgs A) {
+  __shared__ double lds[GEMM_LDS_DOUBLES];
+  __shared__ double red[4 * NWAVE];
+  const bool tall = A.m >= A.c;
+  const int mt = tall ? A.m : A.c, ct = tall ? A.c : A.m;
+  for (long e = threadIdx.x; e < (long)mt * ct; e += NT) {
+    const int i = (int)(e / ct), j = (int)(e % ct);
+    A.W[e] = tall ? A.Y[e] : A.Y[(long)j * A.c + i];
+  }
+  __syncthreads();
+  wg_householder(A.W, mt, ct, A.tau, red);
+  wg_form_q(A.W, mt, ct, A.tau, A.Qm);
+  // B = R^T (ct x ct)
+  for (long e = threadIdx.x; e < (long)ct * ct; e += NT) {
+    const int i = (int)(e / ct), j = (int)(e % ct);
+    A.B[e] = (i >= j) ? A.W[(long)j * ct + i] : 0.0;
+  }
+  __syncthreads();
+  wg_jacobi_svd(A.B, ct, ct, A.Vm, A.sig, A.perm, red);
+  int kk = 0;
+  const double s0 = A.sig[A.perm[0]];
+  for (int j = 0; j < ct; ++j) {
+    const double sj = A.sig[A.perm[j]];
+    if (A.thr == 0.0 || (A.rel ? (s0 > 0.0 && sj / s0 > A.thr) : sj > A.thr)) ++kk;
+  }
+  if (A.maxr > 0 && kk > A.maxr) kk = A.maxr;
+  // left factor of the tall orientation: (Q V)[:, perm[:kk]] (mt x kk)
+  wg_gemm<double, double>(mt, ct, ct, crm<double>(A.Qm, ct), crm<double>(A.Vm, ct), mrm<double>(A.L, ct), lds);
+  for (long e = threadIdx.x; e < (long)mt * kk; e += NT) {
+    const int i = (int)(e / kk), j = (int)(e % kk);
+    const double v = A.L[(long)i * ct + A.perm[j]];
+    if (tall) A.U[(long)i * kk + j] = v;       // U = left factor
+    else A.Vt[(long)j * mt + i] = v;            // Vt = (left factor)^T
+  }
+  // right factor: B[:, perm[j]] / s (ct x kk)
+  for (long e = threadIdx.x; e < (long)ct * kk; e += NT) {
+    const int i = (int)(e / kk), j = (int)(e % kk);
+    const double sj = A.sig[A.perm[j]];
+    const double v = sj > 0.0 ? A.B[(long)i * ct + A.perm[j]] / sj : 0.0;
+    if (tall) A.Vt[(long)j * ct + i] = v;       // Vt = (right factor)^T
+    else A.U[(long)i * kk + j] = v;             // U = right factor
+  }
+  for (int j = threadIdx.x; j < kk; j += NT) A.s[j] = A.sig[A.perm[j]];
+  if (threadIdx.x == 0) *A.kout = kk;
+}
+
+// Supercore of two neighbouring cores:
+// out[r1, (m1, m2), (n1, n2), r2] = sum_s A[r1, m1, n1, s] B[s, m2, n2, r2]
+__global__ void k_core_merge(int r1, int m1, int n1, int sr, int m2, int n2, int r2, const double* __restrict__ A,
+                             const double* __restrict__ B, double* __restrict__ out) {
+  const long total = (long)r1 * m1 * m2 * n1 * n2 * r2;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    long t = e;
+    const int j = (int)(t % r2); t /= r2;
+    const int q2 = (int)(t % n2); t /= n2;
+    const int q1 = (int)(t % n1); t /= n1;
+    const int p2 = (int)(t % m2); t /= m2;
+    const int p1 = (int)(t % m1); t /= m1;
+    const int i = (int)t;
+    double acc = 0.0;
+    for (int k = 0; k < sr; ++k)
+      acc += A[(((long)i * m1 + p1) * n1 + q1) * sr + k] * B[(((long)k * m2 + p2) * n2 + q2) * r2 + j];
+    out[e] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Static scratch of the allocation-free entry point
 // (ttals_sle_als_batch_static): every GPU the library runs on holds one copy
@@ -1708,6 +1796,60 @@ int ttals_residual_norm_batch(ttals_ctx* c, void* stream, int count, const ttals
     rhs_norm[s] = h[2 * s + 1];
   }
   return 0;
+}
+
+int ttals_truncated_svd(ttals_ctx* c, void* stream, int m, int cdim, const double* Y, double threshold, int rel,
+                        int max_rank, double* U, double* s, double* Vt, int* k) {
+  if (!c) return TTALS_E_ARG;
+  if (m < 1 || cdim < 1 || !Y || !U || !s || !Vt || !k || !(threshold >= 0.0))
+    return fail(c, TTALS_E_ARG, "bad truncated_svd arguments");
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long mt = std::max(m, cdim), ct = std::min(m, cdim);
+  Layout L;
+  const size_t oW = L.take(8 * mt * ct), oQ = L.take(8 * mt * ct), oB = L.take(8 * ct * ct), oV = L.take(8 * ct * ct),
+               oL = L.take(8 * mt * ct), osig = L.take(8 * ct), otau = L.take(8 * ct), operm = L.take(4 * ct),
+               ok = L.take(8);
+  int rc;
+  if ((rc = ensure_dws(c, L.bytes))) return rc;
+  char* db = reinterpret_cast<char*>(c->dws);
+  SvdArgs a;
+  a.m = m;
+  a.c = cdim;
+  a.maxr = max_rank;
+  a.rel = rel;
+  a.thr = threshold;
+  a.Y = Y;
+  a.U = U;
+  a.s = s;
+  a.Vt = Vt;
+  a.W = reinterpret_cast<double*>(db + oW);
+  a.Qm = reinterpret_cast<double*>(db + oQ);
+  a.B = reinterpret_cast<double*>(db + oB);
+  a.Vm = reinterpret_cast<double*>(db + oV);
+  a.L = reinterpret_cast<double*>(db + oL);
+  a.sig = reinterpret_cast<double*>(db + osig);
+  a.tau = reinterpret_cast<double*>(db + otau);
+  a.perm = reinterpret_cast<int*>(db + operm);
+  a.kout = reinterpret_cast<int*>(db + ok);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_svd_trunc, dim3(1), dim3(NT), 0, st, a);
+  HIPCHK(c, hipGetLastError());
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, a.kout, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  *k = h;
+  return 0;
+}
+
+int ttals_core_merge(void* stream, int r1, int m1, int n1, int sr, int m2, int n2, int r2, const double* A,
+                     const double* B, double* out) {
+  if (!dims_ok({r1, m1, n1, sr, m2, n2, r2}) || !A || !B || !out) return TTALS_E_ARG;
+  const long total = (long)r1 * m1 * m2 * n1 * n2 * r2;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_core_merge, dim3(blocks), dim3(256), 0, (hipStream_t)stream, r1, m1, n1, sr, m2, n2, r2, A,
+                     B, out);
+  return hipGetLastError() == hipSuccess ? 0 : TTALS_E_HIP;
 }
 
 int ttals_set_trace(ttals_ctx* c, double* buf, long long cap) {

@@ -93,7 +93,10 @@ constexpr long GANG_SPIN_LIMIT = 1L << 24;  // ~seconds: far beyond any phase
 constexpr int GANG_LD = 4 + 2 * (KMAX + 2);
 
 // Every workgroup of the gang calls this at the same points.  Returns false
-// once the gang has aborted (a wait ran out): callers then unwind.
+// once the gang has aborted (a wait ran out): callers then unwind.  Thread 0
+// makes this workgroup's writes visible with ONE release fence, arrives with a
+// relaxed atomic, spins with relaxed loads (an acquire load in the spin would
+// invalidate the caches on every iteration) and acquires once at the end.
 __device__ bool gang_sync(GangSync* s, int G) {
   __shared__ int ok;
   __syncthreads();
@@ -102,14 +105,15 @@ __device__ bool gang_sync(GangSync* s, int G) {
     if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
       good = 0;
     } else {
-      const unsigned gen = __hip_atomic_load(&s->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned prev = __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned gen = __hip_atomic_load(&s->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      const unsigned prev = __hip_atomic_fetch_add(&s->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (prev + 1 == (unsigned)G) {
         __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&s->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         long spins = 0;
-        while (__hip_atomic_load(&s->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        while (__hip_atomic_load(&s->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
           if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             good = 0;
             break;
@@ -119,9 +123,10 @@ __device__ bool gang_sync(GangSync* s, int G) {
             good = 0;
             break;
           }
-          __builtin_amdgcn_s_sleep(2);
+          __builtin_amdgcn_s_sleep(1);
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     ok = good;
   }
@@ -180,6 +185,37 @@ __device__ void gang_multidot(const T* V, long ldv, int cnt, const T* w, long e0
   __syncthreads();
 }
 
+// C[M x N] = A[M x K] B[K x N] for M <= SKINNY_M rows: A staged in LDS
+// (M x K, K <= SKINNY_K), every thread owns whole output columns and keeps
+// the M accumulators in registers; B rows are read with unit column stride
+// (coalesced over the threads).  For the first matvec stage of a gang member
+// (M = P times its rows), where 64 x 64 tiles would be mostly padding.
+constexpr int SKINNY_M = 8, SKINNY_K = 256;
+template <class T>
+__device__ void wg_gemm_skinny(int M, int N, int K, CView<T> A, const T* B, int ldb, MView<T> C, T* lds) {
+  for (int e = threadIdx.x; e < M * K; e += NT) {
+    const int i = e / K, k = e - i * K;
+    lds[e] = A.at(i, k);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < N; j += NT) {
+    T acc[SKINNY_M];
+#pragma unroll
+    for (int i = 0; i < SKINNY_M; ++i) acc[i] = (T)0;
+    const T* bp = B + j;
+    for (int k = 0; k < K; ++k) {
+      const T bv = bp[(long)k * ldb];
+#pragma unroll
+      for (int i = 0; i < SKINNY_M; ++i)
+        if (i < M) acc[i] += lds[i * K + k] * bv;
+    }
+#pragma unroll
+    for (int i = 0; i < SKINNY_M; ++i)
+      if (i < M) C.at(i, j) = acc[i];
+  }
+  __syncthreads();
+}
+
 // --- split contractions -------------------------------------------------------------
 
 // y[a', m, b'] for a' in [r0, r1) of the local matvec (all three stages
@@ -190,8 +226,12 @@ __device__ void gang_matvec(int a, int P, int n, int Q, int b, const T* L, const
   const int s = r1 - r0;
   if (s <= 0) return;
   // T1[a'_l, b, P, n] = sum_a L[a, P, r0 + a'_l] v[a, n, b]
-  wg_gemm<T, T>(P * s, n * b, a, cview<T>(L + r0, split(s, a, 1), plain(P * a)), crm<T>(v, n * b),
-                mview<T>(t1, split(s, n, b * P * n), split(b, 1, P * n)), lds);
+  if (P * s <= SKINNY_M && a <= SKINNY_K && (long)P * s * a * sizeof(T) <= GEMM_LDS_DOUBLES * 8)
+    wg_gemm_skinny<T>(P * s, n * b, a, cview<T>(L + r0, split(s, a, 1), plain(P * a)), v, n * b,
+                      mview<T>(t1, split(s, n, b * P * n), split(b, 1, P * n)), lds);
+  else
+    wg_gemm<T, T>(P * s, n * b, a, cview<T>(L + r0, split(s, a, 1), plain(P * a)), crm<T>(v, n * b),
+                  mview<T>(t1, split(s, n, b * P * n), split(b, 1, P * n)), lds);
   // T2[a'_l, m, b, Q] = sum_{P,n} T1[(a'_l, b), (P, n)] A[P, m, n, Q]
   wg_gemm<T, T>(s * b, n * Q, P * n, crm<T>(t1, P * n), cview<T>(Ac, split(n, n * n * Q, Q), split(Q, n * Q, 1)),
                 mview<T>(t2, split(b, n * b * Q, Q), split(Q, b * Q, 1)), lds);
@@ -230,7 +270,12 @@ struct GangCtx {
   double* t2;
   double* lds;
   SolveLDS& S;
-  __device__ bool sync() { return gang_sync(D.sync, G); }
+  __device__ bool sync() {
+    const long long t0 = ph_now();
+    const bool r = gang_sync(D.sync, G);
+    ph_add(S, PH_OTHER, t0);  // barrier waits (profiling builds)
+    return r;
+  }
 };
 
 // Local operator of position i: FP64 operands in global memory.
@@ -297,7 +342,9 @@ __device__ int gang_cg(GangCtx& C, const GangLocal& o, const T* L, const T* A, c
   if (fn == 0.0 || sqrt(rr) <= tol * fn) return 0;
   while (*iters < maxit) {
     // q = M p (p complete: the barrier above / at the end of the last step)
+    const long long tm = ph_now();
     gang_matvec<T>(o.a, o.P, o.n, o.Q, o.b, L, A, R, p, q, o.r0, o.r1, t1, t2, lds);
+    ph_add(C.S, PH_MATVEC, tm);
     double pq = 0.0;
     for (long e = o.e0 + threadIdx.x; e < o.e1; e += NT) pq += (double)p[e] * (double)q[e];
     double u1 = 0.0, u2 = 0.0;
@@ -383,7 +430,9 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
     double res = beta;
     bool brk = false;
     for (int j = 0; j < m; ++j) {
+      const long long tm = ph_now();
       gang_matvec<T>(o.a, o.P, o.n, o.Q, o.b, L, A, R, V + (long)j * N, w, o.r0, o.r1, t1, t2, lds);
+      ph_add(S, PH_MATVEC, tm);
       double* col = S.H + j * (KMAX + 1);  // column j (rows 0..j+1), identical in every member
       for (int k = threadIdx.x; k <= j + 1; k += NT) col[k] = 0.0;
       double w0 = 0.0, hn = 0.0;
@@ -651,6 +700,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
   ++st.nsolve;
   // QR of the new core and the warm start of the next position (member 0)
   int status = ST_OK;
+  const long long tq = ph_now();
   if (C.g == 0) {
     const QRBuf B{D.qw, D.qq, D.rf, D.tau};
     const double nv = wg_nrm2(D.g, o.N, S.red);
@@ -679,6 +729,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
     __syncthreads();
     if (threadIdx.x == 0) D.part[0] = (double)status;
   }
+  ph_add(S, PH_QR, tq);
   if (!C.sync()) return ST_ABORT;
   status = (int)D.part[0];
   // every member has read the status before member 0 writes part again
@@ -694,6 +745,9 @@ __device__ void gang_als(GangCtx& C) {
   int status = ST_OK;
   int sweeps = 0;
   long nsolves = 0;
+  if (threadIdx.x < 8) S.prof[threadIdx.x] = 0;
+  __syncthreads();
+  const long long tstart = ph_now();
   // x <- x0 (split copy of every core)
   for (int i = 0; i < d; ++i) {
     const double* src = D.x0c[i];
@@ -742,7 +796,9 @@ __device__ void gang_als(GangCtx& C) {
     ok = C.sync();
     for (int i = 0; i < d && ok && status == ST_OK; ++i) {
       if (i > 0) {
+        const long long ts = ph_now();
         st.flops += gang_build_left(C, i);
+        ph_add(S, PH_STACK, ts);
         ok = C.sync();
         if (!ok) break;
       }
@@ -752,7 +808,9 @@ __device__ void gang_als(GangCtx& C) {
       }
     }
     for (int i = d - 1; i >= 0 && ok && status == ST_OK; --i) {
+      const long long ts = ph_now();
       if (i < d - 1) st.flops += gang_build_right(C, i, &ok);
+      ph_add(S, PH_STACK, ts);
       if (!ok) break;
       status = gang_step(C, i, false, sym, st);
       ++nsolves;
@@ -771,7 +829,10 @@ __device__ void gang_als(GangCtx& C) {
     D.out[6] = (double)st.maxit_one;
     D.out[7] = 0.0;
     D.out[8] = st.flops;
-    for (int k = 9; k < 16; ++k) D.out[k] = 0.0;
+    // profiling builds: member 0's cycles -- total, matvec, -, -, QR, stacks, -, barrier waits
+    D.out[9] = (double)(ph_now() - tstart);
+    for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
+    D.out[15] = (double)S.prof[PH_OTHER];
     for (int k = 0; k < 16; ++k) D.res[k] = D.out[k];
     for (int k = 0; k <= d; ++k) D.res[16 + k] = (double)D.rx[k];
   }

@@ -13,11 +13,15 @@ The initial guess is right-orthonormalised on the device before the right
 stacks are built; ALS is gauge invariant, so this changes nothing in exact
 arithmetic and keeps the stacks well scaled.
 """
+import ctypes
+
+import numpy as np
+
 from .. import _lib
 from ..tensor_train import TT, _stream
 from . import _engine
 
-__all__ = ['als', 'als_batch', 'als_gang']
+__all__ = ['als', 'mals', 'als_batch', 'als_gang']
 
 _SOLVERS = {'solve': 0, 'lu': 1}
 
@@ -92,6 +96,122 @@ def als_gang(operators, initial_guesses, right_hand_sides, repeats=1, solver='so
         _engine.raise_first(infos, 'sle.als', count)
     sols = _engine.Solutions(cores, opacks, initial_guesses)
     return (sols, infos) if return_info else sols
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def mals(operator, initial_guess, right_hand_side, repeats=1, solver='solve', threshold=1e-12, max_rank=np.inf):
+    """Modified ALS, two-site and rank-adaptive (scikit_tt/solvers/sle.py:98-191).
+
+    Same sweep as the reference: left stacks forward, right stacks
+    backward; the two-site problem at (i, i+1) (stacks L_i, R_{i+1} and the
+    supercore of operator cores i, i+1, sle.py:350-390, 433-472) is solved on
+    the device, the two-site solution split by a truncated SVD
+    (s/s0 > threshold, then max_rank, sle.py:589-645): forward keeps U in
+    core i, backward V^T in core i+1 and U S in core 0 at the end.  Ranks
+    grow and shrink with the truncation, so the sweep runs from the host,
+    piece by piece (stack updates, supercores, local rhs, local solve,
+    truncated SVD are device calls); local systems of size <= 96 use LU
+    with partial pivoting like the reference, larger ones CG/GMRES to
+    relative residual 1e-12 (as in ``als``).
+    """
+    import torch
+    if solver not in _SOLVERS:
+        raise NotImplementedError("solver must be 'solve' or 'lu', got %r" % (solver,))
+    _engine.check_tts([operator, initial_guess, right_hand_side])
+    from ..utils import truncated_svd
+    lib = _lib.load()
+    dev = operator.device
+    ctx = _lib.context(dev.index)
+    st = _stream(dev)
+    d = operator.order
+    n = list(operator.row_dims)
+    ra, rb = list(operator.ranks), list(right_hand_side.ranks)
+    A = [c.contiguous() for c in operator.device_cores]
+    B = [c.contiguous() for c in right_hand_side.device_cores]
+    ranks = list(initial_guess.ranks)
+    cores = [c.reshape(c.shape[0], c.shape[1], c.shape[3]).clone() for c in initial_guess.device_cores]
+    sym = all(bool(torch.equal(c, c.transpose(1, 2))) for c in A)
+    f64 = dict(dtype=torch.float64, device=dev)
+    Lop, Lb, Rop, Rb = [None] * d, [None] * d, [None] * d, [None] * d
+
+    def chk(rc, what):
+        _lib.check(rc, ctx, what)
+
+    def left(i):
+        if i == 0:
+            Lop[0], Lb[0] = torch.ones((1, 1, 1), **f64), torch.ones((1, 1), **f64)
+            return
+        j = i - 1
+        a, b = ranks[j], ranks[i]
+        Lop[i] = torch.empty((b, ra[i], b), **f64)
+        chk(lib.ttals_stack_left_op(ctx, st, a, ra[j], n[j], ra[i], b, _ptr(Lop[j]), _ptr(A[j]), _ptr(cores[j]),
+                                    _ptr(Lop[i])), 'stack_left_op')
+        Lb[i] = torch.empty((rb[i], b), **f64)
+        chk(lib.ttals_stack_left_rhs(ctx, st, a, rb[j], n[j], rb[i], b, _ptr(Lb[j]), _ptr(B[j]), _ptr(cores[j]),
+                                     _ptr(Lb[i])), 'stack_left_rhs')
+
+    def right(i):
+        if i == d - 1:
+            Rop[i], Rb[i] = torch.ones((1, 1, 1), **f64), torch.ones((1, 1), **f64)
+            return
+        j = i + 1
+        a, b = ranks[j], ranks[j + 1]
+        Rop[i] = torch.empty((a, ra[j], a), **f64)
+        chk(lib.ttals_stack_right_op(ctx, st, a, ra[j], n[j], ra[j + 1], b, _ptr(Rop[j]), _ptr(A[j]),
+                                     _ptr(cores[j]), _ptr(Rop[i])), 'stack_right_op')
+        Rb[i] = torch.empty((rb[j], a), **f64)
+        chk(lib.ttals_stack_right_rhs(ctx, st, a, rb[j], n[j], rb[j + 1], b, _ptr(Rb[j]), _ptr(B[j]),
+                                      _ptr(cores[j]), _ptr(Rb[i])), 'stack_right_rhs')
+
+    def merge(X, Y, r1, m1, n1, k, m2, n2, r2):
+        out = torch.empty((r1, m1 * m2, n1 * n2, r2), **f64)
+        rc = lib.ttals_core_merge(st, r1, m1, n1, k, m2, n2, r2, _ptr(X), _ptr(Y), _ptr(out))
+        if rc != 0:
+            raise RuntimeError('core_merge failed (%d)' % rc)
+        return out
+
+    def step(i, direction):
+        a, b = ranks[i], ranks[i + 2]
+        n1, n2 = n[i], n[i + 1]
+        nn = n1 * n2
+        N = a * nn * b
+        Asup = merge(A[i], A[i + 1], ra[i], n1, n1, ra[i + 1], n2, n2, ra[i + 2])
+        Bsup = merge(B[i], B[i + 1], rb[i], n1, 1, rb[i + 1], n2, 1, rb[i + 2])
+        f = torch.empty(N, **f64)
+        chk(lib.ttals_micro_rhs(ctx, st, a, rb[i], nn, rb[i + 2], b, _ptr(Lb[i]), _ptr(Bsup), _ptr(Rb[i + 1]),
+                                _ptr(f)), 'micro_rhs')
+        # warm start (matrix-free local solves): the current two-site contraction
+        v = torch.tensordot(cores[i], cores[i + 1], dims=([2], [0])).reshape(N).contiguous()
+        its = ctypes.c_int()
+        rc = lib.ttals_local_solve(ctx, st, a, ra[i], nn, ra[i + 2], b, _ptr(Lop[i]), _ptr(Asup), _ptr(Rop[i + 1]),
+                                   _ptr(f), _ptr(v), 0, 1 if sym else 0, 1e-12, 4000, ctypes.byref(its))
+        if rc == _lib.S_SINGULAR:
+            raise np.linalg.LinAlgError('Singular matrix (sle.mals)')
+        chk(rc if rc < 0 else 0, 'local_solve')
+        u, s, vt = truncated_svd(v.reshape(a * n1, n2 * b), threshold=threshold, max_rank=max_rank)
+        k = s.shape[0]
+        ranks[i + 1] = k
+        if direction == 'forward':
+            cores[i] = u.reshape(a, n1, k).contiguous()
+            cores[i + 1] = (s[:, None] * vt).reshape(k, n2, b).contiguous()
+        else:
+            cores[i + 1] = vt.reshape(k, n2, b).contiguous()
+            cores[i] = (u * s[None, :]).reshape(a, n1, k).contiguous()
+
+    for i in range(d - 1, 0, -1):
+        right(i)
+    for _ in range(repeats):
+        for i in range(d - 1):
+            left(i)
+            if i < d - 2:
+                step(i, 'forward')
+        for i in range(d - 2, -1, -1):
+            right(i + 1)
+            step(i, 'backward')
+    return TT._from_device([c.reshape(c.shape[0], c.shape[1], 1, c.shape[2]) for c in cores])
 
 
 def als(operator, initial_guess, right_hand_side, repeats=1, solver='solve'):

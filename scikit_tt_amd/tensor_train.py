@@ -123,10 +123,11 @@ class _CoreList:
 class TT:
     """Tensor train / TT operator with cores on the HIP device.
 
-    Constructor semantics follow scikit_tt/tensor_train.py:149-267 for a
-    list of cores: every element must be 4-D (ValueError), neighbouring
-    ranks must match (ValueError), anything other than a list raises
-    TypeError.  Full-tensor input (TT-SVD) is not part of this build.
+    Constructor semantics follow scikit_tt/tensor_train.py:149-267: for a
+    list of cores every element must be 4-D (ValueError) and neighbouring
+    ranks must match (ValueError); a full array of shape (m_1..m_d,
+    n_1..n_d) is decomposed by TT-SVD on the device (truncated SVDs with
+    ``threshold`` / ``max_rank``); anything else raises TypeError.
     """
 
     def __init__(self, x, threshold=0, max_rank=np.inf, progress=False, string=None):
@@ -145,7 +146,8 @@ class TT:
             if threshold != 0 or max_rank != np.inf:
                 self.ortho(threshold=threshold, max_rank=max_rank)
         elif isinstance(x, np.ndarray):
-            raise NotImplementedError('TT from a full array (TT-SVD) is not implemented in scikit_tt_amd')
+            t = _tt_svd(x, threshold, max_rank)
+            self.__dict__.update(t.__dict__)
         else:
             raise TypeError('Parameter must be either a list of cores or an ndarray.')
 
@@ -411,8 +413,59 @@ class TT:
         return TT._from_device([c.permute(0, 2, 1, 3).contiguous() for c in self.device_cores])
 
 
+    def tt2qtt(self, row_dims, col_dims, threshold=0):
+        """Split every core into a chain of smaller cores (QTT format,
+        tensor_train.py:1432-1516): core i with row/col dims m_i = prod
+        row_dims[i], n_i = prod col_dims[i] is split left to right by
+        truncated SVDs on the device (s/s0 > threshold when threshold != 0)."""
+        from .utils import truncated_svd
+        qtt = []
+        for i in range(self.order):
+            core = self.device_cores[i]
+            rank = self.ranks[i]
+            row_dim, col_dim = self.row_dims[i], self.col_dims[i]
+            r_right = self.ranks[i + 1]
+            for j in range(len(row_dims[i]) - 1):
+                row_dim //= row_dims[i][j]
+                col_dim //= col_dims[i][j]
+                c6 = core.reshape(rank, row_dims[i][j], row_dim, col_dims[i][j], col_dim, r_right)
+                c6 = c6.permute(0, 1, 3, 2, 4, 5).contiguous()
+                u, s, v = truncated_svd(c6.reshape(rank * row_dims[i][j] * col_dims[i][j], -1), threshold=threshold)
+                k = s.shape[0]
+                qtt.append(u.reshape(rank, row_dims[i][j], col_dims[i][j], k))
+                core = s[:, None] * v
+                rank = k
+            qtt.append(core.reshape(rank, row_dim, col_dim, r_right).contiguous())
+        return TT._from_device(qtt)
+
+
 # ---------------------------------------------------------------------------
 # helpers
+
+
+def _tt_svd(x, threshold, max_rank):
+    """TT decomposition of a full array (tensor_train.py:200-261): modes
+    interleaved (m_i, n_i), then split left to right by truncated SVDs on the
+    device; singular values pushed right."""
+    from .utils import truncated_svd
+    if x.ndim % 2:
+        raise ValueError('Number of dimensions must be a multiple of 2.')
+    if np.iscomplexobj(x):
+        raise NotImplementedError('complex tensor trains are not supported')
+    order = x.ndim // 2
+    row_dims, col_dims = x.shape[:order], x.shape[order:]
+    perm = [order * j + i for i in range(order) for j in range(2)]
+    y = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(device()).permute(perm).contiguous()
+    ranks = [1] * (order + 1)
+    cores = []
+    for i in range(order - 1):
+        y = y.reshape(ranks[i] * row_dims[i] * col_dims[i], -1)
+        u, s, v = truncated_svd(y, threshold=threshold, max_rank=max_rank)
+        ranks[i + 1] = s.shape[0]
+        cores.append(u.reshape(ranks[i], row_dims[i], col_dims[i], ranks[i + 1]).contiguous())
+        y = s[:, None] * v
+    cores.append(y.reshape(ranks[-2], row_dims[-1], col_dims[-1], 1).contiguous())
+    return TT._from_device(cores)
 
 
 def _check_ortho_args(order, start_index, end_index, threshold, max_rank):

@@ -20,6 +20,7 @@ __all__ = [
     'laplace_1d', 'laplace_operator', 'tfim_operator', 'tfim_ground_energy',
     'identity_operator', 'random_tt', 'ones_tt', 'shifted_operator', 'tt_sum',
     'laplace_small', 'tfim_chain', 'laplace_large', 'shifted_batch_instance', 'right_orthonormal',
+    'laplace_lowest_eigenvalue', 'serving_request',
 ]
 
 
@@ -209,3 +210,51 @@ def right_orthonormal(cores, normalize=True):
         out[0] = out[0] / np.linalg.norm(out[0])
     return out
 
+
+
+def laplace_lowest_eigenvalue(d, n):
+    """Lowest eigenvalue of the d-mode -Laplacian of laplace_operator: d times
+    the lowest eigenvalue 4 (n+1)^2 sin^2(pi / (2 (n+1))) of the 1-D factor
+    (a Kronecker sum's eigenvalues are sums of the factors')."""
+    return d * 4.0 * (n + 1) ** 2 * np.sin(np.pi / (2 * (n + 1))) ** 2
+
+
+def serving_request(k, d_range=(8, 64), n_choices=(2, 4, 8, 16), r_range=(8, 128)):
+    """Request k of the mixed serving workload (rng(k)): sle or evp with equal
+    probability, d ~ U{8..64}, n in {2, 4, 8, 16}, rank log-uniform in [8, 128].
+
+    sle: the -Laplacian with a rank-4 N(0,1) rhs (the large-Laplacian
+    configuration's form) or d*I + a rank-3 N(0, 0.1^2) perturbation with a
+    rank-4 rhs (the uneven-batch form), equally likely; x0 rank r N(0,1).
+    evp: the TFIM chain for n = 2, else the -Laplacian; x0 rank r N(0,1),
+    right-orthonormal, unit norm; the lowest eigenvalue ('eig', sigma=-100).
+    Returns a dict with 'kind', the host cores and the sizes.
+    """
+    rng = np.random.default_rng(k)
+    kind = 'sle' if rng.random() < 0.5 else 'evp'
+    d = int(rng.integers(d_range[0], d_range[1] + 1))
+    n = int(rng.choice(n_choices))
+    r = int(round(float(np.exp(rng.uniform(np.log(r_range[0]), np.log(r_range[1]))))))
+    out = {'kind': kind, 'd': d, 'n': n, 'rank': r}
+    if kind == 'sle':
+        if rng.random() < 0.5:
+            out['family'] = 'laplace'
+            out['A'] = laplace_operator(d, n)
+        else:
+            out['family'] = 'shifted'
+            out['A'] = shifted_operator(rng, d, n)
+        out['b'] = random_tt(rng, [n] * d, 4)
+        out['x0'] = random_tt(rng, [n] * d, r)
+    else:
+        if n == 2:
+            out['family'] = 'tfim'
+            J = rng.uniform(0.5, 1.5, d - 1)
+            g = rng.uniform(0.5, 1.5, d)
+            out['A'] = tfim_operator(J, g)
+            out['exact'] = tfim_ground_energy(J, g)
+        else:
+            out['family'] = 'laplace'
+            out['A'] = laplace_operator(d, n)
+            out['exact'] = laplace_lowest_eigenvalue(d, n)
+        out['x0'] = right_orthonormal(random_tt(rng, [n] * d, r))
+    return out

@@ -187,10 +187,66 @@ def power_case(ref_root):
     print('power', float(lam), W.tfim_ground_energy(J, g))
 
 
-CASES = {'base': main, 'c2': c2_instance, 'ode': ode_cases, 'power': power_case}
+def mals_case(ref_root):
+    """sle.mals on the reference's own test system (tests/test_sle.py:14-35,
+    61-85): the Toeplitz matrix toeplitz(1..1024) as a TT operator (TT-SVD of
+    the full array at threshold 1e-12), all-ones rhs, ones rank-5 start right-orthonormalised,
+    one sweep, threshold 1e-14, max_rank 10, for 'solve' and 'lu'.  Also the
+    TT-SVD of that array (operator cores and ranks) and tt2qtt of a small
+    train."""
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    import scipy.linalg as sla
+    from scikit_tt.tensor_train import TT
+    import scikit_tt.tensor_train as rtt
+    import scikit_tt.solvers.sle as rsle
+    d = 10
+    mat = sla.toeplitz(np.arange(1, 2 ** d + 1), np.arange(1, 2 ** d + 1)).astype(float)
+    # (truncated at 1e-12: the untruncated TT-SVD of the test keeps numerically zero singular
+    # values and ranks up to 2^d; the local problems of (M)ALS depend only on the operator itself)
+    op = TT(mat.reshape([2] * 2 * d), threshold=1e-12)
+    rhs = rtt.ones(op.row_dims, [1] * d)
+    x0 = rtt.ones(op.row_dims, [1] * d, ranks=5).ortho_right()
+    s = {}
+    _save_tt(s, 'A', op.cores)
+    s['A_ranks'] = np.array(op.ranks)
+    _save_tt(s, 'x0', x0.cores)
+    _save_tt(s, 'b', rhs.cores)
+    for solver in ('solve', 'lu'):
+        x = rsle.mals(op, x0, rhs, repeats=1, solver=solver, threshold=1e-14, max_rank=10)
+        _save_tt(s, 'x_' + solver, x.cores)
+        s['ranks_' + solver] = np.array(x.ranks)
+    s['dense_solution'] = np.linalg.solve(mat, np.ones(2 ** d))
+    np.savez_compressed(os.path.join(HERE, 'sle_mals_toeplitz_d10.npz'), **s)
+    # TT-SVD with truncation of a smaller full operator and tt2qtt of a train
+    rng = np.random.default_rng(9)
+    small = sla.toeplitz(np.arange(1, 65), np.arange(1, 65)).astype(float) + 0.01 * rng.standard_normal((64, 64))
+    t = {}
+    for thr, tag in ((0.0, 'exact'), (1e-10, 'thr')):
+        tt_ = TT(small.reshape([2] * 12), threshold=thr)
+        _save_tt(t, 'svd_' + tag, tt_.cores)
+        t['svd_ranks_' + tag] = np.array(tt_.ranks)
+    tt_ = TT(small.reshape([4] * 6), max_rank=5)
+    _save_tt(t, 'svd_maxr', tt_.cores)
+    t['svd_ranks_maxr'] = np.array(tt_.ranks)
+    t['full'] = small
+    base = TT([rng.standard_normal((1, 4, 4, 3)), rng.standard_normal((3, 8, 2, 2)), rng.standard_normal((2, 4, 4, 1))])
+    _save_tt(t, 'q_in', base.cores)
+    q = base.tt2qtt([[2, 2], [2, 2, 2], [4]], [[2, 2], [2, 1, 1], [2, 2]])
+    _save_tt(t, 'q_out', q.cores)
+    t['q_ranks'] = np.array(q.ranks)
+    qt = base.tt2qtt([[2, 2], [2, 2, 2], [4]], [[2, 2], [2, 1, 1], [2, 2]], threshold=1e-2)
+    _save_tt(t, 'q_out_thr', qt.cores)
+    t['q_ranks_thr'] = np.array(qt.ranks)
+    np.savez_compressed(os.path.join(HERE, 'tt_svd_qtt.npz'), **t)
+    print('mals ranks', list(s['ranks_solve']), list(s['ranks_lu']), 'svd ranks', list(t['svd_ranks_exact']),
+          list(t['svd_ranks_thr']), list(t['svd_ranks_maxr']), 'qtt', list(t['q_ranks']), list(t['q_ranks_thr']))
+
+
+CASES = {'base': main, 'c2': c2_instance, 'ode': ode_cases, 'power': power_case, 'mals': mals_case}
 
 if __name__ == '__main__':
-    # usage: make_golden.py [reference root] [case ...]   (cases: base, c2, ode, power; default: all)
+    # usage: make_golden.py [reference root] [case ...]   (cases: base, c2, ode, power, mals; default: all)
     root = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
     for name in (sys.argv[2:] or list(CASES)):
         CASES[name](root)
