@@ -1116,7 +1116,8 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   struct GOff {
     SysOff o;
     size_t w_r, w_p, w_q, w_Vg, w_xt, w_At, w_L32, w_A32, w_R32, w_r32, w_p32, w_q32, w_x32, w_V32, w_t1, w_t2,
-        w_part, w_sync, w_res;
+        w_part, w_sync, w_res, w_gram, w_qrs;
+    long rmax2;
     long tmax;
     int G;
   };
@@ -1154,6 +1155,9 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
     go.w_t1 = dd.take(8 * (size_t)G * z.tmax);
     go.w_t2 = dd.take(8 * (size_t)G * z.tmax);
     go.w_part = dd.take(8 * (size_t)G * GANG_LD);
+    go.rmax2 = z.rmax * z.rmax;
+    go.w_gram = dd.take(8 * (size_t)G * go.rmax2);
+    go.w_qrs = dd.take(8 * (size_t)G * 3 * go.rmax2);
     go.w_sync = dd.take(sizeof(GangSync));
     go.w_res = dd.take(8 * (16 + (size_t)d + 1));
   };
@@ -1281,6 +1285,9 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
     D.qq = wptr(o.w_qq);
     D.tau = wptr(o.w_tau);
     D.rf = wptr(o.w_rf);
+    D.gram = wptr(go.w_gram);
+    D.qrs = wptr(go.w_qrs);
+    D.rmax2 = go.rmax2;
     D.sync = reinterpret_cast<GangSync*>(dbase + go.w_sync);
     D.out = wptr(o.w_out);
     D.res = wptr(go.w_res);

@@ -75,10 +75,13 @@ struct GangDesc {
   double* t2;
   long tmax;
   double* part;  // G x GANG_LD partial sums
-  double* qw;    // workgroup 0: QR work (nmax), explicit Q (nmax), tau, R (rmax^2)
+  double* qw;    // QR panel (nmax, shared); workgroup 0's Householder fallback: qq (nmax), tau, R (rmax^2)
   double* qq;
   double* tau;
   double* rf;
+  double* gram;  // G x rmax^2 partial Gram matrices of the split QR
+  double* qrs;   // per workgroup: 3 x rmax^2 (Cholesky factor, accumulated R, product)
+  long rmax2;    // rmax^2
   GangSync* sync;
   double* out;   // [16] as SysDesc::out
   double* res;   // compact result row: out[0..15], then the d+1 final ranks
@@ -606,6 +609,98 @@ __device__ double gang_build_right(GangCtx& C, int i, bool* ok) {
   return fl_op(a, P, n, Q, b) + fl_rhs(a, p, n, q, b);
 }
 
+// Split QR of the new core's panel W (m x c, m >= c) over the gang by shifted
+// Cholesky QR applied three times (Fukaya, Kannan, Nakatsukasa, Yamamoto,
+// Yanagisawa: "Shifted Cholesky QR for computing the QR factorization of
+// ill-conditioned matrices", SIAM J. Sci. Comput. 42 (2020)): each pass forms
+// the Gram matrix from per-member partials (one barrier), every member
+// factors it (identical inputs and order: identical R), and solves its own
+// rows against R; the first pass shifts the Gram matrix by
+// 11 (m c + c (c + 1)) u ||W||_F^2 so that it stays positive definite for
+// condition numbers up to ~1/u.  The panel is copied from g into D.qw:
+// forward W[row, col] = g[row c + col] (m = a n, c = b), backward
+// W[row, col] = g[col m + row] (m = n b, c = a).  On success Q is in D.qw
+// (own rows) and R (c x c, upper) in this member's qrs + rmax2; returns 0, or
+// 1 when a pivot failed (every member alike), 2 non-finite, -1 aborted.
+__device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool transposed, double* fl) {
+  const GangDesc& D = C.D;
+  SolveLDS& S = C.S;
+  double* Wq = D.qw;
+  int m0, m1;
+  gang_span(m, C.G, C.g, m0, m1);
+  const int rows = m1 - m0;
+  double* Rp = D.qrs + (long)C.g * 3 * D.rmax2;  // this pass's factor
+  double* Rt = Rp + D.rmax2;                       // accumulated R
+  double* Tm = Rt + D.rmax2;                       // product temporary
+  const long cc = (long)c * c;
+  for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+    const long row = e / c, col = e - row * c;
+    Wq[e] = transposed ? g[col * m + row] : g[e];
+  }
+  for (long e = threadIdx.x; e < cc; e += NT) Rt[e] = (e / c == e % c) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int pass = 0; pass < 3; ++pass) {
+    // partial Gram of this member's rows
+    double* slot = D.gram + (long)C.g * D.rmax2;
+    wg_gemm<double, double>(c, c, rows, ctr<double>(Wq + (long)m0 * c, c), crm<double>(Wq + (long)m0 * c, c),
+                            mrm<double>(slot, c), C.lds);
+    if (!C.sync()) return -1;
+    // total Gram into Rp (every member, same order)
+    for (long e = threadIdx.x; e < cc; e += NT) {
+      double sg = 0.0;
+      for (int h = 0; h < C.G; ++h) sg += D.gram[(long)h * D.rmax2 + e];
+      Rp[e] = sg;
+    }
+    __syncthreads();
+    if (pass == 0) {
+      double tr = 0.0;
+      for (int j = 0; j < c; ++j) tr += Rp[(long)j * c + j];
+      if (!isfinite(tr)) return 2;
+      const double shift = 11.0 * ((double)m * c + (double)c * (c + 1)) * 0x1p-53 * tr;
+      __syncthreads();
+      for (int j = threadIdx.x; j < c; j += NT) Rp[(long)j * c + j] += shift;
+      __syncthreads();
+    }
+    // Cholesky Gram = R^T R (upper R in place, right-looking)
+    for (int j = 0; j < c; ++j) {
+      const double piv = Rp[(long)j * c + j];
+      if (!(piv > 0.0) || !isfinite(piv)) return 1;
+      const double rjj = sqrt(piv), inv = 1.0 / rjj;
+      __syncthreads();
+      for (int q = j + 1 + threadIdx.x; q < c; q += NT) Rp[(long)j * c + q] *= inv;
+      if (threadIdx.x == 0) Rp[(long)j * c + j] = rjj;
+      __syncthreads();
+      const int rest = c - j - 1;
+      for (int e = threadIdx.x; e < rest * rest; e += NT) {
+        const int p = j + 1 + e / rest, q = j + 1 + e % rest;
+        if (q >= p) Rp[(long)p * c + q] -= Rp[(long)j * c + p] * Rp[(long)j * c + q];
+      }
+      __syncthreads();
+    }
+    // own rows: W <- W R^{-1} (forward substitution per row)
+    for (int row = m0 + threadIdx.x; row < m1; row += NT) {
+      double* w = Wq + (long)row * c;
+      for (int j = 0; j < c; ++j) {
+        double t = w[j];
+        for (int q = 0; q < j; ++q) t -= w[q] * Rp[(long)q * c + j];
+        w[j] = t / Rp[(long)j * c + j];
+      }
+    }
+    // R_total <- R_pass R_total (upper triangular)
+    for (long e = threadIdx.x; e < cc; e += NT) {
+      const int p = (int)(e / c), q = (int)(e % c);
+      double t = 0.0;
+      for (int k2 = p; k2 <= q; ++k2) t += Rp[(long)p * c + k2] * Rt[(long)k2 * c + q];
+      Tm[e] = (q >= p) ? t : 0.0;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < cc; e += NT) Rt[e] = Tm[e];
+    __syncthreads();
+  }
+  *fl += 3.0 * (4.0 * (double)m * c * c + (double)c * c * c);
+  return 0;
+}
+
 // One local step at position i: local rhs, the split Krylov solve (FP64, or
 // FP32 with FP64 refinement), then on member 0 the QR of the new core, the
 // core store and the warm start of the next position.  Returns a Status.
@@ -698,9 +793,56 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
   st.kit += its;
   st.maxit_one = its > st.maxit_one ? its : st.maxit_one;
   ++st.nsolve;
-  // QR of the new core and the warm start of the next position (member 0)
+  // QR of the new core and the warm start of the next position: split
+  // Cholesky QR over the gang for tall panels, else (or when a pivot fails)
+  // Householder QR on member 0
   int status = ST_OK;
   const long long tq = ph_now();
+  if (i > 0 || forward) {
+    const int m = forward ? a * n : n * b, c = forward ? b : a;
+    if (m >= c && (long)c * c <= D.rmax2) {
+      double qfl = 0.0;
+      const int q = gang_cholqr(C, D.g, m, c, !forward, &qfl);
+      if (q < 0) return ST_ABORT;
+      if (q == 2) status = ST_NONFINITE;
+      if (q == 0) {
+        const double* Rt = D.qrs + (long)C.g * 3 * D.rmax2 + D.rmax2;
+        int m0, m1;
+        gang_span(m, C.G, C.g, m0, m1);
+        // Q (own rows) into the core
+        for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+          const long row = e / c, col = e - row * c;
+          if (forward) D.xc[i][e] = D.qw[e];
+          else D.xc[i][col * m + row] = D.qw[e];
+        }
+        if (!C.sync()) return ST_ABORT;  // every member done with g (its panel copy) before g is overwritten
+        if (forward) {
+          // g[(k), (n1 c1)] = R core_{i+1}: split over the columns
+          const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
+          int j0, j1;
+          gang_span(nc, C.G, C.g, j0, j1);
+          if (j1 > j0)
+            wg_gemm<double, double>(c, j1 - j0, b, crm<double>(Rt, c), crm<double>(D.xc[i + 1] + j0, nc),
+                                    mrm<double>(D.g + j0, nc), C.lds);
+          st.flops += qfl + 2.0 * c * nc * b;
+        } else {
+          // g[(c0 n0), k] = core_{i-1} R^T: split over the rows
+          const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
+          int r0, r1;
+          gang_span(cn, C.G, C.g, r0, r1);
+          if (r1 > r0)
+            wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(Rt, c),
+                                    mrm<double>(D.g + (long)r0 * c, c), C.lds);
+          st.flops += qfl + 2.0 * cn * c * a;
+        }
+        ph_add(S, PH_QR, tq);
+        if (!C.sync()) return ST_ABORT;
+        return status;
+      }
+      // a pivot failed (or non-finite): every member got here alike; member 0 below
+      if (!C.sync()) return ST_ABORT;
+    }
+  }
   if (C.g == 0) {
     const QRBuf B{D.qw, D.qq, D.rf, D.tau};
     const double nv = wg_nrm2(D.g, o.N, S.red);
