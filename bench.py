@@ -132,12 +132,12 @@ def cpu_baseline(instances, sweeps, budget_s=25.0, nmax=1500):
 # ---------------------------------------------------------------------------
 # counters committed under profiles/
 
-def measured_traffic(tag):
-    """HBM bytes per k_als launch for workload `tag` from the committed
+def measured_traffic(kernel, tag):
+    """HBM bytes per launch of `kernel` for workload `tag` from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a
     known-size kernel), or None if no summary covers it."""
     import re
-    path = os.path.join(ROOT, 'profiles', 'k_als_%s_pmc.md' % tag)
+    path = os.path.join(ROOT, 'profiles', '%s_%s_pmc.md' % (kernel, tag))
     if not os.path.exists(path):
         return None, None
     m = re.search(r'^traffic_bytes_per_launch: (\d+)', open(path).read(), re.M)
@@ -243,6 +243,9 @@ def main():
                     help='systems timed one per call on the plain path, spread evenly over the batch '
                          '(0 = every system)')
     ap.add_argument('--tfim-batch', type=int, default=2048, help='secondary evp workload (0 = skip)')
+    ap.add_argument('--schedule', default='auto', choices=('auto', 'one'),
+                    help="sle batch schedule: 'auto' gangs for the systems that set the launch time (model fitted "
+                         "on this batch first), 'one' one workgroup per system")
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--serve-rates', default='100,1000', help='Poisson arrival rates (req/s) of the serving phase')
     ap.add_argument('--serve-seconds', type=float, default=1.0)
@@ -275,8 +278,16 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    tuning = None
+    if args.schedule == 'auto':
+        from scikit_tt_amd import tune
+        log('fitting the launch-time model on this batch')
+        tt0 = time.perf_counter()
+        tuning = tune.calibrate(As, Xs, Bs, repeats=args.sweeps)
+        tuning['seconds'] = round(time.perf_counter() - tt0, 2)
+        log('model: %s' % tuning['params'])
     for w in range(args.warmup):
-        sle.als_batch(As, Xs, Bs, repeats=args.sweeps)
+        sle.als_batch(As, Xs, Bs, repeats=args.sweeps, schedule=args.schedule)
         log('warmup %d done' % w)
     torch.cuda.synchronize()
     lat, kernel_ms, flops, sols, infos = [], [], 0.0, None, None
@@ -285,7 +296,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
-        sols, infos = sle.als_batch(As, Xs, Bs, repeats=args.sweeps, return_info=True)
+        sols, infos = sle.als_batch(As, Xs, Bs, repeats=args.sweeps, return_info=True, schedule=args.schedule)
         torch.cuda.synchronize()
         lat.append(1e3 * (time.perf_counter() - ts))
         # the engine kernel alone: events recorded by the library right around its launch
@@ -297,6 +308,15 @@ def main():
     arr = infos.array
     flops = float(arr['flops'].sum())
     status_ok = arr['status'] == 0
+    gangs = np.asarray(infos.gangs)
+    # the same batch on one workgroup per system, once (reported beside the schedule used)
+    one_ms = None
+    if args.schedule != 'one':
+        sle.als_batch(As, Xs, Bs, repeats=args.sweeps)
+        torch.cuda.synchronize()
+        _lib.check(_lib.load().ttals_last_launch_ms(ctx, ctypes.byref(ms)), ctx, 'last_launch_ms')
+        one_ms = ms.value
+        log('one workgroup per system: kernel %.1f ms' % one_ms)
 
     # the gate: FP64 rel.res of every system (untimed)
     res, bn = residual_norms(As, sols, Bs)
@@ -330,7 +350,8 @@ def main():
     log('plain path: %d systems in %.1f s, %d meet the gate' % (P, plain_time, plain_ok))
 
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    traffic, traffic_src = measured_traffic('uneven_sle_b%d_s%d' % (B, args.sweeps))
+    kernel = 'k_hybrid' if gangs.sum() > 0 else 'k_als'
+    traffic, traffic_src = measured_traffic(kernel, 'uneven_sle_b%d_s%d' % (B, args.sweeps))
     achieved = flops / avg_kernel_s / 1e12
     line = {
         'metric': 'TT-ALS solves/s at rel.res <=1e-8, batched vs plain; p50/p99 latency',
@@ -349,7 +370,13 @@ def main():
                                'n in {4,8,16}, rank~U{4..48}; A = d*I + rank-3 N(0,0.1^2) TT perturbation; '
                                'rank-4 N(0,1) rhs; rank-r N(0,1) start), %d sweeps, one launch; only systems at '
                                'FP64 rel.res <= 1e-8 counted' % (B, args.sweeps),
-                   'batch': B, 'sweeps': args.sweeps, 'gate': GATE, 'parallelism': 'one workgroup per system'},
+                   'batch': B, 'sweeps': args.sweeps, 'gate': GATE,
+                   'parallelism': ('one launch: gangs of workgroups for the systems that set the launch time, one '
+                                   'workgroup for the rest' if args.schedule == 'auto' else 'one workgroup per system')},
+        'schedule': {'mode': args.schedule, 'gang_sizes': {int(k): int(v) for k, v in enumerate(np.bincount(gangs)) if v},
+                     'gang_workgroups': int(gangs.sum()), 'launch_ms': round(avg_kernel_s * 1e3, 3),
+                     'one_workgroup_per_system_launch_ms': None if one_ms is None else round(one_ms, 3),
+                     'model_fit': tuning},
         'systems_meeting_gate': n_ok,
         'systems_total': B,
         'meet_gate_by_mode_size': by_n,
@@ -368,7 +395,7 @@ def main():
         'latency_ms_p99': round(float(np.percentile(lat, 99)), 3),
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': traffic, 'traffic_source': traffic_src,
-                     'kernel': 'k_als<double>', 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
+                     'kernel': kernel, 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
     }
     if args.tfim_batch > 0:
         log('tfim evp batch %d' % args.tfim_batch)

@@ -117,8 +117,9 @@ int ttals_sle_als_batch_static(ttals_ctx* ctx, void* stream, int count, const tt
  * automatic, one per row of the widest bond up to 32) that split every
  * phase -- the local matvec over the row-side rank index, the interface
  * stacks over their first index, the Krylov vector operations over the
- * same rows -- and meet at gang barriers; all gangs are co-resident (one
- * cooperative launch).  precision 0: FP64 throughout; 1: every Krylov
+ * same rows -- and meet at gang barriers; all gangs fit the device at once
+ * and are launched together (one plain launch; workgroups are dispatched in
+ * index order).  precision 0: FP64 throughout; 1: every Krylov
  * iteration in FP32 on an FP32 copy of the local operator, refined with the
  * FP64 local residual f - M v until it meets `tol` (cores, stacks, QR and
  * the residual FP64).  Local problems are solved by CG (operator symmetric
@@ -172,6 +173,47 @@ int ttals_set_local_tol_policy(ttals_ctx* ctx, int policy);
  * the three-stage form.  Results agree to rounding.  No counterpart in the
  * reference: a tuning and test knob. */
 int ttals_set_matvec_variant(ttals_ctx* ctx, int variant);
+
+/* Schedule of later sle batches (ttals_sle_als_batch / ttals_sle_submit) on
+ * this context.  0 (default): one workgroup per system (k_als, two systems
+ * per CU).  1: one mixed launch (k_hybrid) -- the systems the launch-time
+ * model says would set the batch's time are solved by gangs of 2..32
+ * workgroups (as ttals_sle_als_gang, precision `gang_precision`), the rest
+ * by one workgroup each, largest first; when no gang shortens the modelled
+ * time the batch runs as under 0.  Gang-bearing launches on one device are
+ * ordered one after another (an event dependency, no host wait).  evp
+ * batches always use 0.  No counterpart in the reference: a scheduling knob
+ * of the batched engine. */
+int ttals_set_schedule(ttals_ctx* ctx, int schedule, int gang_precision);
+
+/* Launch-time model of the mixed schedule, `n` = 9 doubles: ms per on-chip
+ * and per off-chip matvec flop of one sweep, ms per local solve, then the
+ * time of a gang of 1, 2, 4, 8, 16, 32 workgroups relative to one workgroup
+ * (see ttals_sle_cost).  Set fits it to the device (scikit_tt_amd.tune);
+ * get reads the current one.  No counterpart in the reference. */
+int ttals_set_cost_model(ttals_ctx* ctx, const double* params, int n);
+int ttals_get_cost_model(ttals_ctx* ctx, double* params, int n);
+
+/* Static cost of `count` sle systems, 3 doubles each: the flops of the
+ * three-stage local matvec summed over the positions of one sweep (forward
+ * and back) whose local problem fits the on-chip arena, the same for those
+ * that do not, and the local solves per sweep.  Host arithmetic only (ctx
+ * may be NULL: the default arena and matvec form).  No counterpart in the
+ * reference. */
+int ttals_sle_cost(ttals_ctx* ctx, int count, const ttals_tt* A, const ttals_tt* x0, double* out);
+
+/* The split the mixed schedule would make of `count` sle systems on `slots`
+ * resident workgroups (the device's k_hybrid capacity; 256 on MI355X): gang
+ * size per system (0: one workgroup) and the modelled launch time.  Host
+ * arithmetic only (ctx may be NULL: the default model).  No counterpart in
+ * the reference. */
+int ttals_sle_plan(ttals_ctx* ctx, int count, const ttals_tt* A, const ttals_tt* x0, int repeats, int slots,
+                   int* gangs, double* makespan_ms);
+
+/* Gang size the last sle batch on this context gave each of its `count`
+ * systems (0: one workgroup).  No counterpart in the reference: a
+ * diagnostic of the mixed schedule. */
+int ttals_last_plan(ttals_ctx* ctx, int* gangs, int count);
 
 /* Device time (ms) of the engine kernel of the context's last finished
  * submission, from HIP events recorded on its stream immediately before and

@@ -4,6 +4,7 @@ The shared library is built in-tree (``python -m scikit_tt_amd.build``) and
 loaded from this package directory.  There is no fallback: if the library or
 a HIP device is missing, every compute call raises.
 """
+import contextlib
 import ctypes
 import os
 import threading
@@ -62,6 +63,12 @@ SIGNATURES = {
     'ttals_set_arena_kb': (_i, [_vp, _i]),
     'ttals_set_local_tol_policy': (_i, [_vp, _i]),
     'ttals_set_matvec_variant': (_i, [_vp, _i]),
+    'ttals_set_schedule': (_i, [_vp, _i, _i]),
+    'ttals_set_cost_model': (_i, [_vp, _vp, _i]),
+    'ttals_get_cost_model': (_i, [_vp, _vp, _i]),
+    'ttals_sle_cost': (_i, [_vp, _i, _TTp, _TTp, _vp]),
+    'ttals_last_plan': (_i, [_vp, _vp, _i]),
+    'ttals_sle_plan': (_i, [_vp, _i, _TTp, _TTp, _i, _i, _vp, _vp]),
     'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_left_rhs': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
@@ -118,6 +125,18 @@ def context(device):
             c = h
             _ctx[device] = c
     return c
+
+
+@contextlib.contextmanager
+def schedule(ctx, mode, gang_precision=0):
+    """The context's sle batch schedule set to ``mode`` for the block, then
+    back to one workgroup per system (the contexts are shared per device)."""
+    lib = load()
+    check(lib.ttals_set_schedule(ctx, int(mode), int(gang_precision)), ctx, 'set_schedule')
+    try:
+        yield
+    finally:
+        lib.ttals_set_schedule(ctx, 0, 0)
 
 
 def new_context(device):

@@ -5,7 +5,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, 'csrc', 'ttals_capi.hip')
+SRCS = [os.path.join(HERE, 'csrc', f) for f in ('ttals_capi.hip', 'ttals_hybrid.hip')]
 OUT = os.path.join(HERE, 'libttals.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
@@ -35,7 +35,7 @@ def build(verbose=False, out=OUT, defines=()):
     """Compile libttals.so; ``defines`` adds -D options (e.g. TTALS_PROFILE=1
     for a build whose results carry per-phase cycle counters)."""
     tmp = out + '.tmp'
-    cmd = [HIPCC] + FLAGS + ['-D' + d for d in defines] + ['-Rpass-analysis=kernel-resource-usage', '-o', tmp, SRC]
+    cmd = [HIPCC] + FLAGS + ['-D' + d for d in defines] + ['-Rpass-analysis=kernel-resource-usage', '-o', tmp] + SRCS
     if verbose:
         print(' '.join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -12,6 +13,7 @@
 #include "ttals_engine.hpp"
 #include "ttals_resid.hpp"
 #include "ttals_gang.hpp"
+#include "ttals_hybrid.hpp"
 #include "../../include/ttals.h"
 
 using namespace ttals;
@@ -21,30 +23,17 @@ using namespace ttals;
 // ---------------------------------------------------------------------------
 // kernels
 
-// One workgroup per system.  Dynamic LDS (`arena` doubles, at least the
-// GEMM staging tiles) holds either the staging tiles or the on-chip copy of
-// a local problem.
-template <class ACC>
 #ifndef TTALS_ALS_WAVES_PER_EU
 #define TTALS_ALS_WAVES_PER_EU 2  // two resident systems per CU
 #endif
+
+// One workgroup per system (als_block).
+template <class ACC>
 __global__ __launch_bounds__(NT, TTALS_ALS_WAVES_PER_EU) void k_als(const SysDesc* descs, long arena) {
+  static_assert(sizeof(ACC) == 8, "the engine computes in FP64");
   extern __shared__ double dyn_lds[];
   __shared__ SolveLDS S;
-  const SysDesc& D = descs[blockIdx.x];
-  // x <- x0: the engine works in place on the output cores
-  for (int i = 0; i < D.d; ++i) {
-    const double* src = D.x0c[i];
-    double* dst = D.xc[i];
-    if (src == dst) continue;
-    const long len = (long)D.rx[i] * D.n[i] * D.rx[i + 1];
-    for (long e = threadIdx.x; e < len; e += NT) dst[e] = src[e];
-  }
-  __syncthreads();
-  als_solve<ACC>(D, reinterpret_cast<ACC*>(dyn_lds), arena, S);
-  __syncthreads();
-  // one compact row per system, so the host fetches every result with one copy
-  for (int k = threadIdx.x; k < 17 + D.d; k += NT) D.res[k] = k < 16 ? D.out[k] : (double)D.rx[k - 16];
+  als_block(descs[blockIdx.x], dyn_lds, arena, S);
 }
 
 struct PieceArgs {
@@ -533,6 +522,21 @@ struct Pending {
   int* ranks;          // caller's output rank array
 };
 
+// Launch-time model of one sle system, used to split a batch between gangs
+// and single workgroups.  A system's cost is the flops of its three-stage
+// local matvec summed over the positions of one sweep (forward and back),
+// split by whether the local problem fits the on-chip arena:
+//   t1 = repeats * (on * F_on + off * F_off + solve * local_solves)   [ms]
+// for one workgroup, and repeats * (off * (F_on + F_off) * gang[k] + solve *
+// local_solves) for a gang of 2^k workgroups: a gang keeps its operands in
+// HBM/L2 and splits the flops, not the per-solve latency.
+// The defaults are MI355X measurements; the tuner (scikit_tt_amd.tune)
+// refits them on the device.
+struct CostModel {
+  double on = 1.0e-7, off = 3.4e-7, solve = 0.1;
+  double gang[6] = {1.0, 0.65, 0.40, 0.36, 0.32, 0.32};
+};
+
 struct ttals_ctx {
   int device = 0;
   void* dws = nullptr;
@@ -551,6 +555,10 @@ struct ttals_ctx {
   long arena = 61 * 1024 / 8;
   int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
   int matvec_variant = 0;    // SysDesc::mvar
+  int schedule = 0;          // sle batches: 0 one workgroup per system, 1 gangs for the systems that set the launch time
+  int gang_prec = 0;         // GangDesc::prec of the gangs of a mixed schedule
+  CostModel cost;
+  std::vector<int> last_plan;  // gang size per system of the last sle batch (0: one workgroup)
   std::string err;
 };
 
@@ -805,18 +813,125 @@ void take_offsets(int kind, const SysSizes& z, Layout& hl, Layout& dl, SysOff& o
   o.w_sym = dl.take(kind == 1 ? 8 * 4 * (size_t)z.ramax * z.ramax : 8);
 }
 
+// Host metadata of system s at the offsets `o`: dims, ranks, core pointers
+// and stack offsets (metadata lives at hbase on the host and dmeta on the
+// device, the workspace at dbase).
+void fill_meta(int kind, const ttals_tt& A, const ttals_tt& x0, const ttals_tt* b, const ttals_tt& xo,
+               const SysOff& o, char* hbase, char* dbase) {
+  auto hp = [&](size_t off) { return reinterpret_cast<void*>(hbase + off); };
+  const int d = A.order;
+  int* hn = (int*)hp(o.m_n);
+  int* hrx = (int*)hp(o.m_rx);
+  int* hra = (int*)hp(o.m_ra);
+  int* hrb = (int*)hp(o.m_rb);
+  int* hbrx = (int*)hp(o.m_brx);
+  for (int i = 0; i < d; ++i) hn[i] = A.row_dims[i];
+  for (int i = 0; i <= d; ++i) {
+    hrx[i] = hbrx[i] = x0.ranks[i];
+    hra[i] = A.ranks[i];
+    hrb[i] = b ? b->ranks[i] : 1;
+  }
+  double** xcp = (double**)hp(o.m_xc);
+  const double** acp = (const double**)hp(o.m_ac);
+  const double** bcp = (const double**)hp(o.m_bc);
+  double** bestp = (double**)hp(o.m_bestc);
+  const double** x0p = (const double**)hp(o.m_x0c);
+  long* Lo = (long*)hp(o.m_Loff);
+  long* Ro = (long*)hp(o.m_Roff);
+  long* Lbo = (long*)hp(o.m_Lboff);
+  long* Rbo = (long*)hp(o.m_Rboff);
+  double* best = reinterpret_cast<double*>(dbase + o.w_best);
+  size_t boff = 0;
+  long lo = 0, ro = 0, lbo = 0, rbo = 0;
+  for (int i = 0; i < d; ++i) {
+    const long a = x0.ranks[i], bb = x0.ranks[i + 1], n = A.row_dims[i];
+    x0p[i] = x0.cores[i];
+    xcp[i] = xo.cores[i];
+    acp[i] = A.cores[i];
+    bcp[i] = b ? b->cores[i] : nullptr;
+    bestp[i] = best + (kind == 1 ? boff : 0);
+    boff += (size_t)(a * n * bb);
+    Lo[i] = lo;
+    lo += a * A.ranks[i] * a;
+    Ro[i] = ro;
+    ro += bb * A.ranks[i + 1] * bb;
+    Lbo[i] = lbo;
+    lbo += (long)hrb[i] * a;
+    Rbo[i] = rbo;
+    rbo += (long)hrb[i + 1] * bb;
+  }
+}
+
+struct BatchArgs {
+  int kind, count, repeats, solver, maxit;
+  double tol, sigma, conv_eps;
+  const ttals_tt *A, *x0, *b, *xo;
+};
+
+// Metadata and the single-workgroup descriptor of system s.
+void fill_sys(ttals_ctx* c, const BatchArgs& B, int s, const SysOff& o, char* hbase, char* dmeta, char* dbase,
+              double* dres, SysDesc& D) {
+  const int kind = B.kind;
+  const ttals_tt& A = B.A[s];
+  const ttals_tt* b = kind == 0 ? &B.b[s] : nullptr;
+  fill_meta(kind, A, B.x0[s], b, B.xo[s], o, hbase, dbase);
+  auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
+  auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
+  std::memset(&D, 0, sizeof(D));
+  D.d = A.order;
+  D.kind = kind;
+  D.solver = B.solver;
+  D.repeats = B.repeats;
+  D.ortho_x0 = (kind == 0) ? 1 : 0;
+  D.sym = -1;
+  D.gm_restart = KMAX;
+  D.adapt = (kind == 1) ? c->tol_policy : 0;
+  D.mvar = c->matvec_variant;
+  D.maxit = B.maxit;
+  D.tol = B.tol;
+  D.sigma = B.sigma;
+  D.conv_eps = B.conv_eps;
+  D.n = (const int*)dptr(o.m_n);
+  D.rx = (int*)dptr(o.m_rx);
+  D.ra = (const int*)dptr(o.m_ra);
+  D.rb = (const int*)dptr(o.m_rb);
+  D.best_rx = (int*)dptr(o.m_brx);
+  D.xc = (double* const*)dptr(o.m_xc);
+  D.ac = (const double* const*)dptr(o.m_ac);
+  D.bc = (const double* const*)dptr(o.m_bc);
+  D.bestc = (double* const*)dptr(o.m_bestc);
+  D.Ls = wptr(o.w_L);
+  D.Rs = wptr(o.w_R);
+  D.Lbs = wptr(o.w_Lb);
+  D.Rbs = wptr(o.w_Rb);
+  D.Loff = (const long*)dptr(o.m_Loff);
+  D.Roff = (const long*)dptr(o.m_Roff);
+  D.Lboff = (const long*)dptr(o.m_Lboff);
+  D.Rboff = (const long*)dptr(o.m_Rboff);
+  D.f = wptr(o.w_f);
+  D.g = wptr(o.w_g);
+  D.t1 = wptr(o.w_t1);
+  D.t2 = wptr(o.w_t2);
+  D.V = wptr(o.w_V);
+  D.dense = wptr(o.w_dense);
+  D.qw = wptr(o.w_qw);
+  D.qq = wptr(o.w_qq);
+  D.tau = wptr(o.w_tau);
+  D.rf = wptr(o.w_rf);
+  D.out = wptr(o.w_out);
+  D.x0c = (const double* const*)dptr(o.m_x0c);
+  D.res = dres;
+  D.trace = (c->trace && s == 0) ? c->trace : nullptr;
+  D.trace_cap = c->trace_cap;
+  D.symw = wptr(o.w_sym);
+}
+
 // Where a batch's metadata, workspace and results live.
 struct Placement {
   char* host;       // staging: metadata, then the compact results
   char* dev;        // workspace, then the metadata mirror
   size_t host_cap;  // 0: the context's grow-only buffers (allocated here)
   size_t dev_cap;
-};
-
-struct BatchArgs {
-  int kind, count, repeats, solver, maxit;
-  double tol, sigma, conv_eps;
-  const ttals_tt *A, *x0, *b, *xo;
 };
 
 // Plans and enqueues one k_als launch for a batch (descriptors, one metadata
@@ -869,105 +984,14 @@ int enqueue_batch(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, Placemen
   char* dmeta = dbase + dl.bytes;  // metadata mirrored after the workspace
   auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
   auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
-  auto hp = [&](size_t off) { return reinterpret_cast<void*>(hbase + off); };
   SysDesc* hdesc = reinterpret_cast<SysDesc*>(hbase + m_descs);
   // pass 2: the same offsets again, descriptors written
   Layout hl2, dl2;
   (void)hl2.take(sizeof(SysDesc) * count);
   for (int s = 0; s < count; ++s) {
-    const ttals_tt& A = B.A[s];
-    const ttals_tt& x0 = B.x0[s];
-    const ttals_tt* b = kind == 0 ? &B.b[s] : nullptr;
-    (void)size_system(c, kind, A, x0, b, B.xo[s], s, z);
+    (void)size_system(c, kind, B.A[s], B.x0[s], kind == 0 ? &B.b[s] : nullptr, B.xo[s], s, z);
     take_offsets(kind, z, hl2, dl2, o);
-    const int d = z.d;
-    int* hn = (int*)hp(o.m_n);
-    int* hrx = (int*)hp(o.m_rx);
-    int* hra = (int*)hp(o.m_ra);
-    int* hrb = (int*)hp(o.m_rb);
-    int* hbrx = (int*)hp(o.m_brx);
-    for (int i = 0; i < d; ++i) hn[i] = A.row_dims[i];
-    for (int i = 0; i <= d; ++i) {
-      hrx[i] = hbrx[i] = x0.ranks[i];
-      hra[i] = A.ranks[i];
-      hrb[i] = b ? b->ranks[i] : 1;
-    }
-    double** xcp = (double**)hp(o.m_xc);
-    const double** acp = (const double**)hp(o.m_ac);
-    const double** bcp = (const double**)hp(o.m_bc);
-    double** bestp = (double**)hp(o.m_bestc);
-    const double** x0p = (const double**)hp(o.m_x0c);
-    long* Lo = (long*)hp(o.m_Loff);
-    long* Ro = (long*)hp(o.m_Roff);
-    long* Lbo = (long*)hp(o.m_Lboff);
-    long* Rbo = (long*)hp(o.m_Rboff);
-    size_t boff = 0;
-    long lo = 0, ro = 0, lbo = 0, rbo = 0;
-    for (int i = 0; i < d; ++i) {
-      const long a = x0.ranks[i], bb = x0.ranks[i + 1], n = A.row_dims[i];
-      x0p[i] = x0.cores[i];
-      xcp[i] = B.xo[s].cores[i];
-      acp[i] = A.cores[i];
-      bcp[i] = b ? b->cores[i] : nullptr;
-      bestp[i] = wptr(o.w_best) + (kind == 1 ? boff : 0);
-      boff += (size_t)(a * n * bb);
-      Lo[i] = lo;
-      lo += a * A.ranks[i] * a;
-      Ro[i] = ro;
-      ro += bb * A.ranks[i + 1] * bb;
-      Lbo[i] = lbo;
-      lbo += (long)hrb[i] * a;
-      Rbo[i] = rbo;
-      rbo += (long)hrb[i + 1] * bb;
-    }
-    SysDesc& D = hdesc[s];
-    std::memset(&D, 0, sizeof(D));
-    D.d = d;
-    D.kind = kind;
-    D.solver = B.solver;
-    D.repeats = B.repeats;
-    D.ortho_x0 = (kind == 0) ? 1 : 0;
-    D.sym = -1;
-    D.gm_restart = KMAX;
-    D.adapt = (kind == 1) ? c->tol_policy : 0;
-    D.mvar = c->matvec_variant;
-    D.maxit = B.maxit;
-    D.tol = B.tol;
-    D.sigma = B.sigma;
-    D.conv_eps = B.conv_eps;
-    D.n = (const int*)dptr(o.m_n);
-    D.rx = (int*)dptr(o.m_rx);
-    D.ra = (const int*)dptr(o.m_ra);
-    D.rb = (const int*)dptr(o.m_rb);
-    D.best_rx = (int*)dptr(o.m_brx);
-    D.xc = (double* const*)dptr(o.m_xc);
-    D.ac = (const double* const*)dptr(o.m_ac);
-    D.bc = (const double* const*)dptr(o.m_bc);
-    D.bestc = (double* const*)dptr(o.m_bestc);
-    D.Ls = wptr(o.w_L);
-    D.Rs = wptr(o.w_R);
-    D.Lbs = wptr(o.w_Lb);
-    D.Rbs = wptr(o.w_Rb);
-    D.Loff = (const long*)dptr(o.m_Loff);
-    D.Roff = (const long*)dptr(o.m_Roff);
-    D.Lboff = (const long*)dptr(o.m_Lboff);
-    D.Rboff = (const long*)dptr(o.m_Rboff);
-    D.f = wptr(o.w_f);
-    D.g = wptr(o.w_g);
-    D.t1 = wptr(o.w_t1);
-    D.t2 = wptr(o.w_t2);
-    D.V = wptr(o.w_V);
-    D.dense = wptr(o.w_dense);
-    D.qw = wptr(o.w_qw);
-    D.qq = wptr(o.w_qq);
-    D.tau = wptr(o.w_tau);
-    D.rf = wptr(o.w_rf);
-    D.out = wptr(o.w_out);
-    D.x0c = (const double* const*)dptr(o.m_x0c);
-    D.res = wptr(w_res) + res_stride * s;
-    D.trace = (c->trace && s == 0) ? c->trace : nullptr;
-    D.trace_cap = c->trace_cap;
-    D.symw = wptr(o.w_sym);
+    fill_sys(c, B, s, o, hbase, dmeta, dbase, reinterpret_cast<double*>(dbase + w_res) + res_stride * s, hdesc[s]);
   }
   HIPCHK(c, hipMemcpyAsync(dmeta, hbase, meta_bytes, hipMemcpyHostToDevice, stream));
   const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
@@ -1010,6 +1034,8 @@ int worst_status(int worst, int st) {
   return worst;
 }
 
+int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t* r_res, size_t* res_stride_out);
+
 // Builds descriptors for `count` systems and enqueues k_als (one workgroup
 // per system) and the result copy on the context's buffers; ttals_wait
 // (finish_batch) unpacks.
@@ -1020,7 +1046,9 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
   BatchArgs B{kind, count, repeats, solver, maxit, tol, sigma, conv_eps, A, x0, b, xo};
   Placement pl{nullptr, nullptr, 0, 0};
   size_t r_res = 0, stride = 0;
-  int rc = enqueue_batch(c, stream, B, pl, &r_res, &stride, true);
+  int rc = (kind == 0 && c->schedule == 1) ? enqueue_hybrid(c, stream, B, &r_res, &stride)
+                                           : enqueue_batch(c, stream, B, pl, &r_res, &stride, true);
+  if (!rc && !(kind == 0 && c->schedule == 1)) c->last_plan.assign(kind == 0 ? count : 0, 0);
   if (rc) return rc;
   if (!c->done) HIPCHK(c, hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
   HIPCHK(c, hipEventRecord(c->done, stream));
@@ -1038,7 +1066,9 @@ int finish_batch(ttals_ctx* c, ttals_result* res) {
   int worst = 0;
   for (size_t s = 0; s < c->pending.size(); ++s) {
     const Pending& pd = c->pending[s];
-    worst = worst_status(worst, unpack_result(hptr<double>(c, pd.r_res), pd.d, pd.ranks, res[s]));
+    const int st = unpack_result(hptr<double>(c, pd.r_res), pd.d, pd.ranks, res[s]);
+    if (st == ST_ABORT) return fail(c, TTALS_E_HIP, "gang barrier timed out (system " + std::to_string(s) + ")");
+    worst = worst_status(worst, st);
   }
   return worst;
 }
@@ -1072,10 +1102,11 @@ bool dims_ok(std::initializer_list<int> v) {
   return true;
 }
 
-// Workgroups that can be resident at once for the gang kernel (all CUs).
-int gang_capacity(ttals_ctx* c, int* cap) {
+// Workgroups of k_hybrid (the kernel that runs every gang) that can be
+// resident at once on the whole device with `dyn_lds` bytes of dynamic LDS.
+int gang_capacity(ttals_ctx* c, size_t dyn_lds, int* cap) {
   int nb = 0, cus = 0;
-  HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_gang), NT, 0));
+  HIPCHK(c, hybrid_occupancy(&nb, dyn_lds));
   HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   *cap = nb * cus;
   return 0;
@@ -1088,6 +1119,375 @@ int gang_auto_size(const ttals_tt& x0) {
   return std::max(1, std::min(32, r));
 }
 
+// Device-workspace offsets of one gang system beyond the single-workgroup ones.
+struct GOff {
+  SysOff o;
+  size_t w_r, w_p, w_q, w_Vg, w_xt, w_At, w_L32, w_A32, w_R32, w_r32, w_p32, w_q32, w_x32, w_V32, w_t1, w_t2,
+      w_part, w_gram, w_qrs;
+  long rmax2, tmax;
+};
+
+void gang_offsets(const SysSizes& z, const ttals_tt& A, const ttals_tt& x0, int G, bool mixed, Layout& h,
+                  Layout& dd, GOff& go) {
+  take_offsets(0, z, h, dd, go.o);
+  long kmax = 1, lmax = 1, rmax2 = 1;
+  for (int i = 0; i < z.d; ++i) {
+    const long a = x0.ranks[i], bb = x0.ranks[i + 1], n = A.row_dims[i];
+    kmax = std::max(kmax, (long)A.ranks[i] * n * n * A.ranks[i + 1]);
+    lmax = std::max(lmax, a * A.ranks[i] * a);
+    rmax2 = std::max(rmax2, bb * A.ranks[i + 1] * bb);
+  }
+  go.tmax = z.tmax;
+  const size_t nm = (size_t)z.nmax;
+  go.w_r = dd.take(8 * nm);
+  go.w_p = dd.take(8 * nm);
+  go.w_q = dd.take(8 * nm);
+  go.w_Vg = dd.take(8 * (size_t)(KMAX + 1) * nm);
+  go.w_xt = dd.take(8 * nm);
+  go.w_At = dd.take(8 * (size_t)kmax);
+  go.w_L32 = dd.take(mixed ? 4 * (size_t)lmax : 4);
+  go.w_A32 = dd.take(mixed ? 4 * (size_t)kmax : 4);
+  go.w_R32 = dd.take(mixed ? 4 * (size_t)rmax2 : 4);
+  go.w_r32 = dd.take(mixed ? 4 * nm : 4);
+  go.w_p32 = dd.take(mixed ? 4 * nm : 4);
+  go.w_q32 = dd.take(mixed ? 4 * nm : 4);
+  go.w_x32 = dd.take(mixed ? 4 * nm : 4);
+  go.w_V32 = dd.take(mixed ? 4 * (size_t)(KMAX + 1) * nm : 4);
+  go.w_t1 = dd.take(8 * (size_t)G * z.tmax);
+  go.w_t2 = dd.take(8 * (size_t)G * z.tmax);
+  go.w_part = dd.take(8 * (size_t)G * GANG_LD);
+  go.rmax2 = z.rmax * z.rmax;
+  go.w_gram = dd.take(8 * (size_t)G * go.rmax2);
+  go.w_qrs = dd.take(8 * (size_t)G * 3 * go.rmax2);
+}
+
+struct GangParams {
+  int solver, repeats, maxit, prec;
+  double tol;
+};
+
+// Metadata and the gang descriptor of one system (members first .. first+G-1).
+void gang_fill(const GOff& go, const ttals_tt& A, const ttals_tt& x0, const ttals_tt& b, const ttals_tt& xo, int G,
+               int first, const GangParams& P, char* hbase, char* dmeta, char* dbase, GangSync* sync, double* dres,
+               GangDesc& D) {
+  const SysOff& o = go.o;
+  fill_meta(0, A, x0, &b, xo, o, hbase, dbase);
+  auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
+  auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
+  auto fptr = [&](size_t off) { return reinterpret_cast<float*>(dbase + off); };
+  std::memset(&D, 0, sizeof(D));
+  D.d = A.order;
+  D.solver = P.solver;
+  D.repeats = P.repeats;
+  D.maxit = P.maxit;
+  D.prec = P.prec;
+  D.G = G;
+  D.first = first;
+  D.maxit_inner = P.maxit;
+  D.gm_restart = KMAX;
+  D.tol = P.tol;
+  D.tol_inner = 1e-5;
+  D.n = (const int*)dptr(o.m_n);
+  D.rx = (int*)dptr(o.m_rx);
+  D.ra = (const int*)dptr(o.m_ra);
+  D.rb = (const int*)dptr(o.m_rb);
+  D.xc = (double* const*)dptr(o.m_xc);
+  D.ac = (const double* const*)dptr(o.m_ac);
+  D.bc = (const double* const*)dptr(o.m_bc);
+  D.x0c = (const double* const*)dptr(o.m_x0c);
+  D.Ls = wptr(o.w_L);
+  D.Rs = wptr(o.w_R);
+  D.Lbs = wptr(o.w_Lb);
+  D.Rbs = wptr(o.w_Rb);
+  D.Loff = (const long*)dptr(o.m_Loff);
+  D.Roff = (const long*)dptr(o.m_Roff);
+  D.Lboff = (const long*)dptr(o.m_Lboff);
+  D.Rboff = (const long*)dptr(o.m_Rboff);
+  D.f = wptr(o.w_f);
+  D.g = wptr(o.w_g);
+  D.r = wptr(go.w_r);
+  D.p = wptr(go.w_p);
+  D.q = wptr(go.w_q);
+  D.V = wptr(go.w_Vg);
+  D.xt = wptr(go.w_xt);
+  D.At = wptr(go.w_At);
+  D.L32 = fptr(go.w_L32);
+  D.A32 = fptr(go.w_A32);
+  D.R32 = fptr(go.w_R32);
+  D.r32 = fptr(go.w_r32);
+  D.p32 = fptr(go.w_p32);
+  D.q32 = fptr(go.w_q32);
+  D.x32 = fptr(go.w_x32);
+  D.V32 = fptr(go.w_V32);
+  D.t1 = wptr(go.w_t1);
+  D.t2 = wptr(go.w_t2);
+  D.tmax = go.tmax;
+  D.part = wptr(go.w_part);
+  D.qw = wptr(o.w_qw);
+  D.qq = wptr(o.w_qq);
+  D.tau = wptr(o.w_tau);
+  D.rf = wptr(o.w_rf);
+  D.gram = wptr(go.w_gram);
+  D.qrs = wptr(go.w_qrs);
+  D.rmax2 = go.rmax2;
+  D.sync = sync;
+  D.out = wptr(o.w_out);
+  D.res = dres;
+}
+
+// Gang-bearing launches on one device run one at a time: two of them side by
+// side could each hold part of the CUs with gangs that wait for members the
+// other keeps out.  Each such launch waits on the previous one's event.
+std::mutex g_gang_mu;
+hipEvent_t g_gang_ev[TTALS_MAX_DEVICES] = {};
+
+int gang_serialize_begin(ttals_ctx* c, hipStream_t stream) {
+  if (c->device < 0 || c->device >= TTALS_MAX_DEVICES) return fail(c, TTALS_E_ARG, "device index out of range");
+  if (g_gang_ev[c->device]) HIPCHK(c, hipStreamWaitEvent(stream, g_gang_ev[c->device], 0));
+  return 0;
+}
+
+int gang_serialize_end(ttals_ctx* c, hipStream_t stream) {
+  hipEvent_t& ev = g_gang_ev[c->device];
+  if (!ev) HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIPCHK(c, hipEventRecord(ev, stream));
+  return 0;
+}
+
+// Static cost of one sle system (see CostModel): matvec flops over one sweep,
+// on-chip and off-chip, and the number of local solves per sweep.
+void sle_cost(const ttals_tt& A, const ttals_tt& x0, long arena, int mvar, double* on, double* off, int* nsolve) {
+  const int d = A.order;
+  double fo = 0.0, ff = 0.0;
+  for (int i = 0; i < d; ++i) {
+    const int a = x0.ranks[i], n = A.row_dims[i], b = x0.ranks[i + 1], P = A.ranks[i], Q = A.ranks[i + 1];
+    const double mv = 2.0 * ((double)a * P * a * n * b + (double)a * b * P * n * n * Q + (double)a * n * Q * b * b);
+    const double visits = (i == d - 1) ? 1.0 : 2.0;  // forward half-sweep 0..d-1, backward d-2..0
+    const long N = (long)a * n * b;
+    if (N <= NDENSE || arena_need_onchip(a, P, n, Q, b, 5, mvar) <= arena)
+      fo += visits * mv;
+    else
+      ff += visits * mv;
+  }
+  *on = fo;
+  *off = ff;
+  *nsolve = 2 * d - 1;
+}
+
+// Makespan of a split on `slots` resident workgroups: gang s holds G[s]
+// slots until tg[s], then single-workgroup systems take the first free slot
+// in the order given (largest first).
+double simulate_split(const std::vector<double>& t1, const std::vector<double>& tgang, const std::vector<int>& G,
+                      const std::vector<int>& order, int slots) {
+  std::vector<double> heap;
+  heap.reserve(slots);
+  double end = 0.0;
+  int used = 0;
+  for (size_t s = 0; s < G.size(); ++s)
+    if (G[s] > 0) {
+      for (int k = 0; k < G[s]; ++k) heap.push_back(tgang[s]);
+      used += G[s];
+      end = std::max(end, tgang[s]);
+    }
+  for (int k = used; k < slots; ++k) heap.push_back(0.0);
+  auto cmp = [](double x, double y) { return x > y; };
+  std::make_heap(heap.begin(), heap.end(), cmp);
+  for (int s : order) {
+    if (G[s] > 0) continue;
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    const double e = heap.back() + t1[s];
+    heap.back() = e;
+    std::push_heap(heap.begin(), heap.end(), cmp);
+    end = std::max(end, e);
+  }
+  return end;
+}
+
+// Gang sizes (0: one workgroup) that minimise the modelled makespan: for a
+// range of target times T, every system slower than T alone gets the
+// smallest power-of-two gang (at most gmax[s]) that brings it under T, where
+// a gang of 2^k takes tmv[s] * gang[k] + tov[s].
+double plan_split(const CostModel& cm, const std::vector<double>& t1, const std::vector<double>& tmv,
+                  const std::vector<double>& tov, const std::vector<int>& gmax, int slots, std::vector<int>& best) {
+  const int count = (int)t1.size();
+  std::vector<int> order(count);
+  for (int s = 0; s < count; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return t1[x] > t1[y]; });
+  best.assign(count, 0);
+  std::vector<double> tgang(count, 0.0);
+  double best_t = simulate_split(t1, tgang, best, order, slots);
+  const double tmax = t1[order[0]];
+  std::vector<int> G(count);
+  for (double T = tmax * 0.97; T > tmax * 0.02; T *= 0.95) {
+    std::fill(G.begin(), G.end(), 0);
+    int used = 0;
+    bool any = false;
+    for (int s : order) {
+      if (t1[s] <= T) break;
+      int pick = 0;
+      double tp = t1[s];
+      for (int k = 1; k <= 5 && (1 << k) <= gmax[s]; ++k) {
+        const double tk = tmv[s] * cm.gang[k] + tov[s];
+        if (tk < tp) {
+          pick = 1 << k;
+          tp = tk;
+        }
+        if (tk <= T) break;
+      }
+      if (pick == 0) continue;  // no gang is faster: stays a single workgroup
+      G[s] = pick;
+      tgang[s] = tp;
+      used += pick;
+      any = true;
+    }
+    if (!any || used > slots) continue;
+    const double m = simulate_split(t1, tgang, G, order, slots);
+    if (m < best_t) {
+      best_t = m;
+      best = G;
+    }
+  }
+  return best_t;
+}
+
+// Modelled one-workgroup times (t1), off-chip-priced times (tg1) and gang
+// limits of an sle batch, then the split; returns the modelled makespan.
+double plan_batch(const CostModel& cm, long arena, int mvar, int count, const ttals_tt* A, const ttals_tt* x0,
+                  int repeats, int slots, std::vector<int>& G, std::vector<double>& t1) {
+  std::vector<double> tmv(count), tov(count);
+  std::vector<int> gmax(count);
+  t1.assign(count, 0.0);
+  for (int s = 0; s < count; ++s) {
+    double on, off;
+    int ns;
+    sle_cost(A[s], x0[s], arena, mvar, &on, &off, &ns);
+    t1[s] = repeats * (cm.on * on + cm.off * off + cm.solve * ns);
+    tmv[s] = repeats * cm.off * (on + off);
+    tov[s] = repeats * cm.solve * ns;
+    gmax[s] = gang_auto_size(x0[s]);
+  }
+  return plan_split(cm, t1, tmv, tov, gmax, slots, G);
+}
+
+// One k_hybrid launch for an sle batch: gangs for the systems the cost model
+// says set the launch time, one workgroup each for the rest (largest first).
+// Falls back to the single-workgroup launch when no gang pays off.
+int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t* r_res, size_t* res_stride_out) {
+  const int count = B.count;
+  if (count < 1 || !B.A || !B.x0 || !B.xo || !B.b) return fail(c, TTALS_E_ARG, "bad batch arguments");
+  if (B.repeats < 0) return fail(c, TTALS_E_ARG, "repeats must be >= 0");
+  if (!(B.tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
+  if (B.maxit < 1) return fail(c, TTALS_E_ARG, "maxit must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
+  int per_cu = 0, cus = 0, rc;
+  HIPCHK(c, hybrid_occupancy(&per_cu, (size_t)arena * 8));
+  HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int slots = per_cu * cus;
+  if (slots < 1) return fail(c, TTALS_E_HIP, "k_hybrid cannot be resident on this device");
+  // pass 0: checks, then the split by the launch-time model
+  SysSizes z;
+  for (int s = 0; s < count; ++s)
+    if ((rc = size_system(c, 0, B.A[s], B.x0[s], &B.b[s], B.xo[s], s, z))) return rc;
+  std::vector<int> G;
+  std::vector<double> t1;
+  (void)plan_batch(c->cost, arena, c->matvec_variant, count, B.A, B.x0, B.repeats, slots, G, t1);
+  int ng = 0, wg = 0;
+  for (int s = 0; s < count; ++s)
+    if (G[s] > 0) {
+      ++ng;
+      wg += G[s];
+    }
+  if (ng == 0) {
+    Placement pl{nullptr, nullptr, 0, 0};
+    rc = enqueue_batch(c, stream, B, pl, r_res, res_stride_out, true);
+    if (!rc) c->last_plan = G;
+    return rc;
+  }
+  // gangs first, then single workgroups, each largest first
+  std::vector<int> gs, fs;
+  for (int s = 0; s < count; ++s) (G[s] > 0 ? gs : fs).push_back(s);
+  auto by_time = [&](int x, int y) { return t1[x] > t1[y]; };
+  std::stable_sort(gs.begin(), gs.end(), by_time);
+  std::stable_sort(fs.begin(), fs.end(), by_time);
+  const int nf = (int)fs.size();
+  // pass 1: layout
+  Layout hl, dl;
+  const size_t m_gdesc = hl.take(sizeof(GangDesc) * ng);
+  const size_t m_blocks = hl.take(sizeof(int) * wg);
+  const size_t m_sdesc = hl.take(sizeof(SysDesc) * std::max(nf, 1));
+  int dmax = 0;
+  for (int s = 0; s < count; ++s) dmax = std::max(dmax, B.A[s].order);
+  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t w_sync = dl.take(sizeof(GangSync) * ng);
+  const size_t w_res = dl.take(8 * res_stride * count);
+  GOff go;
+  SysOff o;
+  auto lay = [&](Layout& h, Layout& d, int s) {
+    (void)size_system(c, 0, B.A[s], B.x0[s], &B.b[s], B.xo[s], s, z);
+    if (G[s] > 0)
+      gang_offsets(z, B.A[s], B.x0[s], G[s], c->gang_prec == 1, h, d, go);
+    else
+      take_offsets(0, z, h, d, o);
+  };
+  for (int s : gs) lay(hl, dl, s);
+  for (int s : fs) lay(hl, dl, s);
+  const size_t meta_bytes = hl.bytes;
+  Layout rl;
+  rl.bytes = meta_bytes;
+  *r_res = rl.take(8 * res_stride * count);
+  *res_stride_out = res_stride;
+  if ((rc = ensure_hst(c, rl.bytes))) return rc;
+  if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
+  char* hbase = c->hst;
+  char* dbase = reinterpret_cast<char*>(c->dws);
+  char* dmeta = dbase + dl.bytes;
+  GangDesc* hg = reinterpret_cast<GangDesc*>(hbase + m_gdesc);
+  int* hblocks = reinterpret_cast<int*>(hbase + m_blocks);
+  SysDesc* hs = reinterpret_cast<SysDesc*>(hbase + m_sdesc);
+  GangSync* syncs = reinterpret_cast<GangSync*>(dbase + w_sync);
+  double* dres = reinterpret_cast<double*>(dbase + w_res);
+  // pass 2: the same offsets again, descriptors written
+  Layout hl2, dl2;
+  (void)hl2.take(sizeof(GangDesc) * ng);
+  (void)hl2.take(sizeof(int) * wg);
+  (void)hl2.take(sizeof(SysDesc) * std::max(nf, 1));
+  (void)dl2.take(sizeof(GangSync) * ng);
+  (void)dl2.take(8 * res_stride * count);
+  const GangParams P{B.solver, B.repeats, B.maxit, c->gang_prec, B.tol};
+  int first = 0;
+  for (int k = 0; k < ng; ++k) {
+    const int s = gs[k];
+    lay(hl2, dl2, s);
+    gang_fill(go, B.A[s], B.x0[s], B.b[s], B.xo[s], G[s], first, P, hbase, dmeta, dbase, syncs + k,
+              dres + res_stride * s, hg[k]);
+    for (int m = 0; m < G[s]; ++m) hblocks[first + m] = k;
+    first += G[s];
+  }
+  for (int k = 0; k < nf; ++k) {
+    const int s = fs[k];
+    lay(hl2, dl2, s);
+    fill_sys(c, B, s, o, hbase, dmeta, dbase, dres + res_stride * s, hs[k]);
+  }
+  HIPCHK(c, hipMemcpyAsync(dmeta, hbase, meta_bytes, hipMemcpyHostToDevice, stream));
+  HIPCHK(c, hipMemsetAsync(syncs, 0, sizeof(GangSync) * ng, stream));
+  if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
+  if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
+  {
+    std::lock_guard<std::mutex> lk(g_gang_mu);
+    if ((rc = gang_serialize_begin(c, stream))) return rc;
+    HIPCHK(c, hipEventRecord(c->k0, stream));
+    HIPCHK(c, hybrid_launch(wg + nf, (size_t)arena * 8, stream, (const GangDesc*)(dmeta + m_gdesc),
+                            (const int*)(dmeta + m_blocks), wg, (const SysDesc*)(dmeta + m_sdesc), arena));
+    HIPCHK(c, hipEventRecord(c->k1, stream));
+    if ((rc = gang_serialize_end(c, stream))) return rc;
+  }
+  c->timed = true;
+  HIPCHK(c, hipMemcpyAsync(hbase + *r_res, dres, 8 * res_stride * count, hipMemcpyDeviceToHost, stream));
+  c->last_plan = G;
+  return 0;
+}
+
 int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, const ttals_tt* x0, const ttals_tt* b,
                const ttals_tt* xo, int repeats, int solver, double tol, int maxit, int prec, int gang,
                ttals_result* res) {
@@ -1098,7 +1498,8 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   if (gang > 256) return fail(c, TTALS_E_ARG, "gang size must be <= 256");
   HIPCHK(c, hipSetDevice(c->device));
   int cap = 0, rc;
-  if ((rc = gang_capacity(c, &cap))) return rc;
+  const long arena = GEMM_LDS_DOUBLES;  // a gang stages its GEMM tiles only
+  if ((rc = gang_capacity(c, (size_t)arena * 8, &cap))) return rc;
   // pass 1: checks, gang sizes, totals
   Layout hl, dl;
   const size_t m_descs = hl.take(sizeof(GangDesc) * count);
@@ -1113,203 +1514,60 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
     return fail(c, TTALS_E_ARG, "the gangs need " + std::to_string(total) + " resident workgroups; the device holds " +
                                     std::to_string(cap));
   const size_t m_blocks = hl.take(sizeof(int) * total);
-  struct GOff {
-    SysOff o;
-    size_t w_r, w_p, w_q, w_Vg, w_xt, w_At, w_L32, w_A32, w_R32, w_r32, w_p32, w_q32, w_x32, w_V32, w_t1, w_t2,
-        w_part, w_sync, w_res, w_gram, w_qrs;
-    long rmax2;
-    long tmax;
-    int G;
-  };
-  auto plan = [&](int s, Layout& h, Layout& dd, GOff& go) {
+  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t w_sync = dl.take(sizeof(GangSync) * count);
+  const size_t w_res = dl.take(8 * res_stride * count);
+  GOff go;
+  for (int s = 0; s < count; ++s) {
     (void)size_system(c, 0, A[s], x0[s], &b[s], xo[s], s, z);
-    take_offsets(0, z, h, dd, go.o);
-    const int d = z.d;
-    long amax = 1, kmax = 1, lmax = 1, rmax2 = 1;
-    for (int i = 0; i < d; ++i) {
-      const long a = x0[s].ranks[i], bb = x0[s].ranks[i + 1], n = A[s].row_dims[i];
-      kmax = std::max(kmax, (long)A[s].ranks[i] * n * n * A[s].ranks[i + 1]);
-      lmax = std::max(lmax, a * A[s].ranks[i] * a);
-      rmax2 = std::max(rmax2, bb * A[s].ranks[i + 1] * bb);
-      amax = std::max(amax, a);
-    }
-    const int G = gang > 0 ? gang : gang_auto_size(x0[s]);
-    go.G = G;
-    go.tmax = z.tmax;
-    const size_t nm = (size_t)z.nmax;
-    go.w_r = dd.take(8 * nm);
-    go.w_p = dd.take(8 * nm);
-    go.w_q = dd.take(8 * nm);
-    go.w_Vg = dd.take(8 * (size_t)(KMAX + 1) * nm);
-    go.w_xt = dd.take(8 * nm);
-    go.w_At = dd.take(8 * (size_t)kmax);
-    const bool mixed = prec == 1;
-    go.w_L32 = dd.take(mixed ? 4 * (size_t)lmax : 4);
-    go.w_A32 = dd.take(mixed ? 4 * (size_t)kmax : 4);
-    go.w_R32 = dd.take(mixed ? 4 * (size_t)rmax2 : 4);
-    go.w_r32 = dd.take(mixed ? 4 * nm : 4);
-    go.w_p32 = dd.take(mixed ? 4 * nm : 4);
-    go.w_q32 = dd.take(mixed ? 4 * nm : 4);
-    go.w_x32 = dd.take(mixed ? 4 * nm : 4);
-    go.w_V32 = dd.take(mixed ? 4 * (size_t)(KMAX + 1) * nm : 4);
-    go.w_t1 = dd.take(8 * (size_t)G * z.tmax);
-    go.w_t2 = dd.take(8 * (size_t)G * z.tmax);
-    go.w_part = dd.take(8 * (size_t)G * GANG_LD);
-    go.rmax2 = z.rmax * z.rmax;
-    go.w_gram = dd.take(8 * (size_t)G * go.rmax2);
-    go.w_qrs = dd.take(8 * (size_t)G * 3 * go.rmax2);
-    go.w_sync = dd.take(sizeof(GangSync));
-    go.w_res = dd.take(8 * (16 + (size_t)d + 1));
-  };
-  {
-    Layout h1 = hl, d1 = dl;
-    GOff go;
-    for (int s = 0; s < count; ++s) plan(s, h1, d1, go);
-    hl = h1;
-    dl = d1;
+    gang_offsets(z, A[s], x0[s], gang > 0 ? gang : gang_auto_size(x0[s]), prec == 1, hl, dl, go);
   }
   const size_t meta_bytes = hl.bytes;
   Layout rl;
   rl.bytes = meta_bytes;
-  const size_t res_stride = 16 + (size_t)dmax + 1;
   const size_t r_res = rl.take(8 * res_stride * count);
   if ((rc = ensure_hst(c, rl.bytes))) return rc;
   if ((rc = ensure_dws(c, dl.bytes + meta_bytes))) return rc;
   char* hbase = c->hst;
   char* dbase = reinterpret_cast<char*>(c->dws);
   char* dmeta = dbase + dl.bytes;
-  auto dptr = [&](size_t off) { return reinterpret_cast<void*>(dmeta + off); };
-  auto wptr = [&](size_t off) { return reinterpret_cast<double*>(dbase + off); };
-  auto fptr = [&](size_t off) { return reinterpret_cast<float*>(dbase + off); };
-  auto hp = [&](size_t off) { return reinterpret_cast<void*>(hbase + off); };
   GangDesc* hdesc = reinterpret_cast<GangDesc*>(hbase + m_descs);
   int* hblocks = reinterpret_cast<int*>(hbase + m_blocks);
+  GangSync* syncs = reinterpret_cast<GangSync*>(dbase + w_sync);
+  double* dres = reinterpret_cast<double*>(dbase + w_res);
   // pass 2: descriptors (same offsets again)
   Layout hl2, dl2;
   (void)hl2.take(sizeof(GangDesc) * count);
   (void)hl2.take(sizeof(int) * total);
+  (void)dl2.take(sizeof(GangSync) * count);
+  (void)dl2.take(8 * res_stride * count);
+  const GangParams P{solver, repeats, maxit, prec, tol};
   int first = 0;
-  std::vector<size_t> res_off(count), sync_off(count);
   for (int s = 0; s < count; ++s) {
-    GOff go;
-    plan(s, hl2, dl2, go);
-    const SysOff& o = go.o;
-    const ttals_tt& Ai = A[s];
-    const ttals_tt& xi = x0[s];
-    const ttals_tt& bi = b[s];
-    const int d = Ai.order;
-    int* hn = (int*)hp(o.m_n);
-    int* hrx = (int*)hp(o.m_rx);
-    int* hra = (int*)hp(o.m_ra);
-    int* hrb = (int*)hp(o.m_rb);
-    for (int i = 0; i < d; ++i) hn[i] = Ai.row_dims[i];
-    for (int i = 0; i <= d; ++i) {
-      hrx[i] = xi.ranks[i];
-      hra[i] = Ai.ranks[i];
-      hrb[i] = bi.ranks[i];
-    }
-    double** xcp = (double**)hp(o.m_xc);
-    const double** acp = (const double**)hp(o.m_ac);
-    const double** bcp = (const double**)hp(o.m_bc);
-    const double** x0p = (const double**)hp(o.m_x0c);
-    long* Lo = (long*)hp(o.m_Loff);
-    long* Ro = (long*)hp(o.m_Roff);
-    long* Lbo = (long*)hp(o.m_Lboff);
-    long* Rbo = (long*)hp(o.m_Rboff);
-    long lo = 0, ro = 0, lbo = 0, rbo = 0;
-    for (int i = 0; i < d; ++i) {
-      const long a = xi.ranks[i], bb = xi.ranks[i + 1];
-      x0p[i] = xi.cores[i];
-      xcp[i] = xo[s].cores[i];
-      acp[i] = Ai.cores[i];
-      bcp[i] = bi.cores[i];
-      Lo[i] = lo;
-      lo += a * Ai.ranks[i] * a;
-      Ro[i] = ro;
-      ro += bb * Ai.ranks[i + 1] * bb;
-      Lbo[i] = lbo;
-      lbo += (long)hrb[i] * a;
-      Rbo[i] = rbo;
-      rbo += (long)hrb[i + 1] * bb;
-    }
-    GangDesc& D = hdesc[s];
-    std::memset(&D, 0, sizeof(D));
-    D.d = d;
-    D.solver = solver;
-    D.repeats = repeats;
-    D.maxit = maxit;
-    D.prec = prec;
-    D.G = go.G;
-    D.first = first;
-    D.maxit_inner = maxit;
-    D.gm_restart = KMAX;
-    D.tol = tol;
-    D.tol_inner = 1e-5;
-    D.n = (const int*)dptr(o.m_n);
-    D.rx = (int*)dptr(o.m_rx);
-    D.ra = (const int*)dptr(o.m_ra);
-    D.rb = (const int*)dptr(o.m_rb);
-    D.xc = (double* const*)dptr(o.m_xc);
-    D.ac = (const double* const*)dptr(o.m_ac);
-    D.bc = (const double* const*)dptr(o.m_bc);
-    D.x0c = (const double* const*)dptr(o.m_x0c);
-    D.Ls = wptr(o.w_L);
-    D.Rs = wptr(o.w_R);
-    D.Lbs = wptr(o.w_Lb);
-    D.Rbs = wptr(o.w_Rb);
-    D.Loff = (const long*)dptr(o.m_Loff);
-    D.Roff = (const long*)dptr(o.m_Roff);
-    D.Lboff = (const long*)dptr(o.m_Lboff);
-    D.Rboff = (const long*)dptr(o.m_Rboff);
-    D.f = wptr(o.w_f);
-    D.g = wptr(o.w_g);
-    D.r = wptr(go.w_r);
-    D.p = wptr(go.w_p);
-    D.q = wptr(go.w_q);
-    D.V = wptr(go.w_Vg);
-    D.xt = wptr(go.w_xt);
-    D.At = wptr(go.w_At);
-    D.L32 = fptr(go.w_L32);
-    D.A32 = fptr(go.w_A32);
-    D.R32 = fptr(go.w_R32);
-    D.r32 = fptr(go.w_r32);
-    D.p32 = fptr(go.w_p32);
-    D.q32 = fptr(go.w_q32);
-    D.x32 = fptr(go.w_x32);
-    D.V32 = fptr(go.w_V32);
-    D.t1 = wptr(go.w_t1);
-    D.t2 = wptr(go.w_t2);
-    D.tmax = go.tmax;
-    D.part = wptr(go.w_part);
-    D.qw = wptr(o.w_qw);
-    D.qq = wptr(o.w_qq);
-    D.tau = wptr(o.w_tau);
-    D.rf = wptr(o.w_rf);
-    D.gram = wptr(go.w_gram);
-    D.qrs = wptr(go.w_qrs);
-    D.rmax2 = go.rmax2;
-    D.sync = reinterpret_cast<GangSync*>(dbase + go.w_sync);
-    D.out = wptr(o.w_out);
-    D.res = wptr(go.w_res);
-    res_off[s] = go.w_res;
-    sync_off[s] = go.w_sync;
-    for (int k = 0; k < go.G; ++k) hblocks[first + k] = s;
-    first += go.G;
+    (void)size_system(c, 0, A[s], x0[s], &b[s], xo[s], s, z);
+    const int G = gang > 0 ? gang : gang_auto_size(x0[s]);
+    gang_offsets(z, A[s], x0[s], G, prec == 1, hl2, dl2, go);
+    gang_fill(go, A[s], x0[s], b[s], xo[s], G, first, P, hbase, dmeta, dbase, syncs + s, dres + res_stride * s,
+              hdesc[s]);
+    for (int k = 0; k < G; ++k) hblocks[first + k] = s;
+    first += G;
   }
   HIPCHK(c, hipMemcpyAsync(dmeta, hbase, meta_bytes, hipMemcpyHostToDevice, stream));
-  for (int s = 0; s < count; ++s) HIPCHK(c, hipMemsetAsync(dbase + sync_off[s], 0, sizeof(GangSync), stream));
+  HIPCHK(c, hipMemsetAsync(syncs, 0, sizeof(GangSync) * count, stream));
   if (!c->k0) HIPCHK(c, hipEventCreate(&c->k0));
   if (!c->k1) HIPCHK(c, hipEventCreate(&c->k1));
-  const GangDesc* ddesc = (const GangDesc*)dptr(m_descs);
-  const int* dblocks = (const int*)dptr(m_blocks);
-  void* args[] = {(void*)&ddesc, (void*)&dblocks};
-  HIPCHK(c, hipEventRecord(c->k0, stream));
-  HIPCHK(c, hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_gang), dim3(total), dim3(NT), args, 0, stream));
-  HIPCHK(c, hipEventRecord(c->k1, stream));
+  // every workgroup a gang member: k_hybrid with no single-workgroup part
+  {
+    std::lock_guard<std::mutex> lk(g_gang_mu);
+    if ((rc = gang_serialize_begin(c, stream))) return rc;
+    HIPCHK(c, hipEventRecord(c->k0, stream));
+    HIPCHK(c, hybrid_launch(total, (size_t)arena * 8, stream, (const GangDesc*)(dmeta + m_descs),
+                            (const int*)(dmeta + m_blocks), total, nullptr, arena));
+    HIPCHK(c, hipEventRecord(c->k1, stream));
+    if ((rc = gang_serialize_end(c, stream))) return rc;
+  }
   c->timed = true;
-  for (int s = 0; s < count; ++s)
-    HIPCHK(c, hipMemcpyAsync(hbase + r_res + 8 * res_stride * s, dbase + res_off[s], 8 * (16 + A[s].order + 1),
-                             hipMemcpyDeviceToHost, stream));
+  HIPCHK(c, hipMemcpyAsync(hbase + r_res, dres, 8 * res_stride * count, hipMemcpyDeviceToHost, stream));
   HIPCHK(c, hipStreamSynchronize(stream));
   int worst = 0;
   for (int s = 0; s < count; ++s) {
@@ -1424,6 +1682,79 @@ int ttals_set_matvec_variant(ttals_ctx* c, int variant) {
   if (!c || variant < 0 || variant > 1) return TTALS_E_ARG;
   if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
   c->matvec_variant = variant;
+  return 0;
+}
+
+int ttals_set_schedule(ttals_ctx* c, int schedule, int gang_precision) {
+  if (!c) return TTALS_E_ARG;
+  if (schedule != 0 && schedule != 1) return fail(c, TTALS_E_ARG, "schedule must be 0 or 1");
+  if (gang_precision != 0 && gang_precision != 1) return fail(c, TTALS_E_ARG, "gang precision must be 0 or 1");
+  c->schedule = schedule;
+  c->gang_prec = gang_precision;
+  return 0;
+}
+
+int ttals_set_cost_model(ttals_ctx* c, const double* p, int n) {
+  if (!c) return TTALS_E_ARG;
+  if (!p || n != 9) return fail(c, TTALS_E_ARG, "the cost model has 9 parameters");
+  for (int k = 0; k < 9; ++k)
+    if (!(p[k] > 0.0) || !std::isfinite(p[k])) return fail(c, TTALS_E_ARG, "cost model parameters must be finite and > 0");
+  c->cost.on = p[0];
+  c->cost.off = p[1];
+  c->cost.solve = p[2];
+  for (int k = 0; k < 6; ++k) c->cost.gang[k] = p[3 + k];
+  return 0;
+}
+
+int ttals_get_cost_model(ttals_ctx* c, double* p, int n) {
+  if (!c) return TTALS_E_ARG;
+  if (!p || n != 9) return fail(c, TTALS_E_ARG, "the cost model has 9 parameters");
+  p[0] = c->cost.on;
+  p[1] = c->cost.off;
+  p[2] = c->cost.solve;
+  for (int k = 0; k < 6; ++k) p[3 + k] = c->cost.gang[k];
+  return 0;
+}
+
+int ttals_sle_cost(ttals_ctx* c, int count, const ttals_tt* A, const ttals_tt* x0, double* out) {
+  if (count < 1 || !A || !x0 || !out) return c ? fail(c, TTALS_E_ARG, "bad arguments") : TTALS_E_ARG;
+  const long arena = c ? std::max<long>(c->arena, GEMM_LDS_DOUBLES) : 61 * 1024 / 8;
+  const int mvar = c ? c->matvec_variant : 0;
+  for (int s = 0; s < count; ++s) {
+    int rc;
+    if ((rc = check_tt(c, A[s], false, "operator")) || (rc = check_tt(c, x0[s], true, "initial_guess"))) return rc;
+    if (x0[s].order != A[s].order) return c ? fail(c, TTALS_E_SHAPE, "orders of the TTs differ") : TTALS_E_SHAPE;
+    int ns = 0;
+    sle_cost(A[s], x0[s], arena, mvar, &out[3 * s], &out[3 * s + 1], &ns);
+    out[3 * s + 2] = ns;
+  }
+  return 0;
+}
+
+int ttals_sle_plan(ttals_ctx* c, int count, const ttals_tt* A, const ttals_tt* x0, int repeats, int slots,
+                   int* gangs, double* makespan_ms) {
+  if (count < 1 || !A || !x0 || !gangs || repeats < 0 || slots < 1)
+    return c ? fail(c, TTALS_E_ARG, "bad arguments") : TTALS_E_ARG;
+  for (int s = 0; s < count; ++s) {
+    int rc;
+    if ((rc = check_tt(c, A[s], false, "operator")) || (rc = check_tt(c, x0[s], true, "initial_guess"))) return rc;
+    if (x0[s].order != A[s].order) return c ? fail(c, TTALS_E_SHAPE, "orders of the TTs differ") : TTALS_E_SHAPE;
+  }
+  const CostModel cm = c ? c->cost : CostModel();
+  const long arena = c ? std::max<long>(c->arena, GEMM_LDS_DOUBLES) : 61 * 1024 / 8;
+  std::vector<int> G;
+  std::vector<double> t1;
+  const double m = plan_batch(cm, arena, c ? c->matvec_variant : 0, count, A, x0, repeats, slots, G, t1);
+  for (int s = 0; s < count; ++s) gangs[s] = G[s];
+  if (makespan_ms) *makespan_ms = m;
+  return 0;
+}
+
+int ttals_last_plan(ttals_ctx* c, int* gangs, int count) {
+  if (!c) return TTALS_E_ARG;
+  if (!gangs || count != (int)c->last_plan.size())
+    return fail(c, TTALS_E_ARG, "count differs from the last sle batch (" + std::to_string(c->last_plan.size()) + ")");
+  for (int s = 0; s < count; ++s) gangs[s] = c->last_plan[s];
   return 0;
 }
 

@@ -723,4 +723,23 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   }
 }
 
+// One workgroup per system.  Dynamic LDS (`arena` doubles, at least the
+// GEMM staging tiles) holds either the staging tiles or the on-chip copy of
+// a local problem.
+__device__ __forceinline__ void als_block(const SysDesc& D, double* dyn_lds, long arena, SolveLDS& S) {
+  // x <- x0: the engine works in place on the output cores
+  for (int i = 0; i < D.d; ++i) {
+    const double* src = D.x0c[i];
+    double* dst = D.xc[i];
+    if (src == dst) continue;
+    const long len = (long)D.rx[i] * D.n[i] * D.rx[i + 1];
+    for (long e = threadIdx.x; e < len; e += NT) dst[e] = src[e];
+  }
+  __syncthreads();
+  als_solve<double>(D, dyn_lds, arena, S);
+  __syncthreads();
+  // one compact row per system, so the host fetches every result with one copy
+  for (int k = threadIdx.x; k < 17 + D.d; k += NT) D.res[k] = k < 16 ? D.out[k] : (double)D.rx[k - 16];
+}
+
 }  // namespace ttals

@@ -16,9 +16,11 @@
 //     workgroup takes the same decisions with the same numbers;
 //   - small sequential pieces (rhs stacks, local rhs, the QR of the new core
 //     and the warm start of the next position) run on workgroup 0.
-// The gangs of one launch are co-resident (cooperative launch), so the
-// barrier's waits end; each wait is also bounded, and a gang that times out
-// raises an abort flag every member sees.
+// The gang workgroups of a launch come first in workgroup order and fit the
+// device together (checked on the host); workgroups are dispatched in index
+// order, so every gang eventually runs whole and the barrier's waits end.
+// Each wait is also bounded in time, and a gang that times out raises an
+// abort flag every member sees.
 //
 // Precision: prec 0 runs every Krylov iteration in FP64; prec 1 runs them in
 // FP32 on FP32 copies of the local operator (L, A core, R) and refines the
@@ -89,7 +91,10 @@ struct GangDesc {
 
 // --- gang barrier ---------------------------------------------------------------
 
-constexpr long GANG_SPIN_LIMIT = 1L << 24;  // ~seconds: far beyond any phase
+// Longest wait at one barrier, in ticks of the 100 MHz constant clock
+// (s_memrealtime): 20 s.  A member may wait for the others to be dispatched
+// while other kernels still hold the CUs, so the bound is in time, not spins.
+constexpr unsigned long long GANG_WAIT_TICKS = 20ull * 100000000ull;
 // partial-sum slots per member: 4 scalars, then two regions of KMAX + 2 for
 // the two Gram-Schmidt passes (a member may write the second pass's sums
 // while another still reads the first's)
@@ -115,13 +120,14 @@ __device__ bool gang_sync(GangSync* s, int G) {
         __hip_atomic_store(&s->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&s->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
-        long spins = 0;
+        const unsigned long long t0 = wall_clock64();
+        unsigned spins = 0;
         while (__hip_atomic_load(&s->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
           if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             good = 0;
             break;
           }
-          if (++spins > GANG_SPIN_LIMIT) {
+          if ((++spins & 255u) == 0 && wall_clock64() - t0 > GANG_WAIT_TICKS) {
             __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             good = 0;
             break;
@@ -980,12 +986,13 @@ __device__ void gang_als(GangCtx& C) {
   }
 }
 
-// One launch: gangs of consecutive workgroups, one gang per system.
-__global__ __launch_bounds__(NT) void k_gang(const GangDesc* descs, const int* block_sys) {
-  __shared__ double lds[GEMM_LDS_DOUBLES];
-  __shared__ SolveLDS S;
-  const GangDesc& D = descs[block_sys[blockIdx.x]];
-  const int g = (int)blockIdx.x - D.first;
+// Workgroup `blk` of a launch whose first workgroups form gangs of
+// consecutive workgroups, one gang per system (block_sys: system of each).
+// `lds` holds at least GEMM_LDS_DOUBLES.
+__device__ __forceinline__ void gang_block(const GangDesc* descs, const int* block_sys, int blk, double* lds,
+                                           SolveLDS& S) {
+  const GangDesc& D = descs[block_sys[blk]];
+  const int g = blk - D.first;
   GangCtx C{D, g, D.G, D.t1 + (long)g * D.tmax, D.t2 + (long)g * D.tmax, lds, S};
   gang_als(C);
 }

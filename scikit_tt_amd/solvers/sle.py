@@ -26,9 +26,20 @@ __all__ = ['als', 'mals', 'als_batch', 'als_gang']
 _SOLVERS = {'solve': 0, 'lu': 1}
 
 
+_PRECISIONS = {'fp64': 0, 'mixed': 1}
+_SCHEDULES = {'one': 0, 'auto': 1}
+
+
 def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='solve', tol=1e-12, maxit=4000,
-              return_info=False):
+              return_info=False, schedule='one', gang_precision='fp64'):
     """Solve independent systems A_s x_s = b_s in ONE engine launch.
+
+    ``schedule='one'``: one workgroup per system.  ``schedule='auto'``: the
+    systems that the device's launch-time model (``scikit_tt_amd.tune``)
+    says would set the launch time are solved by gangs of workgroups (as
+    ``als_gang``, in ``gang_precision``), the rest by one workgroup each, all
+    in the same launch; ``infos.gangs`` then holds each system's gang size
+    (0: one workgroup).
 
     Returns a sequence of solutions (and a sequence of SolveInfo if
     ``return_info``), each made on access.  A singular local LU raises numpy.linalg.LinAlgError
@@ -36,6 +47,10 @@ def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='s
     """
     if solver not in _SOLVERS:
         raise NotImplementedError("solver must be 'solve' or 'lu', got %r" % (solver,))
+    if schedule not in _SCHEDULES:
+        raise ValueError("schedule must be 'one' or 'auto', got %r" % (schedule,))
+    if gang_precision not in _PRECISIONS:
+        raise ValueError("gang_precision must be 'fp64' or 'mixed', got %r" % (gang_precision,))
     count = len(operators)
     if count == 0 or len(initial_guesses) != count or len(right_hand_sides) != count:
         raise ValueError('operators, initial guesses and right-hand sides must be non-empty lists of equal length')
@@ -47,17 +62,19 @@ def als_batch(operators, initial_guesses, right_hand_sides, repeats=1, solver='s
     cores, opacks, O = _engine.outputs(initial_guesses)
     res = (_lib.Result * count)()
     ctx = _lib.context(dev.index)
-    rc = _lib.load().ttals_sle_als_batch(ctx, _stream(dev), count, A, X, B, O, int(repeats), _SOLVERS[solver],
-                                         float(tol), int(maxit), res)
-    _lib.check(rc, ctx, 'sle.als')
+    lib = _lib.load()
+    with _lib.schedule(ctx, _SCHEDULES[schedule], _PRECISIONS[gang_precision]):
+        rc = lib.ttals_sle_als_batch(ctx, _stream(dev), count, A, X, B, O, int(repeats), _SOLVERS[solver],
+                                     float(tol), int(maxit), res)
+        _lib.check(rc, ctx, 'sle.als')
+        gangs = np.zeros(count, dtype=np.int32)
+        _lib.check(lib.ttals_last_plan(ctx, gangs.ctypes.data, count), ctx, 'last_plan')
     infos = _engine.Infos(res)
+    infos.gangs = gangs
     if not return_info:
         _engine.raise_first(infos, 'sle.als', count)
     sols = _engine.Solutions(cores, opacks, initial_guesses)
     return (sols, infos) if return_info else sols
-
-
-_PRECISIONS = {'fp64': 0, 'mixed': 1}
 
 
 def als_gang(operators, initial_guesses, right_hand_sides, repeats=1, solver='solve', precision='fp64', gang=0,
@@ -65,7 +82,7 @@ def als_gang(operators, initial_guesses, right_hand_sides, repeats=1, solver='so
     """Large systems on the multi-workgroup engine: each system is solved by a
     gang of ``gang`` workgroups (0: one per row of its widest bond, up to 32)
     that split the local matvec, the interface stacks and the Krylov vector
-    operations; all gangs run at once in one cooperative launch.
+    operations; all gangs run at once in one launch.
 
     ``precision='mixed'``: Krylov iterations in FP32 on an FP32 copy of the
     local operator, refined with the FP64 local residual until it meets

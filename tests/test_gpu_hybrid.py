@@ -1,0 +1,110 @@
+"""GPU: the mixed schedule (one launch of gangs for the slowest systems and
+single workgroups for the rest) against one workgroup per system, and the
+launch-time model's fit on the device.
+
+The gang engine and the one-workgroup engine solve each local problem to
+1e-12 by different Krylov paths, so solutions agree to ~1e-9 (not bit for
+bit) and ranks are identical; a system's result does not depend on which
+path the planner gave it beyond that.
+"""
+import numpy as np
+import pytest
+
+from oracle import tt_ops
+from scikit_tt_amd import workloads as W
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def env():
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    from scikit_tt_amd import _lib, tune
+    torch.cuda.set_device(0)
+    _lib.load()
+    saved = tune.get_model(0)
+    yield
+    tune.set_model(saved, 0)
+
+
+def _tt(cores):
+    from scikit_tt_amd.tensor_train import TT
+    return TT([np.asarray(c) for c in cores])
+
+
+def _reldiff(x, y):
+    return tt_ops.norm(tt_ops.sub(x, y)) / tt_ops.norm(y)
+
+
+def _batch(count, seed0=0):
+    systems = [W.shifted_batch_instance(seed0 + s, d_range=(4, 8), n_choices=(3, 4), r_range=(2, 12))
+               for s in range(count)]
+    return ([_tt(A) for A, _, _ in systems], [_tt(x) for _, x, _ in systems], [_tt(b) for _, _, b in systems])
+
+
+@pytest.mark.parametrize('gang_precision', ['fp64', 'mixed'])
+def test_mixed_schedule_matches_one_workgroup(env, gang_precision):
+    import scikit_tt_amd.solvers.sle as sle
+    from scikit_tt_amd import tune
+    print('case mixed schedule, 12 systems, gangs %s' % gang_precision, flush=True)
+    As, Xs, Bs = _batch(12)
+    # a model under which gangs pay off for the larger systems
+    p = tune.get_model(0)
+    p[3:] = [1.0, 0.05, 0.04, 0.03, 0.03, 0.03]
+    tune.set_model(p, 0)
+    sols, infos = sle.als_batch(As, Xs, Bs, repeats=2, schedule='auto', gang_precision=gang_precision,
+                                return_info=True)
+    ref = sle.als_batch(As, Xs, Bs, repeats=2)
+    g = infos.gangs
+    print('  gangs', g.tolist(), flush=True)
+    assert (g > 0).any() and (g == 0).any()
+    assert (infos.array['status'] == 0).all(), infos.array['status']
+    for k in range(len(As)):
+        assert sols[k].ranks == ref[k].ranks
+        assert _reldiff(sols[k].cores[:], ref[k].cores[:]) < 1e-8, (k, g[k])
+
+
+def test_calibrated_model_and_auto_schedule(env):
+    import scikit_tt_amd.solvers.sle as sle
+    from scikit_tt_amd import tune
+    from scikit_tt_amd.tensor_train import residual_norms
+    print('case calibrate on 16 systems', flush=True)
+    As, Xs, Bs = _batch(16, seed0=100)
+    fit = tune.calibrate(As, Xs, Bs, repeats=1, sample=6, gang_sample=2, gangs=(2, 4))
+    p = np.array(list(fit['params'].values()))
+    print('  fit', fit, flush=True)
+    assert np.all(np.isfinite(p)) and np.all(p > 0)
+    np.testing.assert_array_equal(tune.get_model(0), p)
+    sols, infos = sle.als_batch(As, Xs, Bs, repeats=1, schedule='auto', return_info=True)
+    assert (infos.array['status'] == 0).all()
+    res, bn = residual_norms(As, sols, Bs)
+    ref = sle.als_batch(As, Xs, Bs, repeats=1)
+    res1, _ = residual_norms(As, ref, Bs)
+    np.testing.assert_allclose(res / bn, res1 / bn, rtol=1e-3, atol=1e-10)
+
+
+def test_uneven_batch_full_size_mixed_schedule(env):
+    """The 512-system uneven batch under the default model: the planner gangs
+    its largest systems; every system's rel.res equals the one-workgroup
+    launch's to the agreement of the two engines."""
+    import scikit_tt_amd.solvers.sle as sle
+    from scikit_tt_amd import tune
+    from scikit_tt_amd.tensor_train import residual_norms, from_host_batch
+    print('case uneven batch 512, mixed schedule', flush=True)
+    tune.set_model([1.0e-7, 3.4e-7, 0.1, 1.0, 0.65, 0.40, 0.36, 0.32, 0.32], 0)
+    inst = [W.shifted_batch_instance(s) for s in range(512)]
+    As = from_host_batch([i[0] for i in inst])
+    Xs = from_host_batch([i[1] for i in inst])
+    Bs = from_host_batch([i[2] for i in inst])
+    sols, infos = sle.als_batch(As, Xs, Bs, repeats=1, schedule='auto', return_info=True)
+    g = infos.gangs
+    print('  gangs per size', {int(k): int(v) for k, v in enumerate(np.bincount(g)) if v}, flush=True)
+    assert g.sum() > 0 and (infos.array['status'] == 0).all()
+    ref = sle.als_batch(As, Xs, Bs, repeats=1)
+    res, bn = residual_norms(As, sols, Bs)
+    res1, _ = residual_norms(As, ref, Bs)
+    rel, rel1 = res / bn, res1 / bn
+    # the engines agree to ~1e-9 in the iterate: rel.res to a few per cent
+    np.testing.assert_allclose(rel, rel1, rtol=0.1, atol=1e-11)
+    assert ((rel <= 1e-8) == (rel1 <= 1e-8)).sum() >= 510
