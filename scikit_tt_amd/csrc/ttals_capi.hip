@@ -1157,7 +1157,7 @@ void gang_offsets(const SysSizes& z, const ttals_tt& A, const ttals_tt& x0, int 
   go.w_t2 = dd.take(8 * (size_t)G * z.tmax);
   go.w_part = dd.take(8 * (size_t)G * GANG_LD);
   go.rmax2 = z.rmax * z.rmax;
-  go.w_gram = dd.take(8 * (size_t)G * go.rmax2);
+  go.w_gram = dd.take(8 * (size_t)(G + 1) * go.rmax2);  // per-member partials, then the total
   go.w_qrs = dd.take(8 * (size_t)G * 3 * go.rmax2);
 }
 
@@ -1498,7 +1498,7 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   if (gang > 256) return fail(c, TTALS_E_ARG, "gang size must be <= 256");
   HIPCHK(c, hipSetDevice(c->device));
   int cap = 0, rc;
-  const long arena = GEMM_LDS_DOUBLES;  // a gang stages its GEMM tiles only
+  const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);  // GEMM tiles, then the split QR's factors
   if ((rc = gang_capacity(c, (size_t)arena * 8, &cap))) return rc;
   // pass 1: checks, gang sizes, totals
   Layout hl, dl;

@@ -81,7 +81,7 @@ struct GangDesc {
   double* qq;
   double* tau;
   double* rf;
-  double* gram;  // G x rmax^2 partial Gram matrices of the split QR
+  double* gram;  // (G + 1) x rmax^2: partial Gram matrices of the split QR, then their total
   double* qrs;   // per workgroup: 3 x rmax^2 (Cholesky factor, accumulated R, product)
   long rmax2;    // rmax^2
   GangSync* sync;
@@ -95,10 +95,12 @@ struct GangDesc {
 // (s_memrealtime): 20 s.  A member may wait for the others to be dispatched
 // while other kernels still hold the CUs, so the bound is in time, not spins.
 constexpr unsigned long long GANG_WAIT_TICKS = 20ull * 100000000ull;
-// partial-sum slots per member: 4 scalars, then two regions of KMAX + 2 for
-// the two Gram-Schmidt passes (a member may write the second pass's sums
-// while another still reads the first's)
-constexpr int GANG_LD = 4 + 2 * (KMAX + 2);
+// partial-sum slots per member: 4 scalars, then two regions of 64 (the two
+// Gram-Schmidt passes of GMRES, KMAX + 2 sums each; the split Householder
+// QR's dots, alternating by column): a member may write one region's sums
+// while another still reads the other's
+constexpr int GANG_LD = 4 + 2 * 64;
+static_assert(KMAX + 2 <= 64, "a Gram-Schmidt pass's sums fit one region");
 
 // Every workgroup of the gang calls this at the same points.  Returns false
 // once the gang has aborted (a wait ran out): callers then unwind.  Thread 0
@@ -150,10 +152,29 @@ __device__ __forceinline__ void gang_span(int total, int G, int g, int& lo, int&
 }
 
 // Sum of the G partial sums of slot `k` (stride `ld` per member), same order everywhere.
-__device__ __forceinline__ double gang_total(const double* part, int G, int ld, int k) {
-  double s = 0.0;
-  for (int h = 0; h < G; ++h) s += part[(long)h * ld + k];
-  return s;
+// Sums of the G members' partials part[h ld + k0 + k], k < cnt, into out[k]
+// (LDS).  Every load of a round is issued at once -- the partials sit in
+// other members' L2 or in HBM, and G dependent loads would each pay that
+// latency -- into buf (LDS, GEMM_LDS_DOUBLES doubles: the GEMM staging
+// tiles, free between GEMMs); then each sum runs in member order, identical
+// on every member.  Ends with a barrier of the workgroup.
+__device__ void gang_totals(const double* part, int G, int ld, int k0, int cnt, double* buf, double* out) {
+  const int chunk = GEMM_LDS_DOUBLES / G > 0 ? GEMM_LDS_DOUBLES / G : 1;
+  for (int c0 = 0; c0 < cnt; c0 += chunk) {
+    const int nk = cnt - c0 < chunk ? cnt - c0 : chunk;
+#pragma unroll 4
+    for (int idx = threadIdx.x; idx < nk * G; idx += NT) {
+      const int k = idx / G, h = idx - k * G;
+      buf[idx] = part[(long)h * ld + k0 + c0 + k];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nk; k += NT) {
+      double sg = 0.0;
+      for (int h = 0; h < G; ++h) sg += buf[k * G + h];
+      out[c0 + k] = sg;
+    }
+    __syncthreads();
+  }
 }
 
 // out[k] = <V_k, w> over elements [e0, e1) for k < cnt, and out[cnt] = <w, w>
@@ -277,7 +298,8 @@ struct GangCtx {
   int g, G;
   double* t1;
   double* t2;
-  double* lds;
+  double* lds;   // GEMM staging tiles first, lds_cap doubles in all
+  long lds_cap;
   SolveLDS& S;
   __device__ bool sync() {
     const long long t0 = ph_now();
@@ -344,8 +366,10 @@ __device__ int gang_cg(GangCtx& C, const GangLocal& o, const T* L, const T* A, c
     part[(long)g * ld + 1] = rr;
   }
   if (!C.sync()) return -1;
-  ff = gang_total(part, G, ld, 0);
-  rr = gang_total(part, G, ld, 1);
+  double* tot = C.S.scal + 2;  // two LDS slots for scalar totals
+  gang_totals(part, G, ld, 0, 2, C.lds, tot);
+  ff = tot[0];
+  rr = tot[1];
   const double fn = sqrt(ff);
   if (threadIdx.x == 0) *vfl += 4.0 * o.N;
   if (fn == 0.0 || sqrt(rr) <= tol * fn) return 0;
@@ -360,7 +384,8 @@ __device__ int gang_cg(GangCtx& C, const GangLocal& o, const T* L, const T* A, c
     wg_sum3(pq, u1, u2, C.S.red);
     if (threadIdx.x == 0) part[(long)g * ld + 2] = pq;
     if (!C.sync()) return -1;
-    pq = gang_total(part, G, ld, 2);
+    gang_totals(part, G, ld, 2, 1, C.lds, tot);
+    pq = tot[0];
     if (!(pq > 0.0)) return 1;
     const double al = rr / pq;
     double rn = 0.0;
@@ -373,7 +398,8 @@ __device__ int gang_cg(GangCtx& C, const GangLocal& o, const T* L, const T* A, c
     wg_sum3(rn, u1, u2, C.S.red);
     if (threadIdx.x == 0) part[(long)g * ld + 3] = rn;
     if (!C.sync()) return -1;
-    rn = gang_total(part, G, ld, 3);
+    gang_totals(part, G, ld, 3, 1, C.lds, tot);
+    rn = tot[0];
     ++*iters;
     if (threadIdx.x == 0) *vfl += 10.0 * o.N;
     if (sqrt(rn) <= tol * fn) return 0;
@@ -407,7 +433,9 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
   wg_sum3(ff, u1, u2, S.red);
   if (threadIdx.x == 0) part[(long)g * ld + 0] = ff;
   if (!C.sync()) return -1;
-  const double fn = sqrt(gang_total(part, G, ld, 0));
+  double* tot = S.scal + 2;  // two LDS slots for scalar totals (S.scal[0] is the Givens residual)
+  gang_totals(part, G, ld, 0, 1, C.lds, tot);
+  const double fn = sqrt(tot[0]);
   if (fn == 0.0) {
     for (long e = o.e0 + threadIdx.x; e < o.e1; e += NT) x[e] = (T)0;
     return C.sync() ? 0 : -1;
@@ -424,7 +452,8 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
     wg_sum3(bb, u1, u2, S.red);
     if (threadIdx.x == 0) part[(long)g * ld + 1] = bb;
     if (!C.sync()) return -1;
-    const double beta = sqrt(gang_total(part, G, ld, 1));
+    gang_totals(part, G, ld, 1, 1, C.lds, tot);
+    const double beta = sqrt(tot[0]);
     if (threadIdx.x == 0) *vfl += 4.0 * N;
     if (beta <= tol * fn) return 0;
     if (*iters >= maxit) return 2;
@@ -450,8 +479,7 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
         const int base = 4 + pass * (KMAX + 2);
         gang_multidot<T>(V, N, j + 1, w, o.e0, o.e1, part + (long)g * ld + base, S.part);
         if (!C.sync()) return -1;
-        for (int k = threadIdx.x; k <= j + 1; k += NT) S.h[k] = gang_total(part, G, ld, base + k);
-        __syncthreads();
+        gang_totals(part, G, ld, base, j + 2, C.lds, S.h);
         if (pass == 0) w0 = sqrt(fmax(S.h[j + 1], 0.0));
         for (long e = o.e0 + threadIdx.x; e < o.e1; e += NT) {
           double we = (double)w[e];
@@ -468,7 +496,8 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
       ww = wg_sum(ww, S.red);
       if (threadIdx.x == 0) part[(long)g * ld + 2] = ww;
       if (!C.sync()) return -1;
-      hn = sqrt(gang_total(part, G, ld, 2));
+      gang_totals(part, G, ld, 2, 1, C.lds, tot);
+      hn = sqrt(tot[0]);
       if (!(hn > 1e-14 * w0)) {
         hn = 0.0;  // (lucky) breakdown: the Krylov space is invariant
         brk = true;
@@ -534,7 +563,8 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
       tr = wg_sum(tr, S.red);
       if (threadIdx.x == 0) part[(long)g * ld + 1] = tr;
       if (!C.sync()) return -1;
-      return sqrt(gang_total(part, G, ld, 1)) <= tol * fn ? 0 : 2;
+      gang_totals(part, G, ld, 1, 1, C.lds, tot);
+      return sqrt(tot[0]) <= tol * fn ? 0 : 2;
     }
   }
 }
@@ -620,44 +650,63 @@ __device__ double gang_build_right(GangCtx& C, int i, bool* ok) {
 // Yanagisawa: "Shifted Cholesky QR for computing the QR factorization of
 // ill-conditioned matrices", SIAM J. Sci. Comput. 42 (2020)): each pass forms
 // the Gram matrix from per-member partials (one barrier), every member
-// factors it (identical inputs and order: identical R), and solves its own
-// rows against R; the first pass shifts the Gram matrix by
-// 11 (m c + c (c + 1)) u ||W||_F^2 so that it stays positive definite for
-// condition numbers up to ~1/u.  The panel is copied from g into D.qw:
-// forward W[row, col] = g[row c + col] (m = a n, c = b), backward
-// W[row, col] = g[col m + row] (m = n b, c = a).  On success Q is in D.qw
-// (own rows) and R (c x c, upper) in this member's qrs + rmax2; returns 0, or
-// 1 when a pivot failed (every member alike), 2 non-finite, -1 aborted.
-__device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool transposed, double* fl) {
+// factors it (identical inputs and order: identical R), inverts the factor
+// and multiplies its own rows by the inverse (one GEMM); the first pass
+// shifts the Gram matrix by 11 (m c + c (c + 1)) u ||W||_F^2 so that it stays
+// positive definite for condition numbers up to ~1/u.  The c x c work runs in
+// LDS when 2 c^2 + c doubles fit after the GEMM tiles, else in the member's
+// global slots.  The panel is copied from g: forward W[row, col] =
+// g[row c + col] (m = a n, c = b), backward W[row, col] = g[col m + row]
+// (m = n b, c = a).  On success *Q is the panel's Q (own rows valid, row
+// stride c, in D.qw or D.qq) and *R the upper c x c factor; returns 0, or 1
+// when a pivot failed (every member alike), 2 non-finite, -1 aborted.
+__device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool transposed, double* fl,
+                           const double** Qout, const double** Rout) {
   const GangDesc& D = C.D;
-  SolveLDS& S = C.S;
-  double* Wq = D.qw;
   int m0, m1;
   gang_span(m, C.G, C.g, m0, m1);
   const int rows = m1 - m0;
-  double* Rp = D.qrs + (long)C.g * 3 * D.rmax2;  // this pass's factor
-  double* Rt = Rp + D.rmax2;                       // accumulated R
-  double* Tm = Rt + D.rmax2;                       // product temporary
   const long cc = (long)c * c;
+  double *Rp, *Rt, *tmp;
+  if (GEMM_LDS_DOUBLES + 2 * cc + c <= C.lds_cap) {
+    Rp = C.lds + GEMM_LDS_DOUBLES;
+    Rt = Rp + cc;
+    tmp = Rt + cc;
+  } else {
+    Rp = D.qrs + (long)C.g * 3 * D.rmax2;
+    Rt = Rp + D.rmax2;
+    tmp = Rt + D.rmax2;
+  }
+  double* Win = D.qw;
+  double* Wout = D.qq;
   for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
     const long row = e / c, col = e - row * c;
-    Wq[e] = transposed ? g[col * m + row] : g[e];
+    Win[e] = transposed ? g[col * m + row] : g[e];
   }
   for (long e = threadIdx.x; e < cc; e += NT) Rt[e] = (e / c == e % c) ? 1.0 : 0.0;
   __syncthreads();
   for (int pass = 0; pass < 3; ++pass) {
+    const long long tg = 0;
     // partial Gram of this member's rows
     double* slot = D.gram + (long)C.g * D.rmax2;
-    wg_gemm<double, double>(c, c, rows, ctr<double>(Wq + (long)m0 * c, c), crm<double>(Wq + (long)m0 * c, c),
-                            mrm<double>(slot, c), C.lds);
-    if (!C.sync()) return -1;
-    // total Gram into Rp (every member, same order)
-    for (long e = threadIdx.x; e < cc; e += NT) {
-      double sg = 0.0;
-      for (int h = 0; h < C.G; ++h) sg += D.gram[(long)h * D.rmax2 + e];
-      Rp[e] = sg;
+    if (rows > 0) {
+      wg_gemm<double, double>(c, c, rows, ctr<double>(Win + (long)m0 * c, c), crm<double>(Win + (long)m0 * c, c),
+                              mrm<double>(slot, c), C.lds);
+    } else {
+      for (long e = threadIdx.x; e < cc; e += NT) slot[e] = 0.0;
     }
-    __syncthreads();
+    if (!C.sync()) return -1;
+    // total Gram: each member sums its slice of the c^2 entries over the
+    // members, one barrier, then every member reads the whole total into Rp
+    {
+      const long s0 = cc * C.g / C.G, s1 = cc * (C.g + 1) / C.G;
+      double* total = D.gram + (long)C.G * D.rmax2;
+      gang_totals(D.gram, C.G, (int)D.rmax2, (int)s0, (int)(s1 - s0), C.lds, total + s0);
+      if (!C.sync()) return -1;
+      for (long e = threadIdx.x; e < cc; e += NT) Rp[e] = total[e];
+      __syncthreads();
+    }
+    (void)tg;
     if (pass == 0) {
       double tr = 0.0;
       for (int j = 0; j < c; ++j) tr += Rp[(long)j * c + j];
@@ -667,13 +716,16 @@ __device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool trans
       for (int j = threadIdx.x; j < c; j += NT) Rp[(long)j * c + j] += shift;
       __syncthreads();
     }
-    // Cholesky Gram = R^T R (upper R in place, right-looking)
+    // Cholesky Gram = R^T R (upper R in place, right-looking; lower part zeroed)
     for (int j = 0; j < c; ++j) {
       const double piv = Rp[(long)j * c + j];
       if (!(piv > 0.0) || !isfinite(piv)) return 1;
       const double rjj = sqrt(piv), inv = 1.0 / rjj;
       __syncthreads();
-      for (int q = j + 1 + threadIdx.x; q < c; q += NT) Rp[(long)j * c + q] *= inv;
+      for (int q = j + 1 + threadIdx.x; q < c; q += NT) {
+        Rp[(long)j * c + q] *= inv;
+        Rp[(long)q * c + j] = 0.0;
+      }
       if (threadIdx.x == 0) Rp[(long)j * c + j] = rjj;
       __syncthreads();
       const int rest = c - j - 1;
@@ -683,28 +735,256 @@ __device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool trans
       }
       __syncthreads();
     }
-    // own rows: W <- W R^{-1} (forward substitution per row)
-    for (int row = m0 + threadIdx.x; row < m1; row += NT) {
-      double* w = Wq + (long)row * c;
-      for (int j = 0; j < c; ++j) {
-        double t = w[j];
-        for (int q = 0; q < j; ++q) t -= w[q] * Rp[(long)q * c + j];
-        w[j] = t / Rp[(long)j * c + j];
+    // Rt <- Rp Rt, row by row in place (row p reads rows >= p of the old Rt)
+    for (int p = 0; p < c; ++p) {
+      for (int q = threadIdx.x; q < c; q += NT) {
+        double t = 0.0;
+        for (int k2 = p; k2 <= q; ++k2) t += Rp[(long)p * c + k2] * Rt[(long)k2 * c + q];
+        tmp[q] = (q >= p) ? t : 0.0;
       }
+      __syncthreads();
+      for (int q = threadIdx.x; q < c; q += NT) Rt[(long)p * c + q] = tmp[q];
+      __syncthreads();
     }
-    // R_total <- R_pass R_total (upper triangular)
-    for (long e = threadIdx.x; e < cc; e += NT) {
-      const int p = (int)(e / c), q = (int)(e % c);
-      double t = 0.0;
-      for (int k2 = p; k2 <= q; ++k2) t += Rp[(long)p * c + k2] * Rt[(long)k2 * c + q];
-      Tm[e] = (q >= p) ? t : 0.0;
+    // Rp <- Rp^{-1}, column by column: for i < j,
+    // inv[i][j] = -(sum_{k=i}^{j-1} inv[i][k] R[k][j]) / R[j][j]
+    for (int j = 0; j < c; ++j) {
+      const double rjj = Rp[(long)j * c + j];
+      for (int i = threadIdx.x; i <= j; i += NT) {
+        if (i == j) {
+          tmp[i] = 1.0 / rjj;
+        } else {
+          double t = 0.0;
+          for (int k2 = i; k2 < j; ++k2) t += Rp[(long)i * c + k2] * Rp[(long)k2 * c + j];
+          tmp[i] = -t / rjj;
+        }
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i <= j; i += NT) Rp[(long)i * c + j] = tmp[i];
+      __syncthreads();
     }
+    // own rows: W <- W R^{-1}
+    if (rows > 0)
+      wg_gemm<double, double>(rows, c, c, crm<double>(Win + (long)m0 * c, c), crm<double>(Rp, c),
+                              mrm<double>(Wout + (long)m0 * c, c), C.lds);
     __syncthreads();
-    for (long e = threadIdx.x; e < cc; e += NT) Rt[e] = Tm[e];
+    double* t = Win;
+    Win = Wout;
+    Wout = t;
+  }
+  *fl += 3.0 * (4.0 * (double)m * c * c + 2.0 * (double)c * c * c);
+  *Qout = Win;
+  *Rout = Rt;
+  return 0;
+}
+
+// out[q] = sum over rows i in [r_lo, r_hi) of v_i X[i][col0 + q], q < nq,
+// with v_i = 1 at i = j and V[i][j] below it (a Householder vector stored in
+// column j of V; rows of stride c).  Threads split (row group, column);
+// buf is NT doubles of LDS.  Ends with a barrier.
+__device__ void rows_dots(const double* V, const double* X, int c, int j, int r_lo, int r_hi, int col0, int nq,
+                          double* buf, double* out) {
+  const int groups = NT / nq;
+  const int q = (int)threadIdx.x % nq, grp = (int)threadIdx.x / nq;
+  double acc = 0.0;
+  if (grp < groups)
+    for (int i = r_lo + grp; i < r_hi; i += groups) {
+      const double v = (i == j) ? 1.0 : V[(long)i * c + j];
+      acc += v * X[(long)i * c + col0 + q];
+    }
+  buf[threadIdx.x] = grp < groups ? acc : 0.0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < nq; t += NT) {
+    double sg = 0.0;
+    for (int h = 0; h < groups; ++h) sg += buf[h * nq + t];
+    out[t] = sg;
+  }
+  __syncthreads();
+}
+
+// Split Householder QR of the panel (m x c, m >= c, c <= 64) over the gang,
+// for panels whose Gram matrix is numerically singular (the Cholesky QR
+// fails): the panel of the reference's own QR (numpy.linalg.qr, Householder)
+// and the same completion of a rank-deficient basis, up to rounding.
+// Column j: the norm below the diagonal and the pivot (one gang reduction),
+// then the reflector's dots with the trailing columns (a second one); Q is
+// formed by applying the reflectors to [I; 0] backwards (one reduction per
+// column).  Rows as in gang_cholqr; *Q in D.qq (own rows), *R in D.rf.
+// Returns 0, or -1 when the gang aborted.
+__device__ int gang_hhqr(GangCtx& C, const double* g, int m, int c, bool transposed, double* fl, const double** Qout,
+                         const double** Rout) {
+  const GangDesc& D = C.D;
+  SolveLDS& S = C.S;
+  const int G = C.G, ld = GANG_LD;
+  int m0, m1;
+  gang_span(m, G, C.g, m0, m1);
+  double* W = D.qw;
+  double* Qm = D.qq;
+  double* part = D.part;
+  double* tot = S.scal + 2;
+  double* wv = S.h;       // c <= 64 <= LKMAX + 2 values
+  double* taus = S.alpha;  // c <= LKMAX + 1
+  for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+    const long row = e / c, col = e - row * c;
+    W[e] = transposed ? g[col * m + row] : g[e];
+  }
+  __syncthreads();
+  for (int j = 0; j < c; ++j) {
+    double s2 = 0.0, al = 0.0, u = 0.0;
+    for (int i = (m0 > j + 1 ? m0 : j + 1) + threadIdx.x; i < m1; i += NT) {
+      const double t = W[(long)i * c + j];
+      s2 += t * t;
+    }
+    if (threadIdx.x == 0 && j >= m0 && j < m1) al = W[(long)j * c + j];
+    wg_sum3(s2, al, u, S.red);
+    // slots alternate by column: when column j has no reflector there is no
+    // second barrier before column j + 1 writes its sums
+    const int ns = 2 * (j & 1);
+    if (threadIdx.x == 0) {
+      part[(long)C.g * ld + ns] = s2;
+      part[(long)C.g * ld + ns + 1] = al;
+    }
+    if (!C.sync()) return -1;
+    gang_totals(part, G, ld, ns, 2, C.lds, tot);
+    s2 = tot[0];
+    const double alpha = tot[1];
+    double t = 0.0;
+    if (s2 != 0.0) {  // else H_j = I (LAPACK convention for an already-reduced column)
+      const double nrm = sqrt(alpha * alpha + s2);
+      const double beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int i = (m0 > j + 1 ? m0 : j + 1) + threadIdx.x; i < m1; i += NT) W[(long)i * c + j] *= sc;
+      if (threadIdx.x == 0 && j >= m0 && j < m1) W[(long)j * c + j] = beta;
+    }
+    if (threadIdx.x == 0) taus[j] = t;
+    __syncthreads();
+    const int nq = c - j - 1;
+    if (t != 0.0 && nq > 0) {
+      const int lo = m0 > j ? m0 : j;
+      rows_dots(W, W, c, j, lo, m1, j + 1, nq, C.lds, wv);
+      const int rg = 4 + 64 * (j & 1);
+      for (int q = threadIdx.x; q < nq; q += NT) part[(long)C.g * ld + rg + q] = wv[q];
+      if (!C.sync()) return -1;
+      gang_totals(part, G, ld, rg, nq, C.lds, wv);
+      for (long e = threadIdx.x; e < (long)(m1 > lo ? m1 - lo : 0) * nq; e += NT) {
+        const int i = lo + (int)(e / nq), q = (int)(e % nq);
+        const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+        W[(long)i * c + j + 1 + q] -= t * v * wv[q];
+      }
+      __syncthreads();
+    }
+  }
+  // R: the owners of rows 0 .. c-1 publish them
+  for (long e = (long)m0 * c + threadIdx.x; e < (long)(m1 < c ? m1 : c) * c; e += NT) {
+    const long row = e / c, col = e - row * c;
+    D.rf[e] = col >= row ? W[e] : 0.0;
+  }
+  // Q = H_0 ... H_{c-1} [I; 0], own rows, backwards
+  for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+    const long row = e / c, col = e - row * c;
+    Qm[e] = row == col ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int qcount = 0;
+  for (int j = c - 1; j >= 0; --j) {
+    const double t = taus[j];
+    if (t == 0.0) continue;
+    const int nq = c - j;  // columns j .. c-1
+    const int lo = m0 > j ? m0 : j;
+    rows_dots(W, Qm, c, j, lo, m1, j, nq, C.lds, wv);
+    // one barrier per column: consecutive columns use the two regions in turn
+    const int rg = 4 + 64 * (qcount++ & 1);
+    for (int q = threadIdx.x; q < nq; q += NT) part[(long)C.g * ld + rg + q] = wv[q];
+    if (!C.sync()) return -1;
+    gang_totals(part, G, ld, rg, nq, C.lds, wv);
+    for (long e = threadIdx.x; e < (long)(m1 > lo ? m1 - lo : 0) * nq; e += NT) {
+      const int i = lo + (int)(e / nq), q = (int)(e % nq);
+      const double v = (i == j) ? 1.0 : W[(long)i * c + j];
+      Qm[(long)i * c + j + q] -= t * v * wv[q];
+    }
     __syncthreads();
   }
-  *fl += 3.0 * (4.0 * (double)m * c * c + (double)c * c * c);
+  if (!C.sync()) return -1;  // R published, Q complete
+  *fl += 4.0 * (double)m * c * c - 4.0 / 3.0 * (double)c * c * c;
+  *Qout = Qm;
+  *Rout = D.rf;
   return 0;
+}
+
+// Right-orthonormal start x0 (the sle engine solves from it): for i = d-1 .. 1
+// the split QR of core i's transposed unfolding (n b x a), Q^T into core i,
+// core i-1 <- core i-1 R^T (split over its rows, via g); a panel with fewer
+// rows than columns, or a failed pivot, runs Householder QR on member 0
+// (which may lower the rank, as the reference's QR does).  Returns false
+// once the gang has aborted.
+__device__ bool gang_ortho_right(GangCtx& C, double* fl) {
+  const GangDesc& D = C.D;
+  SolveLDS& S = C.S;
+  for (int i = D.d - 1; i >= 1; --i) {
+    const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
+    const int m = n * b, c = a;
+    const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
+    if (m >= c && (long)c * c <= D.rmax2) {
+      const double *Q, *R;
+      const int q = gang_cholqr(C, D.xc[i], m, c, true, fl, &Q, &R);
+      if (q < 0) return false;
+      if (q == 0) {
+        int m0, m1;
+        gang_span(m, C.G, C.g, m0, m1);
+        for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+          const long row = e / c, col = e - row * c;
+          D.xc[i][col * m + row] = Q[e];
+        }
+        int r0, r1;
+        gang_span(cn, C.G, C.g, r0, r1);
+        if (r1 > r0)
+          wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(R, c),
+                                  mrm<double>(D.g + (long)r0 * c, c), C.lds);
+        __syncthreads();
+        for (long e = (long)r0 * c + threadIdx.x; e < (long)r1 * c; e += NT) D.xc[i - 1][e] = D.g[e];
+        *fl += 2.0 * cn * c * a;
+        if (!C.sync()) return false;
+        continue;
+      }
+      if (!C.sync()) return false;  // a pivot failed on every member alike
+      if (q == 1 && c <= 64) {
+        const double *Q2, *R2;
+        if (gang_hhqr(C, D.xc[i], m, c, true, fl, &Q2, &R2) < 0) return false;
+        int m0, m1;
+        gang_span(m, C.G, C.g, m0, m1);
+        for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+          const long row = e / c, col = e - row * c;
+          D.xc[i][col * m + row] = Q2[e];
+        }
+        int r0, r1;
+        gang_span(cn, C.G, C.g, r0, r1);
+        if (r1 > r0)
+          wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(R2, c),
+                                  mrm<double>(D.g + (long)r0 * c, c), C.lds);
+        __syncthreads();
+        for (long e = (long)r0 * c + threadIdx.x; e < (long)r1 * c; e += NT) D.xc[i - 1][e] = D.g[e];
+        *fl += 2.0 * cn * c * a;
+        if (!C.sync()) return false;
+        continue;
+      }
+    }
+    if (C.g == 0) {
+      const QRBuf B{D.qw, D.qq, D.rf, D.tau};
+      const int k = qr_factor(D.xc[i], m, a, true, B, S);
+      for (long e = threadIdx.x; e < (long)k * m; e += NT) {
+        const int rr = (int)(e / m), col = (int)(e % m);
+        D.xc[i][e] = B.Q[(long)col * k + rr];
+      }
+      ds_gemm<double, double>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+      wg_copy(D.g, D.xc[i - 1], cn * k);
+      if (threadIdx.x == 0) D.rx[i] = k;
+      *fl += fl_qr(m, a) + 2.0 * cn * k * a;
+      __syncthreads();
+    }
+    if (!C.sync()) return false;
+  }
+  return true;
 }
 
 // One local step at position i: local rhs, the split Krylov solve (FP64, or
@@ -762,7 +1042,8 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
         D.part[(long)C.g * ld + 1] = rr;
       }
       if (!C.sync()) return ST_ABORT;
-      const double fn = sqrt(gang_total(D.part, C.G, ld, 0)), rn = sqrt(gang_total(D.part, C.G, ld, 1));
+      gang_totals(D.part, C.G, ld, 0, 2, C.lds, S.scal + 2);
+      const double fn = sqrt(S.scal[2]), rn = sqrt(S.scal[3]);
       vfl += 5.0 * o.N;
       st.flops += fmv;
       if (fn == 0.0 || rn <= D.tol * fn) {
@@ -808,18 +1089,21 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
     const int m = forward ? a * n : n * b, c = forward ? b : a;
     if (m >= c && (long)c * c <= D.rmax2) {
       double qfl = 0.0;
-      const int q = gang_cholqr(C, D.g, m, c, !forward, &qfl);
+      const double *Q = nullptr, *Rt = nullptr;
+      const long long t_cq = ph_now();
+      const int q = gang_cholqr(C, D.g, m, c, !forward, &qfl, &Q, &Rt);
+      ph_add(S, PH_ORTH, t_cq);  // profiling builds: the split QR itself
+      const long long t_ws = ph_now();
       if (q < 0) return ST_ABORT;
       if (q == 2) status = ST_NONFINITE;
       if (q == 0) {
-        const double* Rt = D.qrs + (long)C.g * 3 * D.rmax2 + D.rmax2;
         int m0, m1;
         gang_span(m, C.G, C.g, m0, m1);
         // Q (own rows) into the core
         for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
           const long row = e / c, col = e - row * c;
-          if (forward) D.xc[i][e] = D.qw[e];
-          else D.xc[i][col * m + row] = D.qw[e];
+          if (forward) D.xc[i][e] = Q[e];
+          else D.xc[i][col * m + row] = Q[e];
         }
         if (!C.sync()) return ST_ABORT;  // every member done with g (its panel copy) before g is overwritten
         if (forward) {
@@ -841,12 +1125,46 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
                                     mrm<double>(D.g + (long)r0 * c, c), C.lds);
           st.flops += qfl + 2.0 * cn * c * a;
         }
+        ph_add(S, PH_TRI, t_ws);  // profiling builds: Q store and warm start
         ph_add(S, PH_QR, tq);
         if (!C.sync()) return ST_ABORT;
         return status;
       }
-      // a pivot failed (or non-finite): every member got here alike; member 0 below
+      // a pivot failed (or non-finite): every member got here alike
       if (!C.sync()) return ST_ABORT;
+      if (q == 1 && c <= 64) {
+        const double* Q2 = nullptr;
+        const double* R2 = nullptr;
+        if (gang_hhqr(C, D.g, m, c, !forward, &qfl, &Q2, &R2) < 0) return ST_ABORT;
+        int m0, m1;
+        gang_span(m, C.G, C.g, m0, m1);
+        for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+          const long row = e / c, col = e - row * c;
+          if (forward) D.xc[i][e] = Q2[e];
+          else D.xc[i][col * m + row] = Q2[e];
+        }
+        if (!C.sync()) return ST_ABORT;
+        if (forward) {
+          const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
+          int j0, j1;
+          gang_span(nc, C.G, C.g, j0, j1);
+          if (j1 > j0)
+            wg_gemm<double, double>(c, j1 - j0, b, crm<double>(R2, c), crm<double>(D.xc[i + 1] + j0, nc),
+                                    mrm<double>(D.g + j0, nc), C.lds);
+          st.flops += qfl + 2.0 * c * nc * b;
+        } else {
+          const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
+          int r0, r1;
+          gang_span(cn, C.G, C.g, r0, r1);
+          if (r1 > r0)
+            wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(R2, c),
+                                    mrm<double>(D.g + (long)r0 * c, c), C.lds);
+          st.flops += qfl + 2.0 * cn * c * a;
+        }
+        ph_add(S, PH_QR, tq);
+        if (!C.sync()) return ST_ABORT;
+        return status;
+      }
     }
   }
   if (C.g == 0) {
@@ -908,20 +1226,13 @@ __device__ void gang_als(GangCtx& C) {
   bool ok = C.sync();
   bool sym = false;
   if (ok) {
-    // member 0: slice symmetry, right-orthonormal start, boundary stacks
+    // right-orthonormal start (split QR over the gang)
+    ok = gang_ortho_right(C, &st.flops);
+  }
+  if (ok) {
+    // member 0: slice symmetry, boundary stacks
     if (C.g == 0) {
       sym = gang_slices_symmetric(D, S);
-      SysDesc sd;
-      sd.d = d;
-      sd.n = D.n;
-      sd.rx = D.rx;
-      sd.xc = D.xc;
-      sd.qw = D.qw;
-      sd.qq = D.qq;
-      sd.rf = D.rf;
-      sd.tau = D.tau;
-      sd.g = D.g;
-      ortho_right_x<double>(sd, C.lds, 0, S);
       if (threadIdx.x == 0) {
         D.Ls[D.Loff[0]] = 1.0;
         D.Rs[D.Roff[d - 1]] = 1.0;
@@ -988,12 +1299,12 @@ __device__ void gang_als(GangCtx& C) {
 
 // Workgroup `blk` of a launch whose first workgroups form gangs of
 // consecutive workgroups, one gang per system (block_sys: system of each).
-// `lds` holds at least GEMM_LDS_DOUBLES.
+// `lds` holds lds_cap >= GEMM_LDS_DOUBLES doubles.
 __device__ __forceinline__ void gang_block(const GangDesc* descs, const int* block_sys, int blk, double* lds,
-                                           SolveLDS& S) {
+                                           long lds_cap, SolveLDS& S) {
   const GangDesc& D = descs[block_sys[blk]];
   const int g = blk - D.first;
-  GangCtx C{D, g, D.G, D.t1 + (long)g * D.tmax, D.t2 + (long)g * D.tmax, lds, S};
+  GangCtx C{D, g, D.G, D.t1 + (long)g * D.tmax, D.t2 + (long)g * D.tmax, lds, lds_cap, S};
   gang_als(C);
 }
 

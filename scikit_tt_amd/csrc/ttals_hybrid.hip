@@ -20,7 +20,7 @@ __global__ __launch_bounds__(NT) void k_hybrid(const GangDesc* gdescs, const int
   __shared__ SolveLDS S;
   const int blk = (int)blockIdx.x;
   if (blk < gang_blocks)
-    gang_block(gdescs, block_sys, blk, dyn_lds, S);
+    gang_block(gdescs, block_sys, blk, dyn_lds, arena, S);
   else
     als_block(descs[blk - gang_blocks], dyn_lds, arena, S);
 }
