@@ -553,6 +553,8 @@ struct ttals_ctx {
   // on-chip arena of k_als in doubles: 61 KB, which with the ~18.4 KB of
   // static LDS keeps two systems resident per CU (160 KB of LDS)
   long arena = 61 * 1024 / 8;
+  bool arena_set = false;    // the caller chose the arena (ttals_set_arena_kb)
+  long arena_hybrid = 0;     // k_hybrid's arena in doubles (0: not yet sized)
   int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
   int matvec_variant = 0;    // SysDesc::mvar
   int schedule = 0;          // sle batches: 0 one workgroup per system, 1 gangs for the systems that set the launch time
@@ -1102,6 +1104,28 @@ bool dims_ok(std::initializer_list<int> v) {
   return true;
 }
 
+// On-chip arena of k_hybrid's workgroups, in doubles.  The kernel holds one
+// workgroup per CU (the gang path's registers), so its single-workgroup
+// systems can use the CU's LDS less the static part (1 KB of slack, at most
+// 136 KB): local problems of up to ~3000 unknowns stay on chip, against ~1500
+// in k_als's 61 KB.  An arena the caller set is kept as is.
+int hybrid_arena(ttals_ctx* c, long* out) {
+  if (c->arena_hybrid == 0) {
+    long doubles = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
+    if (!c->arena_set) {
+      int max_lds = 0;
+      size_t stat = 0;
+      HIPCHK(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+      HIPCHK(c, hybrid_static_lds(&stat));
+      const long bytes = std::min<long>((long)max_lds - (long)stat - 1024, 136L * 1024);
+      doubles = std::max(doubles, bytes / 8);
+    }
+    c->arena_hybrid = doubles;
+  }
+  *out = c->arena_hybrid;
+  return 0;
+}
+
 // Workgroups of k_hybrid (the kernel that runs every gang) that can be
 // resident at once on the whole device with `dyn_lds` bytes of dynamic LDS.
 int gang_capacity(ttals_ctx* c, size_t dyn_lds, int* cap) {
@@ -1379,8 +1403,9 @@ int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t*
   if (!(B.tol > 0.0)) return fail(c, TTALS_E_ARG, "tol must be > 0");
   if (B.maxit < 1) return fail(c, TTALS_E_ARG, "maxit must be >= 1");
   HIPCHK(c, hipSetDevice(c->device));
-  const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);
+  long arena = 0;
   int per_cu = 0, cus = 0, rc;
+  if ((rc = hybrid_arena(c, &arena))) return rc;
   HIPCHK(c, hybrid_occupancy(&per_cu, (size_t)arena * 8));
   HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
   const int slots = per_cu * cus;
@@ -1498,7 +1523,8 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   if (gang > 256) return fail(c, TTALS_E_ARG, "gang size must be <= 256");
   HIPCHK(c, hipSetDevice(c->device));
   int cap = 0, rc;
-  const long arena = std::max<long>(c->arena, GEMM_LDS_DOUBLES);  // GEMM tiles, then the split QR's factors
+  long arena = 0;  // GEMM tiles, then the split QR's factors
+  if ((rc = hybrid_arena(c, &arena))) return rc;
   if ((rc = gang_capacity(c, (size_t)arena * 8, &cap))) return rc;
   // pass 1: checks, gang sizes, totals
   Layout hl, dl;
@@ -1668,6 +1694,8 @@ int ttals_set_arena_kb(ttals_ctx* c, int kb) {
   if (!c || kb < 0 || kb > 61) return TTALS_E_ARG;
   if (c->inflight) return fail(c, TTALS_E_ARG, "a submission is pending on this context");
   c->arena = (long)kb * 1024 / 8;
+  c->arena_set = true;
+  c->arena_hybrid = 0;
   return 0;
 }
 

@@ -29,6 +29,13 @@ hipError_t hybrid_occupancy(int* per_cu, size_t dyn_lds) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void*>(k_hybrid), NT, dyn_lds);
 }
 
+hipError_t hybrid_static_lds(size_t* bytes) {
+  hipFuncAttributes fa;
+  const hipError_t e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_hybrid));
+  if (e == hipSuccess) *bytes = fa.sharedSizeBytes;
+  return e;
+}
+
 hipError_t hybrid_launch(int blocks, size_t dyn_lds, hipStream_t stream, const GangDesc* gdescs,
                          const int* block_sys, int gang_blocks, const SysDesc* descs, long arena) {
   hipLaunchKernelGGL(k_hybrid, dim3(blocks), dim3(NT), dyn_lds, stream, gdescs, block_sys, gang_blocks, descs, arena);
