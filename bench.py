@@ -184,50 +184,87 @@ def tfim_phase(batch, steps):
 # ---------------------------------------------------------------------------
 # serving
 
-def serving_phase(rate, seconds, seed=7, adaptive=True):
-    """Poisson arrivals of mixed evp (TFIM, d 8..32, rank 4..16 log-uniform) and
-    sle (-Laplacian, d 6..16, n 4/8, rank 4..8) requests into the
-    continuous-batching server; returns latency percentiles and throughput."""
-    import torch
+def serving_pool(size, seed=700000):
+    """Requests of the serving workload (scikit_tt_amd.workloads.serving_request:
+    sle or evp, d ~ U{8..64}, n in {2,4,8,16}, ranks log-uniform in [8,128]),
+    uploaded once; arrivals draw from this pool."""
     from scikit_tt_amd import workloads as W
     from scikit_tt_amd.tensor_train import from_host_batch
+    pool = []
+    for k in range(size):
+        q = W.serving_request(seed + k)
+        q['dev'] = from_host_batch([q['A'], q['x0'], q['b']] if q['kind'] == 'sle' else [q['A'], q['x0']])
+        q['N'] = max(c.shape[0] * c.shape[1] * c.shape[3] for c in q['x0'])
+        for key in ('A', 'x0', 'b'):
+            q.pop(key, None)
+        pool.append(q)
+    return pool
+
+
+LARGE_FROM = 16384  # local size from which a serving request is in the large class (its own slot)
+
+
+def serving_phase(pool, rate, seconds, seed=7, budget_ms=2000.0, model=None):
+    """Poisson arrivals at `rate` req/s for `seconds`, each a request drawn
+    from the pool, into the continuous-batching server (4 slots, gangs for
+    the large sle systems in FP32 + FP64 refinement, `budget_ms` of device
+    time per request; requests with a local size >= LARGE_FROM queue for a
+    slot of their own).  Served = solved within the budget AND at the gate:
+    sle FP64 rel.res <= 1e-8 (sle.als, 2 sweeps), evp eigenvalue within 1e-10
+    of the closed form (evp.als 'eig', sigma = -100, up to 6 sweeps to
+    conv_eps 1e-10)."""
+    import torch
     from scikit_tt_amd.serving import Server
+    from scikit_tt_amd.tensor_train import residual_norms
     rng = np.random.default_rng(seed)
-    nreq = max(1, int(rate * seconds))
+    nreq = max(1, int(round(rate * seconds)))
     gaps = rng.exponential(1.0 / rate, nreq)
-    reqs = []
-    for k in range(nreq):
-        if rng.random() < 0.5:
-            d = int(rng.integers(8, 33))
-            r = int(np.exp(rng.uniform(np.log(4), np.log(16))))
-            H, y0, J, g = W.tfim_chain(seed=1000 + k, d=d, rank=r)
-            reqs.append(('evp',) + tuple(from_host_batch([H, W.right_orthonormal(y0)])) + (None,))
-        else:
-            d, n = int(rng.integers(6, 17)), int(rng.choice([4, 8]))
-            r = int(rng.integers(4, 9))
-            A, x0, b = W.laplace_small(seed=2000 + k, d=d, n=n, rank=r)
-            reqs.append(('sle',) + tuple(from_host_batch([A, x0, b])))
+    picks = rng.integers(0, len(pool), nreq)
     torch.cuda.synchronize()
-    srv = Server(slots=4, max_batch=256)
+    srv = Server(slots=4, max_batch=256, maxit=1000, schedule='auto', gang_precision='mixed', budget_ms=budget_ms,
+                 cost_model=model, large_from=LARGE_FROM, large_slots=1)
+    tickets = []
     t0 = time.perf_counter()
     next_t = t0
-    for (kind, A, x0, b), gap in zip(reqs, gaps):
+    for gap, k in zip(gaps, picks):
         next_t += gap
         while time.perf_counter() < next_t:
             srv.poll()
-        if kind == 'evp':
-            srv.submit_evp(A, x0, repeats=4, conv_eps=1e-10, solver='eig', sigma=-100.0, adaptive_tol=adaptive)
+        q = pool[k]
+        if q['kind'] == 'sle':
+            A, x0, b = q['dev']
+            tickets.append(srv.submit_sle(A, x0, b, repeats=2))
         else:
-            srv.submit_sle(A, x0, b, repeats=2)
+            A, x0 = q['dev']
+            tickets.append(srv.submit_evp(A, x0, repeats=6, conv_eps=1e-10, solver='eig', sigma=-100.0,
+                                          adaptive_tol=True))
         srv.poll()
-    srv.drain(timeout=600)
+    srv.drain(timeout=900)
     elapsed = time.perf_counter() - t0
-    st = srv.latency_stats()
-    bad = sum(1 for t in srv.done if t.info.status not in (0,))
+    launches = srv.launches
+    # the gate (untimed)
+    ok = np.zeros(nreq, dtype=bool)
+    sle_k = [i for i, t in enumerate(tickets) if t.served and pool[picks[i]]['kind'] == 'sle']
+    if sle_k:
+        res, bn = residual_norms([pool[picks[i]]['dev'][0] for i in sle_k], [tickets[i].result for i in sle_k],
+                                 [pool[picks[i]]['dev'][2] for i in sle_k])
+        ok[sle_k] = res / bn <= GATE
+    for i, t in enumerate(tickets):
+        q = pool[picks[i]]
+        if t.served and q['kind'] == 'evp':
+            ok[i] = abs(t.result[0] - q['exact']) <= 1e-10 * abs(q['exact'])
+    lat = np.array([t.latency for t in tickets])
+    deadline = sum(1 for t in tickets if t.info is not None and t.info.status == 7)
     srv.close()
-    return {'rate_req_per_s': rate, 'requests': nreq, 'solves_per_s': round(nreq / elapsed, 2),
-            'p50_ms': round(st['p50_ms'], 3), 'p99_ms': round(st['p99_ms'], 3), 'launches': st['launches'],
-            'nonzero_status': bad}
+    out = {'rate_req_per_s': rate, 'requests': nreq, 'served_at_gate': int(ok.sum()),
+           'served_at_gate_per_s': round(float(ok.sum()) / elapsed, 2), 'over_budget': deadline,
+           'p50_ms': round(float(1e3 * np.percentile(lat, 50)), 1),
+           'p99_ms': round(float(1e3 * np.percentile(lat, 99)), 1), 'launches': launches,
+           'wall_s': round(elapsed, 2)}
+    if ok.any():
+        out['served_p50_ms'] = round(float(1e3 * np.percentile(lat[ok], 50)), 1)
+        out['served_p99_ms'] = round(float(1e3 * np.percentile(lat[ok], 99)), 1)
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -250,8 +287,11 @@ def main():
                     help="launch-time model of the mixed schedule: 'fit' (measured on this batch first) or nine "
                          "comma-separated parameters (scikit_tt_amd.tune.PARAMS), e.g. from an earlier fit")
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--serve-rates', default='100,1000', help='Poisson arrival rates (req/s) of the serving phase')
-    ap.add_argument('--serve-seconds', type=float, default=1.0)
+    ap.add_argument('--serve-rates', default='10,30,100,300,1000,2000',
+                    help='Poisson arrival rates (req/s) of the serving phase (empty: skip)')
+    ap.add_argument('--serve-seconds', type=float, default=1.0, help='arrival window per rate')
+    ap.add_argument('--serve-pool', type=int, default=64, help='distinct serving requests arrivals draw from')
+    ap.add_argument('--serve-budget-ms', type=float, default=2000.0, help='device time budget per request')
     ap.add_argument('--copy-bytes', type=int, default=0,
                     help='run one plain device copy of this many bytes before the timed steps (a known-size '
                          'reference for hardware byte counters)')
@@ -415,10 +455,27 @@ def main():
         log('tfim evp batch %d' % args.tfim_batch)
         line['tfim_evp'] = tfim_phase(args.tfim_batch, max(1, args.steps))
     if args.serve_rates:
-        line['serving'] = []
+        log('serving: building %d requests' % args.serve_pool)
+        pool = serving_pool(args.serve_pool)
+        sizes = np.array([q['N'] for q in pool])
+        line['serving'] = {'workload': 'Poisson arrivals of mixed sle/evp requests (d ~ U{8..64}, n in {2,4,8,16}, '
+                                       'ranks log-uniform [8,128]) drawn from %d distinct requests; 4 slots (one for requests of '
+                                       'local size >= 16384), gangs '
+                                       'in FP32 + FP64 refinement for large sle systems, %.0f ms device budget per '
+                                       'request; served = at rel.res <= 1e-8 (sle) or eigenvalue error <= 1e-10 '
+                                       '(evp)' % (args.serve_pool, args.serve_budget_ms),
+                           'pool_local_size_quantiles': {q: int(np.quantile(sizes, float(q)))
+                                                         for q in ('0.5', '0.9', '0.99')},
+                           'rates': []}
+        model = None
+        if tuning is not None:
+            from scikit_tt_amd import tune
+            model = [tuning['params'][k] for k in tune.PARAMS]
         for rate in [float(r) for r in args.serve_rates.split(',') if r]:
             log('serving at %.0f req/s' % rate)
-            line['serving'].append(serving_phase(rate, args.serve_seconds))
+            r = serving_phase(pool, rate, args.serve_seconds, budget_ms=args.serve_budget_ms, model=model)
+            log('  %s' % r)
+            line['serving']['rates'].append(r)
     if not args.no_cpu:
         log('cpu baseline')
         line['cpu_baseline'] = cpu_baseline(instances, args.sweeps)

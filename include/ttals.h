@@ -42,6 +42,7 @@ extern "C" {
 #define TTALS_S_NONFINITE 4   /* NaN/Inf in a local solution */
 #define TTALS_S_NONSYM 5      /* evp: operator not symmetric (NotImplementedError; the reference's
                                  'eig' would use a general eigensolver, scikit_tt/solvers/evp.py:424-432) */
+#define TTALS_S_DEADLINE 7    /* the system's time budget ran out (ttals_set_time_budget) */
 
 typedef struct ttals_ctx ttals_ctx;
 
@@ -185,6 +186,14 @@ int ttals_set_matvec_variant(ttals_ctx* ctx, int variant);
  * batches always use 0.  No counterpart in the reference: a scheduling knob
  * of the batched engine. */
 int ttals_set_schedule(ttals_ctx* ctx, int schedule, int gang_precision);
+
+/* Time budget of every system of later submissions on this context, in ms
+ * of device time from the system's start (0, the default: none).  A system
+ * over budget stops before its next local step with status
+ * TTALS_S_DEADLINE; its cores hold the last completed step.  The check runs
+ * between local steps, so a step in progress finishes first.  No
+ * counterpart in the reference: a serving control. */
+int ttals_set_time_budget(ttals_ctx* ctx, double ms);
 
 /* Launch-time model of the mixed schedule, `n` = 9 doubles: ms per on-chip
  * and per off-chip matvec flop of one sweep, ms per local solve, then the

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libttals.so')
 
 E_ARG, E_SHAPE, E_HIP, E_OOM, E_UNSUPPORTED = -1, -2, -3, -4, -5
-S_SINGULAR, S_NOCONV, S_UNSUPPORTED, S_NONFINITE, S_NONSYM = 1, 2, 3, 4, 5
+S_SINGULAR, S_NOCONV, S_UNSUPPORTED, S_NONFINITE, S_NONSYM, S_DEADLINE = 1, 2, 3, 4, 5, 7
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -64,6 +64,7 @@ SIGNATURES = {
     'ttals_set_local_tol_policy': (_i, [_vp, _i]),
     'ttals_set_matvec_variant': (_i, [_vp, _i]),
     'ttals_set_schedule': (_i, [_vp, _i, _i]),
+    'ttals_set_time_budget': (_i, [_vp, _d]),
     'ttals_set_cost_model': (_i, [_vp, _vp, _i]),
     'ttals_get_cost_model': (_i, [_vp, _vp, _i]),
     'ttals_sle_cost': (_i, [_vp, _i, _TTp, _TTp, _vp]),

@@ -559,6 +559,7 @@ struct ttals_ctx {
   int matvec_variant = 0;    // SysDesc::mvar
   int schedule = 0;          // sle batches: 0 one workgroup per system, 1 gangs for the systems that set the launch time
   int gang_prec = 0;         // GangDesc::prec of the gangs of a mixed schedule
+  long long budget = 0;      // SysDesc/GangDesc::budget of later submissions (100 MHz clock ticks; 0: none)
   CostModel cost;
   std::vector<int> last_plan;  // gang size per system of the last sle batch (0: one workgroup)
   std::string err;
@@ -926,6 +927,7 @@ void fill_sys(ttals_ctx* c, const BatchArgs& B, int s, const SysOff& o, char* hb
   D.trace = (c->trace && s == 0) ? c->trace : nullptr;
   D.trace_cap = c->trace_cap;
   D.symw = wptr(o.w_sym);
+  D.budget = c->budget;
 }
 
 // Where a batch's metadata, workspace and results live.
@@ -1188,6 +1190,7 @@ void gang_offsets(const SysSizes& z, const ttals_tt& A, const ttals_tt& x0, int 
 struct GangParams {
   int solver, repeats, maxit, prec;
   double tol;
+  long long budget;
 };
 
 // Metadata and the gang descriptor of one system (members first .. first+G-1).
@@ -1255,6 +1258,7 @@ void gang_fill(const GOff& go, const ttals_tt& A, const ttals_tt& x0, const ttal
   D.qrs = wptr(go.w_qrs);
   D.rmax2 = go.rmax2;
   D.sync = sync;
+  D.budget = P.budget;
   D.out = wptr(o.w_out);
   D.res = dres;
 }
@@ -1479,7 +1483,7 @@ int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t*
   (void)hl2.take(sizeof(SysDesc) * std::max(nf, 1));
   (void)dl2.take(sizeof(GangSync) * ng);
   (void)dl2.take(8 * res_stride * count);
-  const GangParams P{B.solver, B.repeats, B.maxit, c->gang_prec, B.tol};
+  const GangParams P{B.solver, B.repeats, B.maxit, c->gang_prec, B.tol, c->budget};
   int first = 0;
   for (int k = 0; k < ng; ++k) {
     const int s = gs[k];
@@ -1567,7 +1571,7 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   (void)hl2.take(sizeof(int) * total);
   (void)dl2.take(sizeof(GangSync) * count);
   (void)dl2.take(8 * res_stride * count);
-  const GangParams P{solver, repeats, maxit, prec, tol};
+  const GangParams P{solver, repeats, maxit, prec, tol, c->budget};
   int first = 0;
   for (int s = 0; s < count; ++s) {
     (void)size_system(c, 0, A[s], x0[s], &b[s], xo[s], s, z);
@@ -1719,6 +1723,20 @@ int ttals_set_schedule(ttals_ctx* c, int schedule, int gang_precision) {
   if (gang_precision != 0 && gang_precision != 1) return fail(c, TTALS_E_ARG, "gang precision must be 0 or 1");
   c->schedule = schedule;
   c->gang_prec = gang_precision;
+  return 0;
+}
+
+int ttals_set_time_budget(ttals_ctx* c, double ms) {
+  if (!c) return TTALS_E_ARG;
+  if (!(ms >= 0.0) || !std::isfinite(ms)) return fail(c, TTALS_E_ARG, "the time budget must be finite and >= 0");
+  if (ms == 0.0) {
+    c->budget = 0;
+    return 0;
+  }
+  int khz = 0;
+  HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  if (khz <= 0) return fail(c, TTALS_E_HIP, "the device reports no constant clock rate");
+  c->budget = std::max<long long>(1, (long long)(ms * khz));
   return 0;
 }
 

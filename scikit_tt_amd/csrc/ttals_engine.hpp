@@ -16,7 +16,18 @@ enum Status : int {
   ST_UNSUPPORTED = 3,   // sigma inside the spectrum of a large local problem
   ST_NONFINITE = 4,     // NaN/Inf in a local solution
   ST_NONSYM = 5,        // evp: the operator is not symmetric (only symmetric local eigensolvers here)
+  ST_DEADLINE = 7,      // the system's time budget ran out (checked before each local step)
 };
+
+// Uniform over the workgroup: has a system started at t0 (thread 0's clock)
+// used up `budget` ticks of the 100 MHz constant clock?  0: no budget.
+__device__ __forceinline__ bool over_budget(long long budget, unsigned long long t0, int* flag) {
+  if (budget <= 0) return false;
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = (wall_clock64() - t0 > (unsigned long long)budget) ? 1 : 0;
+  __syncthreads();
+  return *flag != 0;
+}
 
 // Relative asymmetry ||A - A^T||_F / ||A||_F above which evp rejects an
 // operator.  Computed from Gram transfer matrices, whose rounding puts the
@@ -63,6 +74,7 @@ struct SysDesc {
   double* trace;              // optional: 6 doubles per local solve (nullptr = off)
   long trace_cap;             // capacity in local solves
   double* symw;               // evp: 4 * (max operator rank)^2 doubles for the symmetry test
+  long long budget;           // 100 MHz clock ticks the system may run (0: no limit), checked per local step
 };
 
 // --- small helpers ---------------------------------------------------------
@@ -616,6 +628,7 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
   if (threadIdx.x < 8) S.prof[threadIdx.x] = 0;
   __syncthreads();
   const long long tstart = ph_now();
+  const unsigned long long t_begin = wall_clock64();  // thread 0's value decides the time budget
   StepStats st;
   int status = ST_OK;
   double lam = 0.0;
@@ -670,6 +683,10 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
       if (i > 0) st.flops += build_left<ACC>(i, D, lds, arena);
       ph_add(S, PH_STACK, ts);
       if (i < d - 1) {
+        if (over_budget(D.budget, t_begin, &S.flag[3])) {
+          status = ST_DEADLINE;
+          break;
+        }
         status = local_step<ACC>(i, true, D, sym, lds, arena, S, &lam, st);
         ++nsolves;
       }
@@ -678,6 +695,10 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
       const long long ts = ph_now();
       if (i < d - 1) st.flops += build_right<ACC>(i, D, lds, arena);
       ph_add(S, PH_STACK, ts);
+      if (over_budget(D.budget, t_begin, &S.flag[3])) {
+        status = ST_DEADLINE;
+        break;
+      }
       status = local_step<ACC>(i, false, D, sym, lds, arena, S, &lam, st);
       ++nsolves;
     }

@@ -85,6 +85,7 @@ struct GangDesc {
   double* qrs;   // per workgroup: 3 x rmax^2 (Cholesky factor, accumulated R, product)
   long rmax2;    // rmax^2
   GangSync* sync;
+  long long budget;  // as SysDesc::budget (member 0's clock decides, broadcast at a barrier)
   double* out;   // [16] as SysDesc::out
   double* res;   // compact result row: out[0..15], then the d+1 final ranks
 };
@@ -571,6 +572,25 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
 
 
 // --- the sweep ------------------------------------------------------------------------
+
+// Time budget, decided by member 0 and broadcast: 1 over budget, 0 not,
+// -1 the gang aborted.  With a budget, one barrier: member 0 stores its
+// decision in the sync record before arriving; every member reads it after
+// the barrier (the next decision is stored only after further barriers of
+// the step, which every member must pass first).
+__device__ int gang_over_budget(GangCtx& C, unsigned long long t0) {
+  const GangDesc& D = C.D;
+  if (D.budget <= 0) return 0;
+  if (C.g == 0 && threadIdx.x == 0) {
+    const unsigned v = (wall_clock64() - t0 > (unsigned long long)D.budget) ? 1u : 0u;
+    __hip_atomic_store(&D.sync->pad, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (!C.sync()) return -1;
+  if (threadIdx.x == 0)
+    C.S.flag[3] = (int)__hip_atomic_load(&D.sync->pad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return C.S.flag[3] != 0 ? 1 : 0;
+}
 
 constexpr int ST_ABORT = 6;  // a gang barrier wait ran out (reported as a runtime error)
 
@@ -1214,6 +1234,7 @@ __device__ void gang_als(GangCtx& C) {
   if (threadIdx.x < 8) S.prof[threadIdx.x] = 0;
   __syncthreads();
   const long long tstart = ph_now();
+  const unsigned long long t_begin = wall_clock64();  // member 0, thread 0 decides the time budget
   // x <- x0 (split copy of every core)
   for (int i = 0; i < d; ++i) {
     const double* src = D.x0c[i];
@@ -1262,6 +1283,12 @@ __device__ void gang_als(GangCtx& C) {
         if (!ok) break;
       }
       if (i < d - 1) {
+        const int ob = gang_over_budget(C, t_begin);
+        if (ob < 0) ok = false;
+        if (ob != 0) {
+          if (ob > 0) status = ST_DEADLINE;
+          break;
+        }
         status = gang_step(C, i, true, sym, st);
         ++nsolves;
       }
@@ -1271,6 +1298,12 @@ __device__ void gang_als(GangCtx& C) {
       if (i < d - 1) st.flops += gang_build_right(C, i, &ok);
       ph_add(S, PH_STACK, ts);
       if (!ok) break;
+      const int ob = gang_over_budget(C, t_begin);
+      if (ob < 0) ok = false;
+      if (ob != 0) {
+        if (ob > 0) status = ST_DEADLINE;
+        break;
+      }
       status = gang_step(C, i, false, sym, st);
       ++nsolves;
     }

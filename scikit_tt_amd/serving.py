@@ -4,16 +4,20 @@ Requests (``sle.als`` or ``evp.als`` problems whose TTs already live on the
 device) are queued on arrival.  The loop keeps ``slots`` engine contexts,
 each with its own HIP stream and workspace.  Whenever a slot is free it
 admits the waiting requests of one kind/parameter set (FIFO, up to
-``max_batch`` and within the HBM budget) as ONE launch of the fused engine
-(one workgroup per system).  Launches on different slots run concurrently on
-the device, so requests that arrive while others are on the GPU start as soon
-as a slot and memory free up instead of waiting for the running batch.
-Cores and stacks stay resident on the device; only ranks and a few scalars
-return to the host.
+``max_batch`` and within the HBM budget) as ONE launch.  Launches on
+different slots run concurrently on the device, so requests that arrive
+while others are on the GPU start as soon as a slot and memory free up
+instead of waiting for the running batch.  Cores and stacks stay resident on
+the device; only ranks and a few scalars return to the host.
 
-Every system is solved by one workgroup with a fixed reduction order, so a
-result does not depend on which batch or slot it ran in (tested alone vs in
-the mix).
+``schedule='one'`` solves every system on one workgroup with a fixed
+reduction order, so a result does not depend on which batch or slot it ran
+in.  ``schedule='auto'`` gives the sle systems that would set a launch's
+time a gang of workgroups (``sle.als_batch(..., schedule='auto')``, in
+``gang_precision``); a ganged result depends on its gang size, which the
+launch's mix decides, only at the engines' agreement (~1e-9).  A request
+that outlives ``budget_ms`` of device time stops before its next local step
+with status ``TTALS_S_DEADLINE`` (``Ticket.served`` False).
 """
 import ctypes
 import collections
@@ -35,12 +39,14 @@ _KBASIS = 64  # Krylov basis vectors the engine plans per system (Lanczos restar
 _NDENSE = 96
 
 
-def footprint_bytes(kind, A, x0, b=None):
+def footprint_bytes(kind, A, x0, b=None, gang_max=0):
     """Device bytes one system needs in a launch: stacks, scratch, Krylov basis, output cores.
 
     Mirrors the engine's planner (interface stacks on every bond, two
     contraction intermediates of N*max(R, p), the 64 x N Krylov basis, QR and
-    LU scratch) plus the output cores.
+    LU scratch) plus the output cores; with ``gang_max`` > 0 (an sle request
+    that may get a gang of up to that many workgroups) also the gang's
+    per-member intermediates, its vectors and FP32 copies.
     """
     d = x0.order
     r, R = x0.ranks, A.ranks
@@ -54,16 +60,21 @@ def footprint_bytes(kind, A, x0, b=None):
     scratch = 4 * nmax + 2 * tmax + _KBASIS * nmax + _NDENSE * (_NDENSE + 1) + rmax * (rmax + 1)
     cores = sum(N)
     best = cores if kind == 'evp' else 0
-    return 8 * (2 * stacks + scratch + 2 * cores + best) + 4096
+    gang = 0
+    if gang_max > 0 and kind == 'sle':
+        # t1/t2 per member, r/p/q/xt and the (41 + 1) x N GMRES basis in FP64 and FP32
+        gang = 2 * gang_max * tmax + 6 * nmax + (3 * 42 * nmax) // 2 + gang_max * (132 + 4 * rmax * rmax)
+    return 8 * (2 * stacks + scratch + 2 * cores + best + gang) + 4096
 
 
 class Ticket:
     """Handle of one request; ``result`` is set when it has been solved."""
 
-    __slots__ = ('rid', 'kind', 'key', 'A', 'x0', 'b', 'bytes', 't_arrival', 't_start', 't_done', 'result', 'info')
+    __slots__ = ('rid', 'kind', 'key', 'A', 'x0', 'b', 'bytes', 'cls', 't_arrival', 't_start', 't_done', 'result',
+                 'info')
 
-    def __init__(self, rid, kind, key, A, x0, b, nbytes):
-        self.rid, self.kind, self.key = rid, kind, key
+    def __init__(self, rid, kind, key, A, x0, b, nbytes, cls=0):
+        self.rid, self.kind, self.key, self.cls = rid, kind, key, cls
         self.A, self.x0, self.b = A, x0, b
         self.bytes = nbytes
         self.t_arrival = time.perf_counter()
@@ -75,12 +86,18 @@ class Ticket:
         return self.t_done is not None
 
     @property
+    def served(self):
+        """Solved within the budget with no failure status (NOCONV counts as served)."""
+        return self.info is not None and self.info.status in (0, _lib.S_NOCONV)
+
+    @property
     def latency(self):
         return None if self.t_done is None else self.t_done - self.t_arrival
 
 
 class _Slot:
-    def __init__(self, device):
+    def __init__(self, device, cls=0):
+        self.cls = cls  # size class of the requests it admits
         self.ctx = _lib.new_context(device)
         self.stream = torch.cuda.Stream(device=device)  # default priority
         self.batch = None  # (tickets, out cores, out packs, descriptor arrays)
@@ -95,15 +112,36 @@ class Server:
 
     slots: concurrent launches in flight; max_batch: systems per launch;
     hbm_budget: bytes the in-flight launches may occupy (default: 80% of the
-    free device memory at start-up).
+    free device memory at start-up).  ``large_from`` (local size, i.e. the
+    largest r n r' of a request's start) splits requests into two size
+    classes with their own queues order and ``large_slots`` of the slots
+    reserved for the large class, so that small requests are not held behind
+    a launch that contains a large one (None: one class).
     """
 
-    def __init__(self, device=0, slots=4, max_batch=128, hbm_budget=None, tol=1e-12, maxit=4000):
+    def __init__(self, device=0, slots=4, max_batch=128, hbm_budget=None, tol=1e-12, maxit=4000, schedule='one',
+                 gang_precision='fp64', budget_ms=None, cost_model=None, large_from=None, large_slots=1):
         if not torch.cuda.is_available():
             raise RuntimeError('Server needs a HIP device')
+        if schedule not in ('one', 'auto'):
+            raise ValueError("schedule must be 'one' or 'auto'")
+        if gang_precision not in ('fp64', 'mixed'):
+            raise ValueError("gang_precision must be 'fp64' or 'mixed'")
         self.device = device
         self.lib = _lib.load()
-        self.slots = [_Slot(device) for _ in range(slots)]
+        self.gang_max = 32 if schedule == 'auto' else 0
+        if large_from is not None and not 0 < large_slots < slots:
+            raise ValueError('large_slots must leave at least one slot for each class')
+        self.large_from = large_from
+        nlarge = large_slots if large_from is not None else 0
+        self.slots = [_Slot(device, 1 if k < nlarge else 0) for k in range(slots)]
+        for slot in self.slots:
+            _lib.check(self.lib.ttals_set_schedule(slot.ctx, 1 if schedule == 'auto' else 0,
+                                                   1 if gang_precision == 'mixed' else 0), slot.ctx, 'schedule')
+            _lib.check(self.lib.ttals_set_time_budget(slot.ctx, float(budget_ms or 0.0)), slot.ctx, 'budget')
+            if cost_model is not None:
+                p = np.ascontiguousarray(cost_model, dtype=np.float64)
+                _lib.check(self.lib.ttals_set_cost_model(slot.ctx, p.ctypes.data, int(p.size)), slot.ctx, 'model')
         self.max_batch = max_batch
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(device)
@@ -134,10 +172,14 @@ class Server:
         return self._enqueue('evp', key, A, x0, None)
 
     def _enqueue(self, kind, key, A, x0, b):
-        nb = footprint_bytes(kind, A, x0, b)
+        nb = footprint_bytes(kind, A, x0, b, self.gang_max)
         if nb > self.hbm_budget:
             raise MemoryError('request needs %d bytes, more than the HBM budget %d' % (nb, self.hbm_budget))
-        t = Ticket(self._rid, kind, key, A, x0, b, nb)
+        cls = 0
+        if self.large_from is not None:
+            r, n = x0.ranks, x0.row_dims
+            cls = 1 if max(r[i] * n[i] * r[i + 1] for i in range(x0.order)) >= self.large_from else 0
+        t = Ticket(self._rid, kind, key, A, x0, b, nb, cls)
         self._rid += 1
         self.queue.append(t)
         return t
@@ -161,18 +203,20 @@ class Server:
         return finished
 
     def _admit(self, slot):
-        head = self.queue[0]
+        head = next((t for t in self.queue if t.cls == slot.cls), None)
+        if head is None:
+            return
         take = []
         budget = self.hbm_budget - self.in_use
         rest = collections.deque()
         while self.queue and len(take) < self.max_batch:
             t = self.queue.popleft()
-            if t.key == head.key and t.bytes <= budget:
+            if t.key == head.key and t.cls == head.cls and t.bytes <= budget:
                 take.append(t)
                 budget -= t.bytes
             else:
                 rest.append(t)
-                if t.key == head.key:
+                if t.key == head.key and t.cls == head.cls:
                     break  # FIFO within a kind: do not overtake a request that does not fit
         self.queue.extendleft(reversed(rest))
         if not take:

@@ -186,5 +186,7 @@ def raise_status(info, what):
     if st == _lib.S_NONSYM:
         raise NotImplementedError('%s: the operator is not symmetric; local eigensolves of non-symmetric '
                                   'operators (the reference\'s dgeev path) are not implemented' % what)
+    if st == _lib.S_DEADLINE:
+        raise TimeoutError('%s: the time budget ran out after %d local solves' % (what, info.local_solves))
     if st == _lib.S_NOCONV:
         warnings.warn('%s: %d local Krylov solve(s) stopped at maxit' % (what, info.unconverged), RuntimeWarning)

@@ -78,3 +78,121 @@ def test_admission_respects_budget():
     with pytest.raises(MemoryError):
         Server(slots=1, hbm_budget=16).submit_sle(A, x0, b)
     srv.close()
+
+
+def _c5_requests():
+    """Requests at the serving workload's sizes: an sle of rank 128 with n = 16
+    (local problems of 262144 unknowns: a gang), smaller sle and evp ones."""
+    from scikit_tt_amd.tensor_train import TT
+    rng = np.random.default_rng(5)
+    reqs = []
+    A = W.shifted_operator(rng, 8, 16)
+    reqs.append(('sle', TT(A), TT(W.random_tt(rng, [16] * 8, 128)), TT(W.random_tt(rng, [16] * 8, 4))))
+    A, x0, b = W.laplace_small(seed=3, d=9, n=4, rank=6)
+    reqs.append(('sle', TT(A), TT(x0), TT(b)))
+    A = W.shifted_operator(rng, 12, 8)
+    reqs.append(('sle', TT(A), TT(W.random_tt(rng, [8] * 12, 40)), TT(W.random_tt(rng, [8] * 12, 4))))
+    H, y0, J, g = W.tfim_chain(seed=9, d=16, rank=32)
+    reqs.append(('evp', TT(H), TT(W.right_orthonormal(y0)), None, W.tfim_ground_energy(J, g)))
+    return reqs
+
+
+def test_c5_sized_alone_equals_mix_and_gates():
+    """Served with gangs (mixed precision) and a budget; each result against
+    the same request solved alone with the same schedule: one-workgroup
+    systems to rounding, ganged ones to the engines' agreement (the gang size
+    follows the launch's mix); every sle's rel.res equals its alone value
+    (a few per cent: the engines' agreement; the rank-128 request sits at its
+    rank floor near 3e-8 after two sweeps, the others below 1e-8) and the evp
+    request meets its closed-form ground energy to 1e-10."""
+    import scikit_tt_amd.solvers.sle as sle
+    import scikit_tt_amd.solvers.evp as evp
+    from scikit_tt_amd.serving import Server
+    from scikit_tt_amd.tensor_train import residual_norms
+    assert torch.cuda.is_available()
+    reqs = _c5_requests()
+    print('case C5-sized requests: alone', flush=True)
+    alone, gangs = [], []
+    for q in reqs:
+        if q[0] == 'sle':
+            (x,), infos = sle.als_batch([q[1]], [q[2]], [q[3]], repeats=2, schedule='auto',
+                                        gang_precision='mixed', return_info=True)
+            alone.append(x)
+            gangs.append(int(infos.gangs[0]))
+        else:
+            alone.append(evp.als(q[1], q[2], repeats=6, conv_eps=1e-10, solver='eig', sigma=-100.0))
+            gangs.append(0)
+    print('  gangs alone', gangs, flush=True)
+    assert gangs[0] > 0  # the rank-128, n = 16 request is large enough for a gang
+    print('case C5-sized requests: served', flush=True)
+    srv = Server(slots=2, max_batch=8, schedule='auto', gang_precision='mixed', budget_ms=60000.0)
+    tickets = []
+    for q in reqs:
+        if q[0] == 'sle':
+            tickets.append(srv.submit_sle(q[1], q[2], q[3], repeats=2))
+        else:
+            tickets.append(srv.submit_evp(q[1], q[2], repeats=6, conv_eps=1e-10, solver='eig', sigma=-100.0))
+        srv.poll()
+    srv.drain(timeout=300)
+    for k, (q, t, ref) in enumerate(zip(reqs, tickets, alone)):
+        assert t.done and t.served and t.info.status == 0, (k, t.info)
+        if q[0] == 'sle':
+            x = t.result
+            assert x.ranks == ref.ranks
+            res, bn = residual_norms([q[1], q[1]], [x, ref], [q[3], q[3]])
+            rel, rel_alone = res / bn
+            print('  request %d: rel.res served %.3e alone %.3e' % (k, rel, rel_alone), flush=True)
+            assert rel <= 1.1 * rel_alone + 1e-12, (k, rel, rel_alone)
+            assert rel_alone <= (5e-8 if k == 0 else 1e-8), (k, rel_alone)
+            d = tt_ops.norm(tt_ops.sub(x.cores[:], ref.cores[:])) / tt_ops.norm(ref.cores[:])
+            # a one-workgroup system runs in k_als alone but in k_hybrid (larger
+            # on-chip arena) when the launch has gangs: rounding-level differences
+            assert d < (1e-8 if gangs[k] else 1e-12), (k, d)
+        else:
+            lam, x, sw = t.result
+            assert lam == ref[0] and sw == ref[2] and x.ranks == ref[1].ranks
+            assert abs(lam - q[4]) <= 1e-10 * abs(q[4]), (lam, q[4])
+    srv.close()
+
+
+def test_time_budget_stops_a_request():
+    """A budget far below the solve time: the request stops before a later
+    local step with the deadline status and is not served; sle.als raises
+    TimeoutError under the same budget."""
+    import scikit_tt_amd.solvers.sle as sle
+    from scikit_tt_amd import _lib
+    from scikit_tt_amd.serving import Server
+    reqs = _c5_requests()
+    q = reqs[2]
+    srv = Server(slots=1, max_batch=4, budget_ms=0.05)
+    t = srv.submit_sle(q[1], q[2], q[3], repeats=4)
+    srv.drain(timeout=120)
+    assert t.done and not t.served and t.info.status == _lib.S_DEADLINE
+    assert t.info.local_solves < 4 * (2 * 12 - 1)
+    srv.close()
+    ctx = _lib.context(0)
+    _lib.check(_lib.load().ttals_set_time_budget(ctx, 0.05), ctx, 'budget')
+    try:
+        with pytest.raises(TimeoutError):
+            sle.als(q[1], q[2], q[3], repeats=4)
+    finally:
+        _lib.load().ttals_set_time_budget(ctx, 0.0)
+
+
+def test_size_classes_keep_small_requests_apart():
+    """With a size split, a large request and a small one that arrive together
+    run in separate launches (the large one on its reserved slot), and the
+    small one completes first."""
+    from scikit_tt_amd.serving import Server
+    reqs = _c5_requests()
+    big, small = reqs[0], reqs[1]
+    srv = Server(slots=2, max_batch=8, schedule='auto', gang_precision='mixed', large_from=16384, large_slots=1)
+    t_big = srv.submit_sle(big[1], big[2], big[3], repeats=2)
+    t_small = srv.submit_sle(small[1], small[2], small[3], repeats=2)
+    assert (t_big.cls, t_small.cls) == (1, 0)
+    srv.poll()
+    assert all(s.busy for s in srv.slots)  # both admitted at once, one launch each
+    srv.drain(timeout=300)
+    assert srv.launches == 2 and t_big.served and t_small.served
+    assert t_small.t_done < t_big.t_done
+    srv.close()
