@@ -19,6 +19,7 @@
 using namespace ttals;
 
 #define TTALS_VERSION 100
+#define TTALS_DENSE_EIG_MAX 2048  // order of the largest dense local eigenproblem
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -479,6 +480,151 @@ __global__ __launch_bounds__(NT) void k_svd_trunc(SvdArgs A) {
   }
   for (int j = threadIdx.x; j < kk; j += NT) A.s[j] = A.sig[A.perm[j]];
   if (threadIdx.x == 0) *A.kout = kk;
+}
+
+// Dense symmetric (A = A^T) or symmetric-definite generalized (A x = l M x,
+// M = M^T positive definite) eigenproblem of order N on one workgroup, for
+// the reference's dense local eigensolves of evp.als with operator_gevp,
+// previous/shift or number_ev > 1 (scikit_tt/solvers/evp.py:411-443):
+// M = L L^T (Cholesky), C = L^{-1} A L^{-T}, C + tau I with tau above
+// C's Gershgorin bound (positive definite, so its one-sided Jacobi SVD is its
+// eigendecomposition), eigenvalues sigma_j - tau, eigenvectors L^{-T} V.
+// Selection: mode 0 the k nearest `sigma` (ascending |l - sigma|), mode 1 the
+// k largest (descending).  unit: eigenvectors scaled to unit 2-norm (as
+// scipy.linalg.eig returns them), else M-orthonormal (as eigh).
+struct EigArgs {
+  int N, k, mode, unit;
+  double sigma;
+  const double* A;
+  const double* M;
+  double* C;
+  double* Lm;
+  double* V;
+  double* sig;
+  int* perm;
+  int* idx;
+  double* vals;
+  double* vecs;  // N x k, row-major
+  int* status;   // 0 ok, 1 M not positive definite, 2 non-finite
+};
+
+__global__ __launch_bounds__(NT) void k_dense_eig(EigArgs E) {
+  __shared__ double red[4 * NWAVE];
+  __shared__ double s_tau;
+  const int N = E.N;
+  const long NN = (long)N * N;
+  for (long e = threadIdx.x; e < NN; e += NT) {
+    const long i = e / N, j = e % N;
+    E.C[e] = 0.5 * (E.A[i * N + j] + E.A[j * N + i]);
+    if (E.M) E.Lm[e] = 0.5 * (E.M[i * N + j] + E.M[j * N + i]);
+  }
+  __syncthreads();
+  if (E.M) {
+    // M = L L^T, lower L in place (right-looking)
+    for (int j = 0; j < N; ++j) {
+      const double piv = E.Lm[(long)j * N + j];
+      if (!(piv > 0.0) || !isfinite(piv)) {
+        if (threadIdx.x == 0) *E.status = 1;
+        return;
+      }
+      const double ljj = sqrt(piv), inv = 1.0 / ljj;
+      __syncthreads();
+      for (int i = j + 1 + threadIdx.x; i < N; i += NT) E.Lm[(long)i * N + j] *= inv;
+      if (threadIdx.x == 0) E.Lm[(long)j * N + j] = ljj;
+      __syncthreads();
+      const int rest = N - j - 1;
+      for (long e = threadIdx.x; e < (long)rest * rest; e += NT) {
+        const int p = j + 1 + (int)(e / rest), q = j + 1 + (int)(e % rest);
+        if (q <= p) E.Lm[(long)p * N + q] -= E.Lm[(long)p * N + j] * E.Lm[(long)q * N + j];
+      }
+      __syncthreads();
+    }
+    // C <- L^{-1} C (forward substitution, one column per thread)
+    for (int col = threadIdx.x; col < N; col += NT)
+      for (int i = 0; i < N; ++i) {
+        double t = E.C[(long)i * N + col];
+        for (int q = 0; q < i; ++q) t -= E.Lm[(long)i * N + q] * E.C[(long)q * N + col];
+        E.C[(long)i * N + col] = t / E.Lm[(long)i * N + i];
+      }
+    __syncthreads();
+    // C <- C L^{-T} (one row per thread)
+    for (int row = threadIdx.x; row < N; row += NT)
+      for (int j = 0; j < N; ++j) {
+        double t = E.C[(long)row * N + j];
+        for (int q = 0; q < j; ++q) t -= E.C[(long)row * N + q] * E.Lm[(long)j * N + q];
+        E.C[(long)row * N + j] = t / E.Lm[(long)j * N + j];
+      }
+    __syncthreads();
+    for (long e = threadIdx.x; e < NN; e += NT) {
+      const long i = e / N, j = e % N;
+      if (j > i) {
+        const double v = 0.5 * (E.C[i * N + j] + E.C[j * N + i]);
+        E.C[i * N + j] = v;
+        E.C[j * N + i] = v;
+      }
+    }
+    __syncthreads();
+  }
+  // Gershgorin bound of |eigenvalues|, then the positive definite shift
+  double rmax = 0.0;
+  for (int i = threadIdx.x; i < N; i += NT) {
+    double sm = 0.0;
+    for (int j = 0; j < N; ++j) sm += fabs(E.C[(long)i * N + j]);
+    rmax = fmax(rmax, sm);
+  }
+  rmax = wave_max(rmax);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int q = 0; q < NWAVE; ++q) m = fmax(m, red[q]);
+    s_tau = m > 0.0 ? 1.0001 * m : 1.0;
+    if (!isfinite(m)) *E.status = 2;
+  }
+  __syncthreads();
+  if (!isfinite(s_tau)) return;
+  const double tau = s_tau;
+  for (int i = threadIdx.x; i < N; i += NT) E.C[(long)i * N + i] += tau;
+  __syncthreads();
+  wg_jacobi_svd(E.C, N, N, E.V, E.sig, E.perm, red);
+  // selection (thread 0; k is small)
+  if (threadIdx.x == 0) {
+    for (int j = 0; j < N; ++j) E.perm[j] = 0;  // reused as a taken-mark
+    for (int s = 0; s < E.k; ++s) {
+      int best = -1;
+      double bv = 0.0;
+      for (int j = 0; j < N; ++j) {
+        if (E.perm[j]) continue;
+        const double lam = E.sig[j] - tau;
+        const double key = E.mode == 0 ? fabs(lam - E.sigma) : -lam;
+        if (best < 0 || key < bv) {
+          best = j;
+          bv = key;
+        }
+      }
+      E.perm[best] = 1;
+      E.idx[s] = best;
+      E.vals[s] = E.sig[best] - tau;
+    }
+  }
+  __syncthreads();
+  // eigenvectors: column idx[s] of V, then L^{-T} (one vector per thread)
+  for (int s = threadIdx.x; s < E.k; s += NT) {
+    const int j = E.idx[s];
+    for (int i = 0; i < N; ++i) E.vecs[(long)i * E.k + s] = E.V[(long)i * N + j];
+    if (E.M)
+      for (int i = N - 1; i >= 0; --i) {
+        double t = E.vecs[(long)i * E.k + s];
+        for (int q = i + 1; q < N; ++q) t -= E.Lm[(long)q * N + i] * E.vecs[(long)q * E.k + s];
+        E.vecs[(long)i * E.k + s] = t / E.Lm[(long)i * N + i];
+      }
+    if (E.unit) {
+      double nn = 0.0;
+      for (int i = 0; i < N; ++i) nn += E.vecs[(long)i * E.k + s] * E.vecs[(long)i * E.k + s];
+      const double sc = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+      for (int i = 0; i < N; ++i) E.vecs[(long)i * E.k + s] *= sc;
+    }
+  }
 }
 
 // Supercore of two neighbouring cores:
@@ -2223,6 +2369,51 @@ int ttals_truncated_svd(ttals_ctx* c, void* stream, int m, int cdim, const doubl
   HIPCHK(c, hipMemcpyAsync(&h, a.kout, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
   *k = h;
+  return 0;
+}
+
+int ttals_dense_eig(ttals_ctx* c, void* stream, int N, const double* A, const double* M, int k, int mode,
+                    double sigma, int unit, double* vals, double* vecs) {
+  if (!c) return TTALS_E_ARG;
+  if (N < 1 || N > TTALS_DENSE_EIG_MAX || !A || k < 1 || k > N || (mode != 0 && mode != 1) || !vals || !vecs)
+    return fail(c, TTALS_E_ARG, "bad dense_eig arguments (1 <= k <= N <= " + std::to_string(TTALS_DENSE_EIG_MAX) + ")");
+  if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
+  HIPCHK(c, hipSetDevice(c->device));
+  const long NN = (long)N * N;
+  Layout L;
+  const size_t oC = L.take(8 * NN), oL = L.take(M ? 8 * NN : 8), oV = L.take(8 * NN), osig = L.take(8 * N),
+               operm = L.take(4 * N), oidx = L.take(4 * k), ost = L.take(8);
+  int rc;
+  if ((rc = ensure_dws(c, L.bytes))) return rc;
+  char* db = reinterpret_cast<char*>(c->dws);
+  EigArgs e;
+  e.N = N;
+  e.k = k;
+  e.mode = mode;
+  e.unit = unit ? 1 : 0;
+  e.sigma = sigma;
+  e.A = A;
+  e.M = M;
+  e.C = reinterpret_cast<double*>(db + oC);
+  e.Lm = reinterpret_cast<double*>(db + oL);
+  e.V = reinterpret_cast<double*>(db + oV);
+  e.sig = reinterpret_cast<double*>(db + osig);
+  e.perm = reinterpret_cast<int*>(db + operm);
+  e.idx = reinterpret_cast<int*>(db + oidx);
+  e.vals = vals;
+  e.vecs = vecs;
+  e.status = reinterpret_cast<int*>(db + ost);
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipMemsetAsync(e.status, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_dense_eig, dim3(1), dim3(NT), 0, st, e);
+  HIPCHK(c, hipGetLastError());
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, e.status, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (h == 1)
+    return fail(c, TTALS_E_UNSUPPORTED, "the local matrix of operator_gevp is not symmetric positive definite "
+                                        "(only symmetric-definite generalized eigenproblems are supported)");
+  if (h == 2) return ST_NONFINITE;
   return 0;
 }
 

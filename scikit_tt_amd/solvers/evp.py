@@ -1,7 +1,10 @@
 """ALS for the extreme/nearest eigenpair in TT format on the HIP device.
 
-Drop-in for scikit_tt.solvers.evp.als (scikit_tt/solvers/evp.py:17-179) for
-``number_ev=1``, ``operator_gevp=None``, ``previous=[]`` and real data.
+Drop-in for scikit_tt.solvers.evp.als (scikit_tt/solvers/evp.py:17-179)
+for real data: ``number_ev=1`` without deflation or ``operator_gevp`` runs
+on the fused engine (below); ``previous``/``shift``, ``operator_gevp`` and
+``number_ev > 1`` run the reference's dense-local-matrix sweep step by step
+on the device (``_als_general``).
 Same signature, defaults and return convention ``(eigenvalue, TT,
 iterations)``, the best-so-far pair by |lambda - sigma| (evp.py:151-155)
 and the same convergence rule (evp.py:158-165).
@@ -27,12 +30,8 @@ ADAPTIVE_CAP_EXP = 9  # adaptive local tolerance never looser than 1e-9
 
 
 def _check_options(previous, operator_gevp, number_ev, solver, real):
-    if previous:
-        raise NotImplementedError('previous/shift deflation is not implemented in scikit_tt_amd')
-    if operator_gevp is not None:
-        raise NotImplementedError('generalised eigenproblems (operator_gevp) are not implemented in scikit_tt_amd')
-    if number_ev != 1:
-        raise NotImplementedError('number_ev > 1 (block TT) is not implemented in scikit_tt_amd')
+    if int(number_ev) < 1:
+        raise ValueError('number_ev must be >= 1')
     if solver not in _SOLVERS:
         raise NotImplementedError("solver must be 'eig' or 'eigh', got %r" % (solver,))
     if real is not True:
@@ -82,13 +81,206 @@ def als(operator, initial_guess, previous=[], shift=0, operator_gevp=None, numbe
         conv_eps=1e-10, solver='eig', sigma=1, real=True):
     """evp.als (scikit_tt/solvers/evp.py:17-179) on the device.
 
-    Unsupported options raise NotImplementedError instead of running a
-    second (CPU) path.
+    Without ``previous``, ``operator_gevp`` and with ``number_ev=1`` the
+    whole solve is one launch of the fused engine (matrix-free Lanczos local
+    eigensolves).  With any of them the sweep runs as the reference's does,
+    from the host, piece by piece on the device (``_als_general``): dense
+    local matrices, a dense local eigensolve, the SVD of the local
+    eigenvectors.  Unsupported options raise NotImplementedError instead of
+    running a second (CPU) path.
     """
     _check_options(previous, operator_gevp, number_ev, solver, real)
-    _engine.check_tts([operator, initial_guess])
+    _engine.check_tts([operator, initial_guess] + list(previous) +
+                      ([operator_gevp] if operator_gevp is not None else []))
+    if previous or operator_gevp is not None or int(number_ev) != 1:
+        return _als_general(operator, initial_guess, list(previous), float(shift), operator_gevp, int(number_ev),
+                            int(repeats), float(conv_eps), solver, float(sigma))
     return als_batch([operator], [initial_guess], repeats=repeats, conv_eps=conv_eps, solver=solver,
                      sigma=sigma)[0]
+
+
+def _als_general(operator, initial_guess, previous, shift, operator_gevp, number_ev, repeats, conv_eps, solver,
+                 sigma):
+    """The reference's evp.als with deflation, a generalised problem or
+    several eigenpairs (scikit_tt/solvers/evp.py:86-179), step by step:
+
+    * stacks of the operator, of ``operator_gevp`` and of the overlaps with
+      each previous eigentensor (evp.py:253-334) by the device stack pieces;
+    * dense local matrices (evp.py:337-384): micro_op, micro_op_gevp, and
+      micro_op += shift t t^T for each previous eigentensor's local part t;
+    * the local eigenpairs (evp.py:411-443) by the device dense solver:
+      'eig' the number_ev nearest sigma (unit eigenvectors), 'eigh' the
+      number_ev largest (M-orthonormal), for symmetric micro_op and symmetric
+      positive definite micro_op_gevp; a non-symmetric local matrix raises
+      NotImplementedError (the reference's general dgeev/QZ path);
+    * the new core from the SVD of the local eigenvectors (evp.py:445-493):
+      forward the leading min(r_{i+1}, .) left singular vectors, backward the
+      leading right ones, and at position 0 of the backward half-sweep the
+      eigenvectors themselves with the extra eigen-index;
+    * the best-so-far pair by |lambda - sigma| for number_ev = 1 and the
+      convergence rule over the last <= 3 sweeps (evp.py:151-165).
+    Returns (eigenvalue, TT, iterations), or for number_ev > 1 (eigenvalues,
+    list of TT, iterations).
+    """
+    import ctypes
+    import numpy as np
+    import torch
+    from ..utils import truncated_svd
+    lib = _lib.load()
+    dev = operator.device
+    ctx = _lib.context(dev.index)
+    st = _stream(dev)
+    d = operator.order
+    n = list(operator.row_dims)
+    if list(operator.col_dims) != n or list(initial_guess.row_dims) != n:
+        raise ValueError('operator must be square and match the initial guess')
+    kev = number_ev
+    f64 = dict(dtype=torch.float64, device=dev)
+
+    def ptr(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def chk(rc, what):
+        if rc > 0:
+            raise np.linalg.LinAlgError('%s: non-finite local result (evp.als)' % what)
+        _lib.check(rc, ctx, what)
+
+    ops = [(list(operator.ranks), [c.contiguous() for c in operator.device_cores])]
+    if operator_gevp is not None:
+        if list(operator_gevp.row_dims) != n or list(operator_gevp.col_dims) != n:
+            raise ValueError('operator_gevp must have the dimensions of operator')
+        ops.append((list(operator_gevp.ranks), [c.contiguous() for c in operator_gevp.device_cores]))
+    prevs = []
+    for p in previous:
+        if list(p.row_dims) != n:
+            raise ValueError('previous eigentensors must have the dimensions of the initial guess')
+        prevs.append((list(p.ranks), [c.contiguous() for c in p.device_cores]))
+    ranks = list(initial_guess.ranks)
+    cores = [c.reshape(c.shape[0], c.shape[1], c.shape[3]).clone() for c in initial_guess.device_cores]
+    Lo = [[None] * d for _ in ops]
+    Ro = [[None] * d for _ in ops]
+    Lp = [[None] * d for _ in prevs]
+    Rp = [[None] * d for _ in prevs]
+
+    def x3(i):  # core i as (r_i, n_i, r_{i+1}) (position 0 may carry the eigen-index after a sweep)
+        c = cores[i]
+        return c if c.dim() == 3 else c[..., 0].contiguous()
+
+    def left(i):
+        if i == 0:
+            for L in Lo:
+                L[0] = torch.ones((1, 1, 1), **f64)
+            for L in Lp:
+                L[0] = torch.ones((1, 1), **f64)
+            return
+        j = i - 1
+        a, b = ranks[j], ranks[i]
+        x = x3(j)
+        for (ra, Ac), L in zip(ops, Lo):
+            L[i] = torch.empty((b, ra[i], b), **f64)
+            chk(lib.ttals_stack_left_op(ctx, st, a, ra[j], n[j], ra[i], b, ptr(L[j]), ptr(Ac[j]), ptr(x), ptr(L[i])),
+                'stack_left_op')
+        for (rp, Pc), L in zip(prevs, Lp):
+            L[i] = torch.empty((rp[i], b), **f64)
+            chk(lib.ttals_stack_left_rhs(ctx, st, a, rp[j], n[j], rp[i], b, ptr(L[j]), ptr(Pc[j]), ptr(x),
+                                         ptr(L[i])), 'stack_left_rhs')
+
+    def right(i):
+        if i == d - 1:
+            for R in Ro:
+                R[i] = torch.ones((1, 1, 1), **f64)
+            for R in Rp:
+                R[i] = torch.ones((1, 1), **f64)
+            return
+        j = i + 1
+        a, b = ranks[j], ranks[j + 1]
+        x = x3(j)
+        for (ra, Ac), R in zip(ops, Ro):
+            R[i] = torch.empty((a, ra[j], a), **f64)
+            chk(lib.ttals_stack_right_op(ctx, st, a, ra[j], n[j], ra[j + 1], b, ptr(R[j]), ptr(Ac[j]), ptr(x),
+                                         ptr(R[i])), 'stack_right_op')
+        for (rp, Pc), R in zip(prevs, Rp):
+            R[i] = torch.empty((rp[j], a), **f64)
+            chk(lib.ttals_stack_right_rhs(ctx, st, a, rp[j], n[j], rp[j + 1], b, ptr(R[j]), ptr(Pc[j]), ptr(x),
+                                          ptr(R[i])), 'stack_right_rhs')
+
+    def micro(i):
+        a, b = ranks[i], ranks[i + 1]
+        N = a * n[i] * b
+        mats = []
+        for (ra, Ac), L, R in zip(ops, Lo, Ro):
+            Mx = torch.empty((N, N), **f64)
+            chk(lib.ttals_micro_matrix(ctx, st, a, ra[i], n[i], ra[i + 1], b, ptr(L[i]), ptr(Ac[i]), ptr(R[i]),
+                                       ptr(Mx)), 'micro_matrix')
+            mats.append(Mx)
+        for (rp, Pc), L, R in zip(prevs, Lp, Rp):
+            t = torch.empty(N, **f64)
+            chk(lib.ttals_micro_rhs(ctx, st, a, rp[i], n[i], rp[i + 1], b, ptr(L[i]), ptr(Pc[i]), ptr(R[i]), ptr(t)),
+                'micro_rhs')
+            mats[0] += shift * torch.outer(t, t)
+        return mats[0], (mats[1] if len(mats) > 1 else None)
+
+    def update(i, micro_op, micro_gevp, direction):
+        N = micro_op.shape[0]
+        scale = float(micro_op.abs().max())
+        if float((micro_op - micro_op.T).abs().max()) > 1e-10 * max(scale, 1e-300):
+            raise NotImplementedError('evp.als: the local matrix is not symmetric; local eigensolves of '
+                                      'non-symmetric problems (the reference\'s dgeev/QZ path) are not implemented')
+        k = min(kev, N)
+        if k < kev:
+            raise ValueError('number_ev exceeds the size of a local problem')
+        vals = torch.empty(kev, **f64)
+        vecs = torch.empty((N, kev), **f64)
+        chk(lib.ttals_dense_eig(ctx, st, N, ptr(micro_op), ptr(micro_gevp) if micro_gevp is not None else None,
+                                kev, _SOLVERS[solver], sigma, 1 if solver == 'eig' else 0, ptr(vals), ptr(vecs)),
+            'dense_eig')
+        a, b = ranks[i], ranks[i + 1]
+        if direction == 'forward':
+            u, _, _ = truncated_svd(vecs.reshape(a * n[i], b * kev), threshold=0, max_rank=b)
+            r = u.shape[1]
+            ranks[i + 1] = r
+            cores[i] = u.reshape(a, n[i], r).contiguous()
+        elif i > 0:
+            _, _, vt = truncated_svd(vecs.T.contiguous().reshape(kev * a, n[i] * b), threshold=0, max_rank=a)
+            r = vt.shape[0]
+            ranks[i] = r
+            cores[i] = vt.reshape(r, n[i], b).contiguous()
+        else:
+            cores[0] = vecs.reshape(a, n[0], b, kev).contiguous()
+        return vals.cpu().numpy()
+
+    for i in range(d - 1, -1, -1):
+        right(i)
+    current = 1
+    eig_pre = np.full((1, kev), np.inf)
+    converged = False
+    best, best_tt = np.inf, None
+    eigenvalues = None
+    while current <= repeats and not converged:
+        for i in range(d):
+            left(i)
+            if i < d - 1:
+                eigenvalues = update(i, *micro(i), 'forward')
+        for i in range(d - 1, -1, -1):
+            right(i)
+            eigenvalues = update(i, *micro(i), 'backward')
+        current += 1
+        if kev == 1 and abs(eigenvalues[0] - sigma) < abs(best - sigma):
+            best = float(eigenvalues[0])
+            best_tt = [cores[0][..., 0].contiguous()] + cores[1:]
+        last = eig_pre[-min(3, eig_pre.shape[0]):, :]
+        if np.amax(np.abs(last - eigenvalues)) < conv_eps:
+            converged = True
+        eig_pre = np.vstack((eig_pre, eigenvalues))
+    iterations = current - 1
+
+    def as_tt(c3):
+        return TT._from_device([c.reshape(c.shape[0], c.shape[1], 1, c.shape[2]) for c in c3])
+
+    if kev == 1:
+        return best, as_tt(best_tt), iterations
+    tts = [as_tt([cores[0][..., e].contiguous()] + cores[1:]) for e in range(kev)]
+    return eigenvalues, tts, iterations
 
 
 def power_method(operator, initial_guess, operator_gevp=None, repeats=10, sigma=0.999):

@@ -243,10 +243,88 @@ def mals_case(ref_root):
           list(t['svd_ranks_thr']), list(t['svd_ranks_maxr']), 'qtt', list(t['q_ranks']), list(t['q_ranks_thr']))
 
 
-CASES = {'base': main, 'c2': c2_instance, 'ode': ode_cases, 'power': power_case, 'mals': mals_case}
+def evp_options_case(ref_root):
+    """evp.als with the options that run outside the fused engine.
+
+    * The reference's own several-eigenpairs problem (tests/test_evp.py:
+      test_als_eigh): the Ulam operator of the triple-well transitions
+      (examples/data/triple_well_transitions.npz, coarse-grained by 2, 25 x 25
+      states, 2000 simulations), made symmetric, number_ev = 3, 'eigh', 10
+      sweeps, ones rank-15 start right-orthonormalised.
+    * A TFIM chain (d = 6, rank-4 start): a generalised problem with a
+      symmetric positive definite operator_gevp (a rank-1 TT of positive
+      diagonals), 'eig' nearest sigma = -100; deflation of the ground state
+      (previous, shift = 50) from another start; number_ev = 2 with 'eig'.
+    Dense eigenvalues of every problem are stored beside the reference's."""
+    sys.path.insert(0, ref_root)
+    sys.path.insert(0, ROOT)
+    import scipy.linalg as sla
+    from scikit_tt.tensor_train import TT
+    import scikit_tt.tensor_train as rtt
+    import scikit_tt.solvers.evp as revp
+    import scikit_tt.data_driven.ulam as ulam
+    from scikit_tt_amd import workloads as W
+    s = {}
+    transitions = np.load(os.path.join(ref_root, 'examples', 'data', 'triple_well_transitions.npz'))['transitions']
+    transitions = np.int64(np.ceil(np.true_divide(transitions, 2)))
+    op = ulam.ulam_2d(transitions, [25, 25], 2000)
+    op = 0.5 * (op + op.transpose())
+    x0 = rtt.ones(op.row_dims, [1] * op.order, ranks=15).ortho_right()
+    vals, vecs, it = revp.als(op, x0, number_ev=3, repeats=10, solver='eigh')
+    _save_tt(s, 'ulam_A', op.cores)
+    _save_tt(s, 'ulam_x0', x0.cores)
+    s['ulam_vals'] = np.array(vals, dtype=float)
+    s['ulam_iters'] = np.array(it)
+    for e in range(3):
+        _save_tt(s, 'ulam_vec%d' % e, vecs[e].cores)
+    s['ulam_ranks'] = np.array(vecs[0].ranks)
+    dense = np.linalg.eigvalsh(op.matricize())
+    s['ulam_dense_top3'] = dense[::-1][:3].copy()
+    # TFIM chain
+    H, y0, J, g = W.tfim_chain(seed=21, d=6, rank=4)
+    rng = np.random.default_rng(22)
+    y0t = TT(y0).ortho_right()
+    y0t = (1.0 / y0t.norm()) * y0t
+    y1 = TT(W.random_tt(rng, [2] * 6, 4)).ortho_right()
+    y1 = (1.0 / y1.norm()) * y1
+    Mcores = [np.diag(1.0 + 0.3 * rng.random(2)).reshape(1, 2, 2, 1) for _ in range(6)]
+    _save_tt(s, 'tfim_A', H)
+    _save_tt(s, 'tfim_x0', y0t.cores)
+    _save_tt(s, 'tfim_x1', y1.cores)
+    _save_tt(s, 'tfim_M', Mcores)
+    Hm = TT(H).matricize()
+    Mm = TT(Mcores).matricize()
+    # generalised
+    lam, y, it = revp.als(TT(H), y0t, operator_gevp=TT(Mcores), repeats=6, solver='eig', sigma=-100.0)
+    s['gevp_val'] = np.array(float(np.real(lam)))
+    s['gevp_iters'] = np.array(it)
+    _save_tt(s, 'gevp_vec', y.cores)
+    s['gevp_dense'] = np.array(sla.eigh(Hm, Mm, eigvals_only=True)[0])
+    # deflation of the ground state
+    lam0, y_gs, _ = revp.als(TT(H), y0t, repeats=6, solver='eig', sigma=-100.0)
+    lam1, y_ex, it = revp.als(TT(H), y1, previous=[y_gs], shift=50.0, repeats=6, solver='eig', sigma=-100.0)
+    _save_tt(s, 'defl_prev', y_gs.cores)
+    s['defl_prev_val'] = np.array(float(np.real(lam0)))
+    s['defl_val'] = np.array(float(np.real(lam1)))
+    s['defl_iters'] = np.array(it)
+    _save_tt(s, 'defl_vec', y_ex.cores)
+    s['dense_lowest2'] = np.linalg.eigvalsh(Hm)[:2].copy()
+    # two eigenpairs
+    vals, vecs, it = revp.als(TT(H), y0t, number_ev=2, repeats=6, solver='eig', sigma=-100.0)
+    s['two_vals'] = np.real(np.array(vals, dtype=complex))
+    s['two_iters'] = np.array(it)
+    for e in range(2):
+        _save_tt(s, 'two_vec%d' % e, vecs[e].cores)
+    np.savez_compressed(os.path.join(HERE, 'evp_options.npz'), **s)
+    print('evp options: ulam', s['ulam_vals'], s['ulam_dense_top3'], 'gevp', s['gevp_val'], s['gevp_dense'],
+          'deflation', s['defl_val'], s['dense_lowest2'], 'two', s['two_vals'])
+
+
+CASES = {'base': main, 'c2': c2_instance, 'ode': ode_cases, 'power': power_case, 'mals': mals_case,
+         'evp_options': evp_options_case}
 
 if __name__ == '__main__':
-    # usage: make_golden.py [reference root] [case ...]   (cases: base, c2, ode, power, mals; default: all)
+    # usage: make_golden.py [reference root] [case ...]   (cases: base, c2, ode, power, mals, evp_options; default: all)
     root = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
     for name in (sys.argv[2:] or list(CASES)):
         CASES[name](root)

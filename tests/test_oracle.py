@@ -428,3 +428,25 @@ def test_evp_closed_form_small():
     Hd = tt_ops.matricize(H)
     v = tt_ops.matricize(y)
     assert abs(v @ Hd @ v - lam) < 1e-10 and abs(np.linalg.norm(v) - 1) < 1e-12
+
+
+def test_evp_options_fixture_against_dense():
+    """The reference-written evp-options fixture against dense eigenvalues of
+    its own stored operators: the Ulam test problem within the reference
+    test's tolerance (5e-3), the generalised TFIM problem and the deflated
+    (first excited) one to the ALS accuracy at these ranks."""
+    import scipy.linalg as sla
+    from tests.conftest import load_golden
+    g = load_golden('evp_options')
+    A = tt_ops.matricize(g['tt']('ulam_A'))
+    top3 = np.linalg.eigvalsh(0.5 * (A + A.T))[::-1][:3]
+    np.testing.assert_allclose(top3, g['ulam_dense_top3'], rtol=1e-12)
+    assert np.all(np.abs(g['ulam_vals'] - top3) <= 5e-3 * np.abs(top3))
+    H = tt_ops.matricize(g['tt']('tfim_A'))
+    M = tt_ops.matricize(g['tt']('tfim_M'))
+    assert np.all(np.linalg.eigvalsh(M) > 0)
+    lowest = sla.eigh(H, M, eigvals_only=True)[0]
+    assert abs(float(g['gevp_val']) - lowest) <= 1e-6 * abs(lowest)
+    two = np.linalg.eigvalsh(H)[:2]
+    assert abs(float(g['defl_val']) - two[1]) <= 1e-6 * abs(two[1])
+    assert abs(float(g['defl_prev_val']) - two[0]) <= 1e-6 * abs(two[0])
