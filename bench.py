@@ -144,6 +144,24 @@ def measured_traffic(kernel, tag):
     return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
+def measured_fp64_counters(kernel, tag):
+    """MFMA utilisation and executed FP64 flops per launch from the committed
+    rocprofv3 FP64 counter summary of `kernel` on workload `tag` ({} if none)."""
+    import re
+    path = os.path.join(ROOT, 'profiles', '%s_%s_fp64_counters.md' % (kernel, tag))
+    if not os.path.exists(path):
+        return {}
+    text = open(path).read()
+    out = {'counters_source': os.path.relpath(path, ROOT)}
+    m = re.search(r'^mfma_util_percent: ([0-9.eE+-]+)', text, re.M)
+    if m:
+        out['mfma_util_percent'] = float(m.group(1))
+    m = re.search(r'^fp64_executed_flops_per_launch: (\d+)', text, re.M)
+    if m:
+        out['fp64_executed_flops_per_launch'] = int(m.group(1))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # secondary workload: TFIM chains, evp
 
@@ -403,6 +421,7 @@ def main():
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
     kernel = 'k_hybrid' if gangs.sum() > 0 else 'k_als'
     traffic, traffic_src = measured_traffic(kernel, 'uneven_sle_b%d_s%d' % (B, args.sweeps))
+    counters = measured_fp64_counters(kernel, 'uneven_sle_b%d_s%d' % (B, args.sweeps))
     achieved = flops / avg_kernel_s / 1e12
     line = {
         'metric': 'TT-ALS solves/s at rel.res <=1e-8, batched vs plain; p50/p99 latency',
@@ -449,7 +468,7 @@ def main():
         'latency_ms_p99': round(float(np.percentile(lat, 99)), 3),
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': traffic, 'traffic_source': traffic_src,
-                     'kernel': kernel, 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3},
+                     'kernel': kernel, 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3, **counters},
     }
     if args.tfim_batch > 0:
         log('tfim evp batch %d' % args.tfim_batch)
