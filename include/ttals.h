@@ -264,15 +264,19 @@ int ttals_local_solve(ttals_ctx* ctx, void* stream, int a, int P, int n, int Q, 
                       const double* Acore, const double* R, const double* f, double* v, int method, int sym,
                       double tol, int maxit, int* iters);
 /* scikit_tt/solvers/evp.py:411-443, dense: the local eigenproblem
- * micro_op x = l micro_op_gevp x of order N <= 2048 on the device, for
- * symmetric micro_op and symmetric positive definite micro_op_gevp (M may be
- * NULL: the standard problem).  mode 0 ('eig'): the k eigenpairs nearest
- * sigma, ascending |l - sigma|; mode 1 ('eigh'): the k largest, descending.
- * unit != 0: eigenvectors of unit 2-norm (scipy.linalg.eig), else
- * M-orthonormal (scipy.linalg.eigh).  vals (k) and vecs (N x k, row-major)
- * are device memory.  TTALS_E_UNSUPPORTED if M is not positive definite. */
+ * micro_op x = l micro_op_gevp x of order N <= 2048 on the device, with a
+ * symmetric positive definite micro_op_gevp (M may be NULL: the standard
+ * problem).  general = 0: micro_op symmetric; mode 0 ('eig'): the k
+ * eigenpairs nearest sigma, ascending |l - sigma|; mode 1 ('eigh'): the k
+ * largest, descending.  general = 1: any real micro_op, mode 0 only
+ * (Hessenberg form and the double-shift QR algorithm, eigenvectors by
+ * inverse iteration); a complex eigenvalue among the k nearest sigma is
+ * TTALS_E_UNSUPPORTED.  unit != 0: eigenvectors of unit 2-norm
+ * (scipy.linalg.eig), else M-orthonormal (scipy.linalg.eigh).  vals (k) and
+ * vecs (N x k, row-major) are device memory.  TTALS_E_UNSUPPORTED if M is
+ * not positive definite. */
 int ttals_dense_eig(ttals_ctx* ctx, void* stream, int N, const double* A, const double* M, int k, int mode,
-                    double sigma, int unit, double* vals, double* vecs);
+                    double sigma, int unit, int general, double* vals, double* vecs);
 /* scikit_tt/solvers/evp.py:411-443 for symmetric local matrices: extreme
  * eigenpair (mode 0 lowest, 1 highest, 2 nearest sigma).  v: start vector
  * in, unit eigenvector out; lambda (host) receives the eigenvalue. */

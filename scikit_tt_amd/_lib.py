@@ -64,7 +64,7 @@ SIGNATURES = {
     'ttals_set_local_tol_policy': (_i, [_vp, _i]),
     'ttals_set_matvec_variant': (_i, [_vp, _i]),
     'ttals_set_schedule': (_i, [_vp, _i, _i]),
-    'ttals_dense_eig': (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _vp, _vp]),
+    'ttals_dense_eig': (_i, [_vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _i, _vp, _vp]),
     'ttals_set_time_budget': (_i, [_vp, _d]),
     'ttals_set_cost_model': (_i, [_vp, _vp, _i]),
     'ttals_get_cost_model': (_i, [_vp, _vp, _i]),

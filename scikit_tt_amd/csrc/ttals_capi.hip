@@ -482,40 +482,321 @@ __global__ __launch_bounds__(NT) void k_svd_trunc(SvdArgs A) {
   if (threadIdx.x == 0) *A.kout = kk;
 }
 
-// Dense symmetric (A = A^T) or symmetric-definite generalized (A x = l M x,
-// M = M^T positive definite) eigenproblem of order N on one workgroup, for
-// the reference's dense local eigensolves of evp.als with operator_gevp,
-// previous/shift or number_ev > 1 (scikit_tt/solvers/evp.py:411-443):
-// M = L L^T (Cholesky), C = L^{-1} A L^{-T}, C + tau I with tau above
-// C's Gershgorin bound (positive definite, so its one-sided Jacobi SVD is its
-// eigendecomposition), eigenvalues sigma_j - tau, eigenvectors L^{-T} V.
-// Selection: mode 0 the k nearest `sigma` (ascending |l - sigma|), mode 1 the
-// k largest (descending).  unit: eigenvectors scaled to unit 2-norm (as
-// scipy.linalg.eig returns them), else M-orthonormal (as eigh).
+// Householder reduction of C (N x N, row-major, global memory) to upper
+// Hessenberg form, in place (Q is not kept: eigenvalues only).
+__device__ void wg_hessenberg(double* C, int N, double* red) {
+  for (int j = 0; j < N - 2; ++j) {
+    double s2 = 0.0;
+    for (int i = j + 2 + threadIdx.x; i < N; i += NT) s2 += C[(long)i * N + j] * C[(long)i * N + j];
+    s2 = wg_sum(s2, red);
+    if (s2 == 0.0) continue;
+    const double alpha = C[(long)(j + 1) * N + j];
+    const double beta = alpha >= 0.0 ? -sqrt(alpha * alpha + s2) : sqrt(alpha * alpha + s2);
+    const double tau = (beta - alpha) / beta, sc = 1.0 / (alpha - beta);
+    __syncthreads();
+    for (int i = j + 2 + threadIdx.x; i < N; i += NT) C[(long)i * N + j] *= sc;  // v below v_{j+1} = 1
+    __syncthreads();
+    // C <- (I - tau v v^T) C on rows j+1.., columns j+1..
+    for (int q = j + 1 + threadIdx.x; q < N; q += NT) {
+      double w = C[(long)(j + 1) * N + q];
+      for (int i = j + 2; i < N; ++i) w += C[(long)i * N + j] * C[(long)i * N + q];
+      w *= tau;
+      C[(long)(j + 1) * N + q] -= w;
+      for (int i = j + 2; i < N; ++i) C[(long)i * N + q] -= C[(long)i * N + j] * w;
+    }
+    __syncthreads();
+    // C <- C (I - tau v v^T) on all rows, columns j+1..
+    for (int r = threadIdx.x; r < N; r += NT) {
+      double w = C[(long)r * N + j + 1];
+      for (int i = j + 2; i < N; ++i) w += C[(long)r * N + i] * C[(long)i * N + j];
+      w *= tau;
+      C[(long)r * N + j + 1] -= w;
+      for (int i = j + 2; i < N; ++i) C[(long)r * N + i] -= w * C[(long)i * N + j];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) C[(long)(j + 1) * N + j] = beta;
+    for (int i = j + 2 + threadIdx.x; i < N; i += NT) C[(long)i * N + j] = 0.0;
+    __syncthreads();
+  }
+}
+
+// Eigenvalues (wr + i wi) of the upper Hessenberg H (N x N, global) by the
+// implicit double-shift QR algorithm (Francis; Golub & Van Loan, Matrix
+// Computations, Alg. 7.5.1-7.5.2) on the active block, deflating at
+// subdiagonals below eps (|h_{l-1,l-1}| + |h_ll|); exceptional shifts at
+// iterations 10 and 20 of an eigenvalue.  The 3-element reflectors are
+// computed by thread 0 and applied by all threads.  Returns 0, or 1 when an
+// eigenvalue needs more than 60 iterations.
+__device__ int wg_hqr(double* H, int N, double* wr, double* wi, double* sd, int* si, double* red) {
+  auto h = [&](int r, int c) -> double& { return H[(long)r * N + c]; };
+  double nrm = 0.0;
+  for (long e = threadIdx.x; e < (long)N * N; e += NT) nrm = fmax(nrm, fabs(H[e]));
+  nrm = wave_max(nrm);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+  __syncthreads();
+  nrm = 0.0;
+  for (int q = 0; q < NWAVE; ++q) nrm = fmax(nrm, red[q]);
+  if (nrm == 0.0) nrm = 1.0;
+  const double eps = 0x1p-52;
+  int hi = N - 1, its = 0;
+  while (hi >= 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int l = hi;
+      while (l > 0) {
+        double s = fabs(h(l - 1, l - 1)) + fabs(h(l, l));
+        if (s == 0.0) s = nrm;
+        if (fabs(h(l, l - 1)) <= eps * s) break;
+        --l;
+      }
+      si[0] = l;
+    }
+    __syncthreads();
+    const int l = si[0];
+    if (l == hi) {
+      if (threadIdx.x == 0) {
+        wr[hi] = h(hi, hi);
+        wi[hi] = 0.0;
+      }
+      --hi;
+      its = 0;
+      continue;
+    }
+    if (l == hi - 1) {
+      if (threadIdx.x == 0) {
+        const double a = h(hi - 1, hi - 1), b = h(hi - 1, hi), c = h(hi, hi - 1), d = h(hi, hi);
+        const double p = 0.5 * (a - d), q = p * p + b * c;
+        if (q >= 0.0) {
+          const double z = p + copysign(sqrt(q), p);
+          wr[hi - 1] = d + z;
+          wr[hi] = z != 0.0 ? d - b * c / z : d + z;
+          wi[hi - 1] = wi[hi] = 0.0;
+        } else {
+          wr[hi - 1] = wr[hi] = d + p;
+          wi[hi - 1] = sqrt(-q);
+          wi[hi] = -sqrt(-q);
+        }
+      }
+      hi -= 2;
+      its = 0;
+      continue;
+    }
+    if (its >= 60) return 1;
+    ++its;
+    if (threadIdx.x == 0) {
+      double sh, t;
+      if (its == 10 || its == 20) {  // exceptional shift
+        const double ex = fabs(h(hi, hi - 1)) + fabs(h(hi - 1, hi - 2));
+        const double x0 = h(hi, hi) + 0.75 * ex;
+        sh = 2.0 * x0;
+        t = x0 * x0 - 0.4375 * ex * ex;
+      } else {
+        sh = h(hi - 1, hi - 1) + h(hi, hi);
+        t = h(hi - 1, hi - 1) * h(hi, hi) - h(hi - 1, hi) * h(hi, hi - 1);
+      }
+      sd[0] = h(l, l) * h(l, l) + h(l, l + 1) * h(l + 1, l) - sh * h(l, l) + t;
+      sd[1] = h(l + 1, l) * (h(l, l) + h(l + 1, l + 1) - sh);
+      sd[2] = h(l + 1, l) * h(l + 2, l + 1);
+    }
+    __syncthreads();
+    for (int k = l; k <= hi - 2; ++k) {
+      if (threadIdx.x == 0) {
+        const double x = sd[0], y = sd[1], z = sd[2];
+        const double s2 = y * y + z * z;
+        if (s2 == 0.0) {
+          sd[3] = 0.0;
+        } else {
+          const double beta = x >= 0.0 ? -sqrt(x * x + s2) : sqrt(x * x + s2);
+          sd[3] = (beta - x) / beta;
+          sd[4] = y / (x - beta);
+          sd[5] = z / (x - beta);
+        }
+      }
+      __syncthreads();
+      const double tau = sd[3], u1 = sd[4], u2 = sd[5];
+      if (tau != 0.0) {
+        for (int q = (k - 1 > l ? k - 1 : l) + threadIdx.x; q <= hi; q += NT) {
+          const double w = tau * (h(k, q) + u1 * h(k + 1, q) + u2 * h(k + 2, q));
+          h(k, q) -= w;
+          h(k + 1, q) -= u1 * w;
+          h(k + 2, q) -= u2 * w;
+        }
+        __syncthreads();
+        const int rlast = k + 3 < hi ? k + 3 : hi;
+        for (int r = l + threadIdx.x; r <= rlast; r += NT) {
+          const double w = tau * (h(r, k) + u1 * h(r, k + 1) + u2 * h(r, k + 2));
+          h(r, k) -= w;
+          h(r, k + 1) -= u1 * w;
+          h(r, k + 2) -= u2 * w;
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        sd[0] = h(k + 1, k);
+        sd[1] = h(k + 2, k);
+        sd[2] = k + 3 <= hi ? h(k + 3, k) : 0.0;
+      }
+      __syncthreads();
+    }
+    // the last 2-element reflector on rows / columns hi-1, hi
+    if (threadIdx.x == 0) {
+      const double x = sd[0], y = sd[1];
+      if (y == 0.0) {
+        sd[3] = 0.0;
+      } else {
+        const double beta = x >= 0.0 ? -sqrt(x * x + y * y) : sqrt(x * x + y * y);
+        sd[3] = (beta - x) / beta;
+        sd[4] = y / (x - beta);
+      }
+    }
+    __syncthreads();
+    const double tau = sd[3], u1 = sd[4];
+    if (tau != 0.0) {
+      for (int q = (hi - 2 > l ? hi - 2 : l) + threadIdx.x; q <= hi; q += NT) {
+        const double w = tau * (h(hi - 1, q) + u1 * h(hi, q));
+        h(hi - 1, q) -= w;
+        h(hi, q) -= u1 * w;
+      }
+      __syncthreads();
+      for (int r = l + threadIdx.x; r <= hi; r += NT) {
+        const double w = tau * (h(r, hi - 1) + u1 * h(r, hi));
+        h(r, hi - 1) -= w;
+        h(r, hi) -= u1 * w;
+      }
+    }
+  }
+  __syncthreads();
+  return 0;
+}
+
+// LU with partial pivoting of B (N x N, global) in place, then two solves of
+// B x = x (inverse iteration) with x normalised in between and at the end.
+// piv: N ints of scratch; vmax/imax: NT doubles/ints of LDS.
+__device__ void wg_inverse_iteration(double* B, int N, double* x, int* piv, double* vmax, int* imax) {
+  for (int j = 0; j < N; ++j) {
+    double best = -1.0;
+    int bi = j;
+    for (int i = j + threadIdx.x; i < N; i += NT) {
+      const double v = fabs(B[(long)i * N + j]);
+      if (v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    vmax[threadIdx.x] = best;
+    imax[threadIdx.x] = bi;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bv = -1.0;
+      int bk = j;
+      for (int t = 0; t < NT; ++t)
+        if (vmax[t] > bv || (vmax[t] == bv && imax[t] < bk)) {
+          bv = vmax[t];
+          bk = imax[t];
+        }
+      piv[j] = bk;
+      if (B[(long)bk * N + j] == 0.0) B[(long)bk * N + j] = 0x1p-1000;  // exactly singular: tiny pivot
+    }
+    __syncthreads();
+    const int p = piv[j];
+    if (p != j)
+      for (int q = threadIdx.x; q < N; q += NT) {
+        const double t = B[(long)j * N + q];
+        B[(long)j * N + q] = B[(long)p * N + q];
+        B[(long)p * N + q] = t;
+      }
+    __syncthreads();
+    const double inv = 1.0 / B[(long)j * N + j];
+    for (int i = j + 1 + threadIdx.x; i < N; i += NT) B[(long)i * N + j] *= inv;
+    __syncthreads();
+    const int rest = N - j - 1;
+    for (long e = threadIdx.x; e < (long)rest * rest; e += NT) {
+      const int i = j + 1 + (int)(e / rest), q = j + 1 + (int)(e % rest);
+      B[(long)i * N + q] -= B[(long)i * N + j] * B[(long)j * N + q];
+    }
+    __syncthreads();
+  }
+  for (int it = 0; it < 2; ++it) {
+    // normalise, permute, L y = P x, U x = y
+    double s2 = 0.0;
+    for (int i = threadIdx.x; i < N; i += NT) s2 += x[i] * x[i];
+    s2 = wg_sum(s2, vmax);
+    const double sc = s2 > 0.0 ? 1.0 / sqrt(s2) : 1.0;
+    for (int i = threadIdx.x; i < N; i += NT) x[i] *= sc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < N; ++j)
+        if (piv[j] != j) {
+          const double t = x[j];
+          x[j] = x[piv[j]];
+          x[piv[j]] = t;
+        }
+    __syncthreads();
+    for (int j = 0; j < N; ++j) {
+      const double xj = x[j];
+      for (int i = j + 1 + threadIdx.x; i < N; i += NT) x[i] -= B[(long)i * N + j] * xj;
+      __syncthreads();
+    }
+    for (int j = N - 1; j >= 0; --j) {
+      if (threadIdx.x == 0) x[j] /= B[(long)j * N + j];
+      __syncthreads();
+      const double xj = x[j];
+      for (int i = threadIdx.x; i < j; i += NT) x[i] -= B[(long)i * N + j] * xj;
+      __syncthreads();
+    }
+  }
+  double s2 = 0.0;
+  for (int i = threadIdx.x; i < N; i += NT) s2 += x[i] * x[i];
+  s2 = wg_sum(s2, vmax);
+  const double sc = s2 > 0.0 ? 1.0 / sqrt(s2) : 1.0;
+  for (int i = threadIdx.x; i < N; i += NT) x[i] *= sc;
+  __syncthreads();
+}
+
+// Dense local eigenproblem of order N on one workgroup, for the reference's
+// dense local eigensolves of evp.als (scikit_tt/solvers/evp.py:411-443):
+// A x = l x, or A x = l M x with M = M^T positive definite (M = L L^T,
+// Cholesky; C = L^{-1} A L^{-T}; x = L^{-T} y).
+// * symmetric A: C shifted positive definite by its Gershgorin bound, its
+//   one-sided Jacobi SVD is its eigendecomposition; mode 0 the k nearest
+//   `sigma` (ascending |l - sigma|), mode 1 the k largest (descending);
+// * general A (mode 0 only): Hessenberg reduction and the double-shift QR
+//   algorithm for the eigenvalues, the k nearest sigma selected (a complex one
+//   among them is reported), each eigenvector by inverse iteration on
+//   C - (l + delta) I with an LU factorisation.
+// unit: eigenvectors of unit 2-norm (scipy.linalg.eig), else M-orthonormal
+// (scipy.linalg.eigh; symmetric path).
 struct EigArgs {
-  int N, k, mode, unit;
+  int N, k, mode, unit, general;
   double sigma;
   const double* A;
   const double* M;
   double* C;
+  double* C0;
   double* Lm;
   double* V;
   double* sig;
+  double* wr;
+  double* wi;
+  double* xv;
   int* perm;
   int* idx;
   double* vals;
   double* vecs;  // N x k, row-major
-  int* status;   // 0 ok, 1 M not positive definite, 2 non-finite
+  int* status;   // 0 ok, 1 M not positive definite, 2 non-finite, 3 a selected eigenvalue is complex, 4 no QR convergence
 };
 
 __global__ __launch_bounds__(NT) void k_dense_eig(EigArgs E) {
-  __shared__ double red[4 * NWAVE];
+  __shared__ double red[NT];
+  __shared__ int ired[NT];
   __shared__ double s_tau;
+  __shared__ double sd[8];
   const int N = E.N;
   const long NN = (long)N * N;
   for (long e = threadIdx.x; e < NN; e += NT) {
     const long i = e / N, j = e % N;
-    E.C[e] = 0.5 * (E.A[i * N + j] + E.A[j * N + i]);
+    E.C[e] = E.general ? E.A[e] : 0.5 * (E.A[i * N + j] + E.A[j * N + i]);
     if (E.M) E.Lm[e] = 0.5 * (E.M[i * N + j] + E.M[j * N + i]);
   }
   __syncthreads();
@@ -555,63 +836,116 @@ __global__ __launch_bounds__(NT) void k_dense_eig(EigArgs E) {
         E.C[(long)row * N + j] = t / E.Lm[(long)j * N + j];
       }
     __syncthreads();
-    for (long e = threadIdx.x; e < NN; e += NT) {
-      const long i = e / N, j = e % N;
-      if (j > i) {
-        const double v = 0.5 * (E.C[i * N + j] + E.C[j * N + i]);
-        E.C[i * N + j] = v;
-        E.C[j * N + i] = v;
+    if (!E.general)
+      for (long e = threadIdx.x; e < NN; e += NT) {
+        const long i = e / N, j = e % N;
+        if (j > i) {
+          const double v = 0.5 * (E.C[i * N + j] + E.C[j * N + i]);
+          E.C[i * N + j] = v;
+          E.C[j * N + i] = v;
+        }
+      }
+    __syncthreads();
+  }
+  if (E.general) {
+    for (long e = threadIdx.x; e < NN; e += NT) E.C0[e] = E.C[e];
+    __syncthreads();
+    wg_hessenberg(E.C, N, red);
+    if (wg_hqr(E.C, N, E.wr, E.wi, sd, ired, red)) {
+      if (threadIdx.x == 0) *E.status = 4;
+      return;
+    }
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < N; ++j) E.perm[j] = 0;
+      for (int s = 0; s < E.k; ++s) {
+        int best = -1;
+        double bv = 0.0;
+        for (int j = 0; j < N; ++j) {
+          if (E.perm[j]) continue;
+          const double key = hypot(E.wr[j] - E.sigma, E.wi[j]);
+          if (best < 0 || key < bv) {
+            best = j;
+            bv = key;
+          }
+        }
+        E.perm[best] = 1;
+        E.idx[s] = best;
+        E.vals[s] = E.wr[best];
+        if (fabs(E.wi[best]) > 1e-12 * fmax(1.0, fabs(E.wr[best]))) *E.status = 3;
       }
     }
     __syncthreads();
-  }
-  // Gershgorin bound of |eigenvalues|, then the positive definite shift
-  double rmax = 0.0;
-  for (int i = threadIdx.x; i < N; i += NT) {
-    double sm = 0.0;
-    for (int j = 0; j < N; ++j) sm += fabs(E.C[(long)i * N + j]);
-    rmax = fmax(rmax, sm);
-  }
-  rmax = wave_max(rmax);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int q = 0; q < NWAVE; ++q) m = fmax(m, red[q]);
-    s_tau = m > 0.0 ? 1.0001 * m : 1.0;
-    if (!isfinite(m)) *E.status = 2;
-  }
-  __syncthreads();
-  if (!isfinite(s_tau)) return;
-  const double tau = s_tau;
-  for (int i = threadIdx.x; i < N; i += NT) E.C[(long)i * N + i] += tau;
-  __syncthreads();
-  wg_jacobi_svd(E.C, N, N, E.V, E.sig, E.perm, red);
-  // selection (thread 0; k is small)
-  if (threadIdx.x == 0) {
-    for (int j = 0; j < N; ++j) E.perm[j] = 0;  // reused as a taken-mark
+    if (*E.status != 0) return;
+    double cn = 0.0;
+    for (long e = threadIdx.x; e < NN; e += NT) cn = fmax(cn, fabs(E.C0[e]));
+    cn = wave_max(cn);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cn;
+    __syncthreads();
+    cn = 0.0;
+    for (int q = 0; q < NWAVE; ++q) cn = fmax(cn, red[q]);
     for (int s = 0; s < E.k; ++s) {
-      int best = -1;
-      double bv = 0.0;
-      for (int j = 0; j < N; ++j) {
-        if (E.perm[j]) continue;
-        const double lam = E.sig[j] - tau;
-        const double key = E.mode == 0 ? fabs(lam - E.sigma) : -lam;
-        if (best < 0 || key < bv) {
-          best = j;
-          bv = key;
-        }
-      }
-      E.perm[best] = 1;
-      E.idx[s] = best;
-      E.vals[s] = E.sig[best] - tau;
+      const double lam = E.wr[E.idx[s]];
+      const double mu = lam + 0x1p-36 * fmax(cn, fabs(lam));
+      // B = C0 - mu I in E.C (the Hessenberg form is no longer needed)
+      for (long e = threadIdx.x; e < NN; e += NT) E.C[e] = E.C0[e] - ((e / N == e % N) ? mu : 0.0);
+      for (int i = threadIdx.x; i < N; i += NT) E.xv[i] = 1.0;
+      __syncthreads();
+      wg_inverse_iteration(E.C, N, E.xv, E.perm, red, ired);
+      for (int i = threadIdx.x; i < N; i += NT) E.vecs[(long)i * E.k + s] = E.xv[i];
+      __syncthreads();
     }
+  } else {
+    // Gershgorin bound of |eigenvalues|, then the positive definite shift
+    double rmax = 0.0;
+    for (int i = threadIdx.x; i < N; i += NT) {
+      double sm = 0.0;
+      for (int j = 0; j < N; ++j) sm += fabs(E.C[(long)i * N + j]);
+      rmax = fmax(rmax, sm);
+    }
+    rmax = wave_max(rmax);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = rmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = 0.0;
+      for (int q = 0; q < NWAVE; ++q) m = fmax(m, red[q]);
+      s_tau = m > 0.0 ? 1.0001 * m : 1.0;
+      if (!isfinite(m)) *E.status = 2;
+    }
+    __syncthreads();
+    if (!isfinite(s_tau)) return;
+    const double tau = s_tau;
+    for (int i = threadIdx.x; i < N; i += NT) E.C[(long)i * N + i] += tau;
+    __syncthreads();
+    wg_jacobi_svd(E.C, N, N, E.V, E.sig, E.perm, red);
+    if (threadIdx.x == 0) {
+      for (int j = 0; j < N; ++j) E.perm[j] = 0;  // reused as a taken-mark
+      for (int s = 0; s < E.k; ++s) {
+        int best = -1;
+        double bv = 0.0;
+        for (int j = 0; j < N; ++j) {
+          if (E.perm[j]) continue;
+          const double lam = E.sig[j] - tau;
+          const double key = E.mode == 0 ? fabs(lam - E.sigma) : -lam;
+          if (best < 0 || key < bv) {
+            best = j;
+            bv = key;
+          }
+        }
+        E.perm[best] = 1;
+        E.idx[s] = best;
+        E.vals[s] = E.sig[best] - tau;
+      }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < E.k; s += NT) {
+      const int j = E.idx[s];
+      for (int i = 0; i < N; ++i) E.vecs[(long)i * E.k + s] = E.V[(long)i * N + j];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  // eigenvectors: column idx[s] of V, then L^{-T} (one vector per thread)
+  // x = L^{-T} y for the generalised problem, then the normalisation
   for (int s = threadIdx.x; s < E.k; s += NT) {
-    const int j = E.idx[s];
-    for (int i = 0; i < N; ++i) E.vecs[(long)i * E.k + s] = E.V[(long)i * N + j];
     if (E.M)
       for (int i = N - 1; i >= 0; --i) {
         double t = E.vecs[(long)i * E.k + s];
@@ -2373,16 +2707,19 @@ int ttals_truncated_svd(ttals_ctx* c, void* stream, int m, int cdim, const doubl
 }
 
 int ttals_dense_eig(ttals_ctx* c, void* stream, int N, const double* A, const double* M, int k, int mode,
-                    double sigma, int unit, double* vals, double* vecs) {
+                    double sigma, int unit, int general, double* vals, double* vecs) {
   if (!c) return TTALS_E_ARG;
   if (N < 1 || N > TTALS_DENSE_EIG_MAX || !A || k < 1 || k > N || (mode != 0 && mode != 1) || !vals || !vecs)
     return fail(c, TTALS_E_ARG, "bad dense_eig arguments (1 <= k <= N <= " + std::to_string(TTALS_DENSE_EIG_MAX) + ")");
+  if (general && mode != 0)
+    return fail(c, TTALS_E_UNSUPPORTED, "the largest eigenvalues ('eigh') of a non-symmetric matrix are not defined here");
   if (c->inflight) return fail(c, TTALS_E_ARG, "workspace busy: a submission is pending on this context");
   HIPCHK(c, hipSetDevice(c->device));
   const long NN = (long)N * N;
   Layout L;
-  const size_t oC = L.take(8 * NN), oL = L.take(M ? 8 * NN : 8), oV = L.take(8 * NN), osig = L.take(8 * N),
-               operm = L.take(4 * N), oidx = L.take(4 * k), ost = L.take(8);
+  const size_t oC = L.take(8 * NN), oC0 = L.take(general ? 8 * NN : 8), oL = L.take(M ? 8 * NN : 8),
+               oV = L.take(general ? 8 : 8 * NN), osig = L.take(8 * N), owr = L.take(8 * N), owi = L.take(8 * N),
+               ox = L.take(8 * N), operm = L.take(4 * N), oidx = L.take(4 * k), ost = L.take(8);
   int rc;
   if ((rc = ensure_dws(c, L.bytes))) return rc;
   char* db = reinterpret_cast<char*>(c->dws);
@@ -2391,13 +2728,18 @@ int ttals_dense_eig(ttals_ctx* c, void* stream, int N, const double* A, const do
   e.k = k;
   e.mode = mode;
   e.unit = unit ? 1 : 0;
+  e.general = general ? 1 : 0;
   e.sigma = sigma;
   e.A = A;
   e.M = M;
   e.C = reinterpret_cast<double*>(db + oC);
+  e.C0 = reinterpret_cast<double*>(db + oC0);
   e.Lm = reinterpret_cast<double*>(db + oL);
   e.V = reinterpret_cast<double*>(db + oV);
   e.sig = reinterpret_cast<double*>(db + osig);
+  e.wr = reinterpret_cast<double*>(db + owr);
+  e.wi = reinterpret_cast<double*>(db + owi);
+  e.xv = reinterpret_cast<double*>(db + ox);
   e.perm = reinterpret_cast<int*>(db + operm);
   e.idx = reinterpret_cast<int*>(db + oidx);
   e.vals = vals;
@@ -2414,6 +2756,9 @@ int ttals_dense_eig(ttals_ctx* c, void* stream, int N, const double* A, const do
     return fail(c, TTALS_E_UNSUPPORTED, "the local matrix of operator_gevp is not symmetric positive definite "
                                         "(only symmetric-definite generalized eigenproblems are supported)");
   if (h == 2) return ST_NONFINITE;
+  if (h == 3)
+    return fail(c, TTALS_E_UNSUPPORTED, "a selected eigenvalue is complex (real=True keeps real eigenpairs only)");
+  if (h == 4) return fail(c, TTALS_E_HIP, "the QR algorithm did not converge (60 iterations for one eigenvalue)");
   return 0;
 }
 

@@ -95,8 +95,14 @@ def als(operator, initial_guess, previous=[], shift=0, operator_gevp=None, numbe
     if previous or operator_gevp is not None or int(number_ev) != 1:
         return _als_general(operator, initial_guess, list(previous), float(shift), operator_gevp, int(number_ev),
                             int(repeats), float(conv_eps), solver, float(sigma))
-    return als_batch([operator], [initial_guess], repeats=repeats, conv_eps=conv_eps, solver=solver,
-                     sigma=sigma)[0]
+    out, infos = als_batch([operator], [initial_guess], repeats=repeats, conv_eps=conv_eps, solver=solver,
+                           sigma=sigma, return_info=True)
+    if infos.array['status'][0] == _lib.S_NONSYM and solver == 'eig':
+        # a non-symmetric operator: the reference's general local eig, dense on the device
+        return _als_general(operator, initial_guess, [], 0.0, None, 1, int(repeats), float(conv_eps), solver,
+                            float(sigma))
+    _engine.raise_first(infos, 'evp.als', 1)
+    return out[0]
 
 
 def _als_general(operator, initial_guess, previous, shift, operator_gevp, number_ev, repeats, conv_eps, solver,
@@ -110,9 +116,11 @@ def _als_general(operator, initial_guess, previous, shift, operator_gevp, number
       micro_op += shift t t^T for each previous eigentensor's local part t;
     * the local eigenpairs (evp.py:411-443) by the device dense solver:
       'eig' the number_ev nearest sigma (unit eigenvectors), 'eigh' the
-      number_ev largest (M-orthonormal), for symmetric micro_op and symmetric
-      positive definite micro_op_gevp; a non-symmetric local matrix raises
-      NotImplementedError (the reference's general dgeev/QZ path);
+      number_ev largest (M-orthonormal); symmetric micro_op by Jacobi,
+      non-symmetric micro_op ('eig' only) by the Hessenberg QR algorithm and
+      inverse iteration; micro_op_gevp must be symmetric positive definite
+      (the reference's QZ path is not built) and a complex eigenvalue among
+      the selected ones raises NotImplementedError;
     * the new core from the SVD of the local eigenvectors (evp.py:445-493):
       forward the leading min(r_{i+1}, .) left singular vectors, backward the
       leading right ones, and at position 0 of the backward half-sweep the
@@ -223,17 +231,17 @@ def _als_general(operator, initial_guess, previous, shift, operator_gevp, number
     def update(i, micro_op, micro_gevp, direction):
         N = micro_op.shape[0]
         scale = float(micro_op.abs().max())
-        if float((micro_op - micro_op.T).abs().max()) > 1e-10 * max(scale, 1e-300):
-            raise NotImplementedError('evp.als: the local matrix is not symmetric; local eigensolves of '
-                                      'non-symmetric problems (the reference\'s dgeev/QZ path) are not implemented')
+        general = float((micro_op - micro_op.T).abs().max()) > 1e-10 * max(scale, 1e-300)
+        if general and solver != 'eig':
+            raise NotImplementedError("evp.als: solver='eigh' needs a symmetric operator")
         k = min(kev, N)
         if k < kev:
             raise ValueError('number_ev exceeds the size of a local problem')
         vals = torch.empty(kev, **f64)
         vecs = torch.empty((N, kev), **f64)
         chk(lib.ttals_dense_eig(ctx, st, N, ptr(micro_op), ptr(micro_gevp) if micro_gevp is not None else None,
-                                kev, _SOLVERS[solver], sigma, 1 if solver == 'eig' else 0, ptr(vals), ptr(vecs)),
-            'dense_eig')
+                                kev, _SOLVERS[solver], sigma, 1 if solver == 'eig' else 0, 1 if general else 0,
+                                ptr(vals), ptr(vecs)), 'dense_eig')
         a, b = ranks[i], ranks[i + 1]
         if direction == 'forward':
             u, _, _ = truncated_svd(vecs.reshape(a * n[i], b * kev), threshold=0, max_rank=b)

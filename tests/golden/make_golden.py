@@ -246,11 +246,13 @@ def mals_case(ref_root):
 def evp_options_case(ref_root):
     """evp.als with the options that run outside the fused engine.
 
-    * The reference's own several-eigenpairs problem (tests/test_evp.py:
-      test_als_eigh): the Ulam operator of the triple-well transitions
-      (examples/data/triple_well_transitions.npz, coarse-grained by 2, 25 x 25
-      states, 2000 simulations), made symmetric, number_ev = 3, 'eigh', 10
-      sweeps, ones rank-15 start right-orthonormalised.
+    * The reference's own evp test problem (tests/test_evp.py): the Ulam
+      operator of the triple-well transitions (examples/data/
+      triple_well_transitions.npz, coarse-grained by 2, 25 x 25 states, 2000
+      simulations), ones rank-15 start right-orthonormalised, 10 sweeps:
+      'eig' with operator_gevp = identity and with number_ev = 3 on the
+      operator as it is (not symmetric; test_als_eig), and number_ev = 3
+      'eigh' on its symmetric part (test_als_eigh).
     * A TFIM chain (d = 6, rank-4 start): a generalised problem with a
       symmetric positive definite operator_gevp (a rank-1 TT of positive
       diagonals), 'eig' nearest sigma = -100; deflation of the ground state
@@ -268,8 +270,23 @@ def evp_options_case(ref_root):
     transitions = np.load(os.path.join(ref_root, 'examples', 'data', 'triple_well_transitions.npz'))['transitions']
     transitions = np.int64(np.ceil(np.true_divide(transitions, 2)))
     op = ulam.ulam_2d(transitions, [25, 25], 2000)
-    op = 0.5 * (op + op.transpose())
     x0 = rtt.ones(op.row_dims, [1] * op.order, ranks=15).ortho_right()
+    # the reference test's 'eig' cases on the operator as it is (not symmetric):
+    # one eigenpair with operator_gevp = identity, and three
+    lam, vec, it = revp.als(op, x0, operator_gevp=rtt.eye(op.row_dims), repeats=10)
+    _save_tt(s, 'ulam_ns_A', op.cores)
+    s['ulam_ns_val'] = np.array(float(np.real(lam)))
+    s['ulam_ns_iters'] = np.array(it)
+    _save_tt(s, 'ulam_ns_vec', [np.real(c) for c in vec.cores])
+    vals, vecs, it = revp.als(op, x0, number_ev=3, repeats=10)
+    s['ulam_ns3_vals'] = np.real(np.array(vals, dtype=complex))
+    s['ulam_ns3_iters'] = np.array(it)
+    for e in range(3):
+        _save_tt(s, 'ulam_ns3_vec%d' % e, [np.real(c) for c in vecs[e].cores])
+    dense_ns = np.linalg.eigvals(op.matricize())
+    s['ulam_ns_dense_top3'] = np.real(dense_ns[np.argsort(-np.real(dense_ns))][:3])
+    s['ulam_ns_complex_parts'] = np.array(max(float(np.max(np.abs(np.imag(c)))) for c in vecs[0].cores))
+    op = 0.5 * (op + op.transpose())
     vals, vecs, it = revp.als(op, x0, number_ev=3, repeats=10, solver='eigh')
     _save_tt(s, 'ulam_A', op.cores)
     _save_tt(s, 'ulam_x0', x0.cores)
@@ -316,7 +333,8 @@ def evp_options_case(ref_root):
     for e in range(2):
         _save_tt(s, 'two_vec%d' % e, vecs[e].cores)
     np.savez_compressed(os.path.join(HERE, 'evp_options.npz'), **s)
-    print('evp options: ulam', s['ulam_vals'], s['ulam_dense_top3'], 'gevp', s['gevp_val'], s['gevp_dense'],
+    print('evp options: ulam eig', s['ulam_ns_val'], s['ulam_ns3_vals'], s['ulam_ns_dense_top3'],
+          'imag', s['ulam_ns_complex_parts'], 'ulam eigh', s['ulam_vals'], s['ulam_dense_top3'], 'gevp', s['gevp_val'], s['gevp_dense'],
           'deflation', s['defl_val'], s['dense_lowest2'], 'two', s['two_vals'])
 
 

@@ -607,9 +607,12 @@ def _nonsym_slices_symmetric_operator(rng, d, n, scale=0.3):
 
 
 def test_evp_nonsymmetric_operator_raises(env):
-    """evp.als with an operator that is not symmetric raises NotImplementedError
-    (the reference's 'eig' would run dgeev, scikit_tt/solvers/evp.py:424-432)
-    instead of returning the eigenpair of a symmetric solver."""
+    """evp.als with an operator that is not symmetric never returns the
+    eigenpair of a symmetric solver: 'eigh' raises NotImplementedError; 'eig'
+    runs the general local eigensolver, whose selected eigenvalues here are
+    complex (the operator's nearest to sigma are), which raises too (the
+    reference would keep their real parts and complex eigenvectors,
+    scikit_tt/solvers/evp.py:424-443)."""
     import scikit_tt_amd.solvers.evp as evp
     rng = np.random.default_rng(11)
     A = W.random_tt(rng, [3] * 4, [1, 2, 2, 2, 1], col_dims=[3] * 4)

@@ -442,6 +442,12 @@ def test_evp_options_fixture_against_dense():
     top3 = np.linalg.eigvalsh(0.5 * (A + A.T))[::-1][:3]
     np.testing.assert_allclose(top3, g['ulam_dense_top3'], rtol=1e-12)
     assert np.all(np.abs(g['ulam_vals'] - top3) <= 5e-3 * np.abs(top3))
+    A = tt_ops.matricize(g['tt']('ulam_ns_A'))
+    ev = np.linalg.eigvals(A)
+    top = np.real(ev[np.argsort(-np.real(ev))][:3])
+    np.testing.assert_allclose(top, g['ulam_ns_dense_top3'], rtol=1e-10)
+    assert abs(float(g['ulam_ns_val']) - 1.0) <= 5e-3
+    assert np.all(np.abs(g['ulam_ns3_vals'] - top) <= 5e-3 * np.abs(top))
     H = tt_ops.matricize(g['tt']('tfim_A'))
     M = tt_ops.matricize(g['tt']('tfim_M'))
     assert np.all(np.linalg.eigvalsh(M) > 0)
