@@ -190,6 +190,8 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
     }
     case P_QR: {
       // m = a, c = b; input x (m x c); out = Q (m x k); v = R (k x c); t1 = work
+      // (W, then tau, then (c + 16) * 16 for the blocked QR's T); the
+      // engine's choice of method for a global-memory panel (qr_factor)
       const int m = A.a, c = A.b;
       double* W = A.t1;
       for (long e = threadIdx.x; e < (long)m * c; e += NT) W[e] = A.x[e];
@@ -198,6 +200,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         wg_qr_small(W, m, c, A.t2, A.out, A.v);
         break;
       }
+      if (wg_hhqr_blocked(W, m, c, A.out, A.v, A.t2 + (m > c ? m : c), dyn_lds, A.arena, S.red)) break;
       wg_householder(W, m, c, A.t2, S.red);
       wg_form_q(W, m, c, A.t2, A.out);
       const int k = m < c ? m : c;
@@ -2472,7 +2475,7 @@ int ttals_local_eig(ttals_ctx* c, void* stream, int a, int P, int n, int Q, int 
 
 int ttals_qr(ttals_ctx* c, void* stream, int m, int cdim, const double* Y, double* Q, double* R) {
   if (!dims_ok({m, cdim})) return fail(c, TTALS_E_ARG, "bad dims");
-  PIECE_PROLOGUE((long)m * cdim + std::max(m, cdim), 1)
+  PIECE_PROLOGUE((long)m * cdim + std::max(m, cdim) + ((long)cdim + 16) * 16, 1)
   pa.op = P_QR;
   pa.a = m; pa.b = cdim;
   pa.x = Y; pa.out = Q; pa.v = R;

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ttals {
 
 constexpr int NT = 256;            // threads per workgroup: 4 x wave64
@@ -236,8 +238,130 @@ __device__ __forceinline__ int gmap_at(const GMap& g, int i) {
   return q * g.hi + (i - q * g.div) * g.lo;
 }
 
+typedef double dx4_t __attribute__((ext_vector_type(4)));
+
+// FP64 GEMM over global (L2/HBM) operands on the matrix cores.  64x64 output
+// tiles, K-steps of 16 staged through LDS ([GBK][GBM+1] and [GBK][GBN+1]:
+// the odd stride keeps the transposing stores of k-fast operands off a single
+// bank); every wave owns a 32x32 quadrant of the tile (2x2
+// v_mfma_f64_16x16x4f64 tiles).  The (tile, K-step) sequence is one
+// pipeline: the operands of step s+1 -- across a tile boundary too -- are
+// loaded into registers while the MFMAs of step s run, so the load latency
+// of L2/HBM overlaps the arithmetic instead of adding to it.  Elements past
+// M, N or K are staged as zeros (their products vanish; rows/columns past M,
+// N are not written).
+__device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A, CView<double> B,
+                                           MView<double> C, double* lds_) {
+  TT_LDS double* As = (TT_LDS double*)lds_;              // [GBK][GBM+1]
+  TT_LDS double* Bs = (TT_LDS double*)lds_ + GBK * (GBM + 1);  // [GBK][GBN+1]
+  static_assert(GBM == 64 && GBN == 64 && GBK == 16 && NT == 256, "tile shape assumed below");
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (w & 1) << 5, wn = (w >> 1) << 5;
+  const GMap Ar = gmap(A.r), Ac = gmap(A.c), Br = gmap(B.r), Bc = gmap(B.c);
+  // contiguous direction of each operand decides which index is thread-fast (coalescing)
+  const bool a_kfast = (A.c.lo == 1 && A.c.div >= K);
+  const bool b_nfast = (B.c.lo == 1 && B.c.div >= N);
+  // per-thread element coordinates in a (64 x 16) / (16 x 64) stage, j = 0..3
+  const int a_i0 = a_kfast ? (tid >> 4) : (tid & 63), a_k0 = a_kfast ? (tid & 15) : (tid >> 6);
+  const int a_di = a_kfast ? 16 : 0, a_dk = a_kfast ? 0 : 4;
+  const int b_n0 = b_nfast ? (tid & 63) : (tid >> 4), b_k0 = b_nfast ? (tid >> 6) : (tid & 15);
+  const int b_dn = b_nfast ? 0 : 16, b_dk = b_nfast ? 4 : 0;
+  const int tn = (N + GBN - 1) / GBN, ks = (K + GBK - 1) / GBK;
+  const int steps = ((M + GBM - 1) / GBM) * tn * ks;
+  double ra[4], rb[4];
+  // loader position (tile origin, K offset) and compute position, advanced without division
+  int lm0 = 0, ln0 = 0, lk0 = 0, ltj = 0;
+  int cm0 = 0, cn0 = 0, ckk = 0, ctj = 0;
+  auto load = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gm = lm0 + a_i0 + j * a_di, gk = lk0 + a_k0 + j * a_dk;
+      ra[j] = (gm < M && gk < K) ? A.p[gmap_at(Ar, gm) + gmap_at(Ac, gk)] : 0.0;
+      const int gn = ln0 + b_n0 + j * b_dn, gk2 = lk0 + b_k0 + j * b_dk;
+      rb[j] = (gn < N && gk2 < K) ? B.p[gmap_at(Br, gk2) + gmap_at(Bc, gn)] : 0.0;
+    }
+    lk0 += GBK;
+    if (lk0 >= K) {
+      lk0 = 0;
+      ln0 += GBN;
+      if (++ltj == tn) {
+        ltj = 0;
+        ln0 = 0;
+        lm0 += GBM;
+      }
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      As[(a_k0 + j * a_dk) * (GBM + 1) + a_i0 + j * a_di] = ra[j];
+      Bs[(b_k0 + j * b_dk) * (GBN + 1) + b_n0 + j * b_dn] = rb[j];
+    }
+  };
+  dx4_t acc[2][2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int g = 0; g < 2; ++g) acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+  if (steps > 0) {
+    load();
+    stage();
+  }
+  __syncthreads();
+  for (int s = 0; s < steps; ++s) {
+    if (s + 1 < steps) load();
+#pragma unroll
+    for (int q = 0; q < GBK / 4; ++q) {
+      const int kr = (4 * q + lk);
+      const double a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
+      const double b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    ckk += GBK;
+    if (ckk >= K) {
+      // tile done: lane l holds C[i0 + l/16 + 4e][j0 + l%16] of each 16x16 block
+      const GMap Cr = gmap(C.r), Cc = gmap(C.c);
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int gn = cn0 + wn + 16 * g + li;
+        const bool nv = gn < N;
+        const int cc = nv ? gmap_at(Cc, gn) : 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int gm = cm0 + wm + 16 * h + lk + 4 * e;
+            if (nv && gm < M) C.p[gmap_at(Cr, gm) + cc] = acc[h][g][e];
+          }
+          acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+        }
+      }
+      ckk = 0;
+      cn0 += GBN;
+      if (++ctj == tn) {
+        ctj = 0;
+        cn0 = 0;
+        cm0 += GBM;
+      }
+    }
+    __syncthreads();
+    if (s + 1 < steps) {
+      stage();
+      __syncthreads();
+    }
+  }
+}
+
 template <class T, class ACC>
 __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
+  if constexpr (std::is_same<T, double>::value && std::is_same<ACC, double>::value) {
+    wg_gemm_mf(M, N, K, A, B, C, lds);
+    return;
+  }
   ACC* As = lds;                      // [GBK][GBM+1]
   ACC* Bs = lds + GBK * (GBM + 1);    // [GBK][GBN+1]
   const int tid = threadIdx.x;
@@ -367,8 +491,6 @@ __device__ void ds_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C)
 // 4-wide accumulator is C[i0 + l/16 + 4*e][j0 + l%16] (checked on gfx950).  Each lane loads
 // one A and one B value per 16 FMAs, against four loads per four FMAs for a
 // 2x2 register tile: the local matvec is LDS-bound, so this is what counts.
-
-typedef double dx4_t __attribute__((ext_vector_type(4)));
 
 // offsets of k, k+4, k+8, ... along an IMap, advanced without division
 struct KStep4 {
@@ -1031,6 +1153,71 @@ __device__ __forceinline__ void hh_apply(double* X, int ldx, int m, int j, const
   }
 }
 
+// The c x c part of one Cholesky QR pass: Rp holds the (shifted) Gram matrix
+// on entry.  Cholesky Rp = R^T R (upper R in place, right-looking, lower part
+// zeroed), then Rt <- R Rt (the accumulated factor; row by row in place, row
+// p reads rows >= p of the old Rt), then Rp <- R^{-1} (column by column: for
+// i < j, inv[i][j] = -(sum_{k=i}^{j-1} inv[i][k] R[k][j]) / R[j][j]).  tmp
+// holds c doubles.  Returns false (nothing else changed after the failing
+// column) when a pivot is not positive and finite: the Gram matrix is
+// numerically singular.
+__device__ __forceinline__ bool cholqr_small_step(double* Rp, double* Rt, double* tmp, int c) {
+  for (int j = 0; j < c; ++j) {
+    const double piv = Rp[(long)j * c + j];
+    if (!(piv > 0.0) || !isfinite(piv)) return false;
+    const double rjj = sqrt(piv), inv = 1.0 / rjj;
+    __syncthreads();
+    for (int q = j + 1 + threadIdx.x; q < c; q += NT) {
+      Rp[(long)j * c + q] *= inv;
+      Rp[(long)q * c + j] = 0.0;
+    }
+    if (threadIdx.x == 0) Rp[(long)j * c + j] = rjj;
+    __syncthreads();
+    const int rest = c - j - 1;
+    for (int e = threadIdx.x; e < rest * rest; e += NT) {
+      const int p = j + 1 + e / rest, q = j + 1 + e % rest;
+      if (q >= p) Rp[(long)p * c + q] -= Rp[(long)j * c + p] * Rp[(long)j * c + q];
+    }
+    __syncthreads();
+  }
+  for (int p = 0; p < c; ++p) {
+    for (int q = threadIdx.x; q < c; q += NT) {
+      double t = 0.0;
+      for (int k2 = p; k2 <= q; ++k2) t += Rp[(long)p * c + k2] * Rt[(long)k2 * c + q];
+      tmp[q] = (q >= p) ? t : 0.0;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < c; q += NT) Rt[(long)p * c + q] = tmp[q];
+    __syncthreads();
+  }
+  for (int j = 0; j < c; ++j) {
+    const double rjj = Rp[(long)j * c + j];
+    for (int i = threadIdx.x; i <= j; i += NT) {
+      if (i == j) {
+        tmp[i] = 1.0 / rjj;
+      } else {
+        double t = 0.0;
+        for (int k2 = i; k2 < j; ++k2) t += Rp[(long)i * c + k2] * Rp[(long)k2 * c + j];
+        tmp[i] = -t / rjj;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j; i += NT) Rp[(long)i * c + j] = tmp[i];
+    __syncthreads();
+  }
+  return true;
+}
+
+// The shift of the first shifted Cholesky QR pass for an (m x c) panel whose
+// Gram matrix has trace tr (= ||W||_F^2): 11 (m c + c (c + 1)) u tr keeps the
+// shifted Gram matrix positive definite in floating point for condition
+// numbers up to ~1/u (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa,
+// "Shifted Cholesky QR for computing the QR factorization of ill-conditioned
+// matrices", SIAM J. Sci. Comput. 42 (2020)).
+__device__ __forceinline__ double cholqr_shift(int m, int c, double tr) {
+  return 11.0 * ((double)m * c + (double)c * (c + 1)) * 0x1p-53 * tr;
+}
+
 __device__ void wg_householder(double* W, int m, int c, double* tau, double* red) {
   const int k = m < c ? m : c;
   for (int j = 0; j < k; ++j) {
@@ -1264,6 +1451,285 @@ __device__ void wg_form_q(const double* W, int m, int c, const double* tau, doub
     if (t != 0.0) hh_apply(Qm, k, m, j, W + j, c, j, k, t);
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Householder QR of a tall row-major (m x c) panel W in global memory
+// (m >= c): the structure of LAPACK's dgeqrf / dlarft / dlarfb / dorgqr.
+// Column blocks of NB are factored with the block in LDS (reflectors in
+// wg_householder's dlarfg convention); each block's compact WY form
+// I - V T V^T then updates the trailing columns, and backwards forms the
+// explicit Q, as two streaming passes over the rows per block (Y = V^T X,
+// X -= V (T' Y)) where the unblocked sweep makes two per column.  The Gram
+// entries V^T V that T needs ride along with the column steps' reductions.
+
+template <int NB>
+struct HHBlockLDS {
+  // rows x (NB + 1) block (odd stride: the rows of neighbouring threads fall
+  // into different banks), then the reduction rows, the Y chunk, T and tau
+  static __host__ __device__ long need(int m) { return (long)m * (NB + 1) + NWAVE * NB + NB * 64 + NB * NB + NB; }
+};
+
+// V(i, p) of the block's reflectors: 1 on the diagonal, 0 above, Pn below.
+template <int NB>
+__device__ __forceinline__ double hh_v(const double* Pn, int i, int p) {
+  return i > p ? Pn[i * (NB + 1) + p] : (i == p ? 1.0 : 0.0);
+}
+
+// Factor the block Pn (rows x kb, ld NB + 1) in place (R on and above the
+// diagonal, reflectors below), tau[0..kb) and the upper part of G = V^T V
+// (G[l * NB + j], l < j).  dred: NWAVE * NB doubles; red: NWAVE doubles.
+// Thread t owns the rows t, t + NT, ... in every column step (rows below the
+// step's pivot only): a row is written and read by one thread, so the steps
+// need no barrier beyond their reductions'.
+template <int NB>
+__device__ void hh_block_factor(double* Pn, int rows, int kb, double* tau, double* G, double* dred, double* red) {
+  constexpr int ld = NB + 1;
+  static_assert(NB < NT, "a block's pivot rows lie in the first row of every thread's set");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  for (int j = 0; j < kb; ++j) {
+    // first own row > j / >= j (j < NB < NT: only the first set can lie below)
+    const int i_gt = tid > j ? tid : tid + NT, i_ge = tid >= j ? tid : tid + NT;
+    double s2 = 0.0;
+    for (int i = i_gt; i < rows; i += NT) {
+      const double t = Pn[i * ld + j];
+      s2 += t * t;
+    }
+    s2 = wg_sum(s2, red);  // (barriers: the previous column's updates are visible)
+    const double alpha = Pn[j * ld + j];
+    double t = 0.0, beta = alpha, sc = 0.0;
+    if (s2 != 0.0) {
+      const double nrm = sqrt(alpha * alpha + s2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    // own rows: scale the reflector; partial dots of v with the later
+    // columns (q > j, for the update) and of the earlier reflectors with v
+    // (l < j, the Gram entries G[l][j])
+    double acc[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) acc[q] = 0.0;
+    for (int i = i_ge; i < rows; i += NT) {
+      double v = 1.0;
+      if (i > j) {
+        v = (t != 0.0) ? Pn[i * ld + j] * sc : 0.0;
+        if (t != 0.0) Pn[i * ld + j] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        if (q != j && q < kb) acc[q] += v * Pn[i * ld + q];
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q != j && q < kb) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) dred[w * NB + q] = acc[q];
+    }
+    __syncthreads();
+    double wq[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      double sm = 0.0;
+#pragma unroll
+      for (int u = 0; u < NWAVE; ++u) sm += dred[u * NB + q];
+      wq[q] = sm;
+    }
+    if (t != 0.0) {
+      for (int i = i_ge; i < rows; i += NT) {
+        const double v = (i == j) ? 1.0 : Pn[i * ld + j];
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          if (q > j && q < kb) Pn[i * ld + q] -= t * v * wq[q];
+      }
+    }
+    if (tid == j) Pn[j * ld + j] = beta;  // (the owner of row j < NT)
+    if (threadIdx.x == 0) {
+      tau[j] = t;
+      for (int l = 0; l < j; ++l) G[l * NB + j] = wq[l];
+    }
+  }
+  __syncthreads();
+}
+
+// T of the block (dlarft, forward, columnwise): T[i][i] = tau_i,
+// T[0:i, i] = -tau_i T[0:i, 0:i] G[0:i, i]; NB x NB, upper.
+template <int NB>
+__device__ void hh_block_t(const double* tau, const double* G, int kb, double* T) {
+  for (int e = threadIdx.x; e < NB * NB; e += NT) T[e] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < kb; ++i) {
+    const int j = threadIdx.x;
+    double v = 0.0;
+    if (j < i) {
+      for (int l = j; l < i; ++l) v += T[j * NB + l] * G[l * NB + i];
+    }
+    __syncthreads();
+    if (j < i) T[j * NB + i] = -tau[i] * v;
+    if (j == i) T[i * NB + i] = tau[i];
+    __syncthreads();
+  }
+}
+
+// X (rows x nc, global, ld ldx) <- (I - V T' V^T) X with T' = T^T (trans:
+// H^T X, the factorisation's trailing update) or T (forming Q).  Lanes over
+// 64-column chunks, waves over rows (HH_U rows loaded before any is used);
+// the waves' partial Y meet in Yb (NB x 64 doubles) in a fixed order.
+// rows in flight per wave (more push the engine kernels past 256 registers)
+constexpr int HH_U = 2;
+template <int NB>
+__device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const double* Pn, int kb, const double* T,
+                               bool trans, double* Yb) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < nc; c0 += 64) {
+    const int col = c0 + lane;
+    const bool on = col < nc;
+    const int cc = on ? col : 0;
+    double y[NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) y[p] = 0.0;
+    int i = w;
+    for (; i + (HH_U - 1) * NWAVE < rows; i += HH_U * NWAVE) {
+      double x[HH_U];
+#pragma unroll
+      for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * NWAVE) * ldx + cc];
+#pragma unroll
+      for (int u = 0; u < HH_U; ++u)
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
+          if (p < kb) y[p] += hh_v<NB>(Pn, i + u * NWAVE, p) * x[u];
+    }
+    for (; i < rows; i += NWAVE) {
+      const double x = X[(long)i * ldx + cc];
+#pragma unroll
+      for (int p = 0; p < NB; ++p)
+        if (p < kb) y[p] += hh_v<NB>(Pn, i, p) * x;
+    }
+    for (int u = 0; u < NWAVE; ++u) {
+      if (w == u) {
+#pragma unroll
+        for (int p = 0; p < NB; ++p) Yb[p * 64 + lane] = (u == 0 ? 0.0 : Yb[p * 64 + lane]) + y[p];
+      }
+      __syncthreads();
+    }
+    // Z = T' Y in place, one wave (a lane per column; Y's column in registers)
+    if (w == 0) {
+#pragma unroll
+      for (int p = 0; p < NB; ++p) y[p] = Yb[p * 64 + lane];
+#pragma unroll 1
+      for (int p = 0; p < NB; ++p) {
+        double sm = 0.0;
+#pragma unroll
+        for (int l = 0; l < NB; ++l) {
+          const double tv = trans ? (l <= p ? T[l * NB + p] : 0.0) : (l >= p ? T[p * NB + l] : 0.0);
+          sm += tv * y[l];
+        }
+        Yb[p * 64 + lane] = sm;
+      }
+    }
+    __syncthreads();
+    double z[NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) z[p] = Yb[p * 64 + lane];
+    if (on) {
+      i = w;
+      for (; i + (HH_U - 1) * NWAVE < rows; i += HH_U * NWAVE) {
+        double x[HH_U];
+#pragma unroll
+        for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * NWAVE) * ldx + col];
+#pragma unroll
+        for (int u = 0; u < HH_U; ++u) {
+          double sm = 0.0;
+#pragma unroll
+          for (int p = 0; p < NB; ++p)
+            if (p < kb) sm += hh_v<NB>(Pn, i + u * NWAVE, p) * z[p];
+          X[(long)(i + u * NWAVE) * ldx + col] = x[u] - sm;
+        }
+      }
+      for (; i < rows; i += NWAVE) {
+        double sm = 0.0;
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
+          if (p < kb) sm += hh_v<NB>(Pn, i, p) * z[p];
+        X[(long)i * ldx + col] -= sm;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NB>
+__device__ __attribute__((noinline)) void hhqr_blocked_run(double* W, int m, int c, double* Qm, double* Rm,
+                                                           double* tws, double* lds, double* red) {
+  constexpr int ld = NB + 1;
+  double* Pn = lds;
+  double* dred = Pn + (long)m * ld;
+  double* Yb = dred + NWAVE * NB;
+  double* T = Yb + NB * 64;
+  double* tau = T + NB * NB;
+  const int k = c;  // m >= c
+  for (int jb = 0; jb < k; jb += NB) {
+    const int kb = (k - jb) < NB ? (k - jb) : NB, rows = m - jb;
+    for (int e = threadIdx.x; e < rows * kb; e += NT) {
+      const int i = e / kb, q = e - i * kb;
+      Pn[i * ld + q] = W[(long)(jb + i) * c + jb + q];
+    }
+    __syncthreads();
+    double* G = tws + (long)jb * NB;  // the block's Gram rows, then its T
+    hh_block_factor<NB>(Pn, rows, kb, tau, G, dred, red);
+    for (int e = threadIdx.x; e < rows * kb; e += NT) {
+      const int i = e / kb, q = e - i * kb;
+      W[(long)(jb + i) * c + jb + q] = Pn[i * ld + q];
+    }
+    hh_block_t<NB>(tau, G, kb, T);
+    for (int e = threadIdx.x; e < NB * NB; e += NT) G[e] = T[e];
+    if (jb + kb < c) hh_block_apply<NB>(W + (long)jb * c + jb + kb, c, rows, c - jb - kb, Pn, kb, T, true, Yb);
+    __syncthreads();
+  }
+  // R, then Q = H_0 ... H_{k-1} I (backwards over the blocks)
+  for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+    const int i = (int)(e / c), j = (int)(e - (long)i * c);
+    Rm[e] = (j >= i) ? W[e] : 0.0;
+  }
+  for (long e = threadIdx.x; e < (long)m * k; e += NT) {
+    const int i = (int)(e / k), j = (int)(e - (long)i * k);
+    Qm[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int nblk = (k + NB - 1) / NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int jb = b * NB, kb = (k - jb) < NB ? (k - jb) : NB, rows = m - jb;
+    for (int e = threadIdx.x; e < rows * kb; e += NT) {
+      const int i = e / kb, q = e - i * kb;
+      Pn[i * ld + q] = W[(long)(jb + i) * c + jb + q];
+    }
+    for (int e = threadIdx.x; e < NB * NB; e += NT) T[e] = tws[(long)jb * NB + e];
+    __syncthreads();
+    hh_block_apply<NB>(Qm + (long)jb * k + jb, k, rows, k - jb, Pn, kb, T, false, Yb);
+  }
+  __syncthreads();
+}
+
+// Blocked Householder QR (see above) of W (m x c, m >= c, global memory):
+// Q (m x c, ld c) into Qm, R (c x c) into Rm, reflectors left in W; tws
+// (global) holds (c + 8) * 8 doubles.  The block (8 or 4 columns) is the
+// larger whose LDS need fits min(cap, 61 KB) -- a bound independent of the
+// engine's arena, so every engine factors a panel alike.  Returns false when
+// neither fits (the caller runs the unblocked sweep).
+constexpr long HHQR_LDS_CAP = 61 * 1024 / 8;
+__device__ bool wg_hhqr_blocked(double* W, int m, int c, double* Qm, double* Rm, double* tws, double* lds, long cap,
+                                double* red) {
+  if (m < c || m >= (1 << 22) / 17) return false;
+  const long lim = cap < HHQR_LDS_CAP ? cap : HHQR_LDS_CAP;
+  if (HHBlockLDS<8>::need(m) <= lim)
+    hhqr_blocked_run<8>(W, m, c, Qm, Rm, tws, lds, red);
+  else if (HHBlockLDS<4>::need(m) <= lim)
+    hhqr_blocked_run<4>(W, m, c, Qm, Rm, tws, lds, red);
+  else
+    return false;
+  return true;
 }
 
 // ---------------------------------------------------------------------------

@@ -83,22 +83,35 @@ __device__ __forceinline__ void set_one(double* p) {
   if (threadIdx.x == 0) p[0] = 1.0;
 }
 
-// Buffers of one QR: W (m x c), Q (m x k), R (k x c), tau (k).
+// Buffers of one QR: W (m x c), Q (m x k), R (k x c), tau (k).  With `lds`
+// set (global-memory panels: the arena is free), a tall panel is factored by
+// the blocked Householder QR: lds / lds_cap its LDS, tws its global T
+// storage.
 struct QRBuf {
   double* W;
   double* Q;
   double* R;
   double* tau;
+  double* lds = nullptr;
+  long lds_cap = 0;
+  double* tws = nullptr;
 };
 
 // On-chip QR buffers carved from the arena when m x c fits, else the global
-// scratch of the system.
+// scratch of the system (then the arena is free for the blocked QR's block,
+// and its T storage is the dense-LU buffer, which no QR overlaps).
 __device__ __forceinline__ QRBuf qr_buffers(const SysDesc& D, int m, int c, double* arena, long cap) {
   const long k = m < c ? m : c;
   const long need = (long)m * c + (long)m * k + k * c + k;
   if (need <= cap) return QRBuf{arena, arena + (long)m * c, arena + (long)m * c + (long)m * k,
                                  arena + (long)m * c + (long)m * k + k * c};
-  return QRBuf{D.qw, D.qq, D.rf, D.tau};
+  QRBuf B{D.qw, D.qq, D.rf, D.tau};
+  if (cap >= GEMM_LDS_DOUBLES && ((long)c + 16) * 16 <= (long)NDENSE * NDENSE) {
+    B.lds = arena;
+    B.lds_cap = cap;
+    B.tws = D.dense;
+  }
+  return B;
 }
 
 // Room for `len` more doubles in the arena after the on-chip QR buffers of an
@@ -110,11 +123,8 @@ __device__ __forceinline__ double* qr_stage(const QRBuf& B, int m, int c, long l
   return used + len <= cap ? arena + used : nullptr;
 }
 
-// v is (m x c) row-major (forward) or (c x m) row-major transposed
-// (backward).  Householder QR of the (m x c) view: explicit Q (m x k) into
-// B.Q, the clean upper-triangular R (k x c) into B.R.  Returns k.
-__device__ int qr_factor(const double* v, int m, int c, bool transposed, const QRBuf& B, SolveLDS& S) {
-  double* W = B.W;
+// W <- the (m x c) view of v: v itself (row-major) or, transposed, v (c x m).
+__device__ __forceinline__ void qr_load_panel(double* W, const double* v, int m, int c, bool transposed) {
   const long mc = (long)m * c;
   if (!transposed) {
     for (long e = threadIdx.x; e < mc; e += NT) W[e] = v[e];
@@ -132,11 +142,22 @@ __device__ int qr_factor(const double* v, int m, int c, bool transposed, const Q
     }
   }
   __syncthreads();
+}
+
+// v is (m x c) row-major (forward) or (c x m) row-major transposed
+// (backward).  Householder QR of the (m x c) view: explicit Q (m x k) into
+// B.Q, the clean upper-triangular R (k x c) into B.R.  Returns k.  Small
+// panels: the register QR (wg_qr_small); tall global-memory panels: the
+// blocked form (wg_hhqr_blocked); else the column sweep.
+__device__ int qr_factor(const double* v, int m, int c, bool transposed, const QRBuf& B, SolveLDS& S) {
+  double* W = B.W;
+  qr_load_panel(W, v, m, c, transposed);
   const int k = m < c ? m : c;
   if (m <= 64 && c <= 64) {
     wg_qr_small(W, m, c, B.tau, B.Q, B.R);
     return k;
   }
+  if (B.lds && wg_hhqr_blocked(W, m, c, B.Q, B.R, B.tws, B.lds, B.lds_cap, S.red)) return k;
   wg_householder(W, m, c, B.tau, S.red);
   wg_form_q(W, m, c, B.tau, B.Q);
   for (long e = threadIdx.x; e < (long)k * c; e += NT) {
@@ -410,24 +431,31 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       st.flops += 3.0 * (double)N * N * P * Q + (2.0 / 3.0) * (double)N * N * N + 2.0 * (double)N * N;
     } else {
       int r = 1;
-      if (sym) r = onchip ? wg_cg<ACC, true>(op, W, st.tol, D.maxit, lds, S, &its)
-                          : wg_cg<ACC, false>(op, W, st.tol, D.maxit, lds, S, &its);
-      if (r == 1) r = wg_gmres<ACC>(op, W, st.tol, D.maxit, D.gm_restart, lds, S, &its);
+      KrylovOut ko;
+      if (sym) r = onchip ? wg_cg<ACC, true>(op, W, st.tol, D.maxit, lds, S, &its, &ko)
+                          : wg_cg<ACC, false>(op, W, st.tol, D.maxit, lds, S, &its, &ko);
+      if (r == 1) r = wg_gmres<ACC>(op, W, st.tol, D.maxit, D.gm_restart, lds, S, &its, &ko);
       if (r != 0) ++st.nunconv;
-      // final true relative residual (diagnostic)
-      apply_op<double, ACC>(op, W.x, W.w, lds);
-      for (int e = threadIdx.x; e < N; e += NT) W.w[e] = W.f[e] - W.w[e];
-      __syncthreads();
-      const double rn = wg_nrm2(W.w, N, S.red), fn = wg_nrm2(W.f, N, S.red);
+      // final true relative residual (diagnostic): the solvers' own final
+      // check where they made one, else one more matvec
+      double rn = ko.tres;
+      if (rn < 0.0) {
+        apply_op<double, ACC>(op, W.x, W.w, lds);
+        ++ko.nmv;
+        for (int e = threadIdx.x; e < N; e += NT) W.w[e] = W.f[e] - W.w[e];
+        __syncthreads();
+        rn = wg_nrm2(W.w, N, S.red);
+      }
+      const double fn = wg_nrm2(W.f, N, S.red);
       st.last_res = fn > 0.0 ? rn / fn : rn;
       st.worst = fmax(st.worst, st.last_res);
       if (onchip) {
         wg_copy(W.x, D.g, N);
         __syncthreads();
       }
-      // matvecs (Krylov steps, the initial and the final residual) + the
-      // vector work the solver counted as it ran (Gram-Schmidt passes as made)
-      st.flops += (its + 2) * fmv + S.vfl + 6.0 * N;
+      // the matvecs made + the vector work the solver counted as it ran
+      // (Gram-Schmidt passes as made)
+      st.flops += ko.nmv * fmv + S.vfl + 6.0 * N;
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
@@ -473,6 +501,8 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       __syncthreads();
       mf_gemm_lin(k, nc, b, B.R, b, 1, xs, nc, 1, gs, fmap(1, nc, 0), fmap(1, 1, 0), gs + (long)k * nc, 768);
       wg_copy(gs, D.g, k * nc);
+    } else if (B.lds) {
+      wg_gemm<double, ACC>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc), lds);
     } else {
       ds_gemm<double, ACC>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc));
     }
@@ -507,6 +537,8 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       __syncthreads();
       mf_gemm_lin(cn, k, a, xs, a, 1, B.R, 1, a, gs, fmap(1, k, 0), fmap(1, 1, 0), gs + (long)cn * k, 768);
       wg_copy(gs, D.g, cn * k);
+    } else if (B.lds) {
+      wg_gemm<double, ACC>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k), lds);
     } else {
       ds_gemm<double, ACC>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
     }
@@ -533,7 +565,10 @@ __device__ void ortho_right_x(const SysDesc& D, ACC* lds, long arena, SolveLDS& 
       D.xc[i][e] = B.Q[(long)c * k + r];
     }
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
-    ds_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+    if (B.lds)
+      wg_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k), lds);
+    else
+      ds_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
     wg_copy(D.g, D.xc[i - 1], c0 * n0 * k);
     if (threadIdx.x == 0) D.rx[i] = k;
     __syncthreads();

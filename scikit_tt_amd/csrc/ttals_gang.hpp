@@ -731,58 +731,13 @@ __device__ int gang_cholqr(GangCtx& C, const double* g, int m, int c, bool trans
       double tr = 0.0;
       for (int j = 0; j < c; ++j) tr += Rp[(long)j * c + j];
       if (!isfinite(tr)) return 2;
-      const double shift = 11.0 * ((double)m * c + (double)c * (c + 1)) * 0x1p-53 * tr;
+      const double shift = cholqr_shift(m, c, tr);
       __syncthreads();
       for (int j = threadIdx.x; j < c; j += NT) Rp[(long)j * c + j] += shift;
       __syncthreads();
     }
-    // Cholesky Gram = R^T R (upper R in place, right-looking; lower part zeroed)
-    for (int j = 0; j < c; ++j) {
-      const double piv = Rp[(long)j * c + j];
-      if (!(piv > 0.0) || !isfinite(piv)) return 1;
-      const double rjj = sqrt(piv), inv = 1.0 / rjj;
-      __syncthreads();
-      for (int q = j + 1 + threadIdx.x; q < c; q += NT) {
-        Rp[(long)j * c + q] *= inv;
-        Rp[(long)q * c + j] = 0.0;
-      }
-      if (threadIdx.x == 0) Rp[(long)j * c + j] = rjj;
-      __syncthreads();
-      const int rest = c - j - 1;
-      for (int e = threadIdx.x; e < rest * rest; e += NT) {
-        const int p = j + 1 + e / rest, q = j + 1 + e % rest;
-        if (q >= p) Rp[(long)p * c + q] -= Rp[(long)j * c + p] * Rp[(long)j * c + q];
-      }
-      __syncthreads();
-    }
-    // Rt <- Rp Rt, row by row in place (row p reads rows >= p of the old Rt)
-    for (int p = 0; p < c; ++p) {
-      for (int q = threadIdx.x; q < c; q += NT) {
-        double t = 0.0;
-        for (int k2 = p; k2 <= q; ++k2) t += Rp[(long)p * c + k2] * Rt[(long)k2 * c + q];
-        tmp[q] = (q >= p) ? t : 0.0;
-      }
-      __syncthreads();
-      for (int q = threadIdx.x; q < c; q += NT) Rt[(long)p * c + q] = tmp[q];
-      __syncthreads();
-    }
-    // Rp <- Rp^{-1}, column by column: for i < j,
-    // inv[i][j] = -(sum_{k=i}^{j-1} inv[i][k] R[k][j]) / R[j][j]
-    for (int j = 0; j < c; ++j) {
-      const double rjj = Rp[(long)j * c + j];
-      for (int i = threadIdx.x; i <= j; i += NT) {
-        if (i == j) {
-          tmp[i] = 1.0 / rjj;
-        } else {
-          double t = 0.0;
-          for (int k2 = i; k2 < j; ++k2) t += Rp[(long)i * c + k2] * Rp[(long)k2 * c + j];
-          tmp[i] = -t / rjj;
-        }
-      }
-      __syncthreads();
-      for (int i = threadIdx.x; i <= j; i += NT) Rp[(long)i * c + j] = tmp[i];
-      __syncthreads();
-    }
+    // Cholesky, R accumulation and the inverse factor (identical in every member)
+    if (!cholqr_small_step(Rp, Rt, tmp, c)) return 1;
     // own rows: W <- W R^{-1}
     if (rows > 0)
       wg_gemm<double, double>(rows, c, c, crm<double>(Win + (long)m0 * c, c), crm<double>(Rp, c),

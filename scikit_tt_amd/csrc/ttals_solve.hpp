@@ -170,16 +170,32 @@ __device__ double wg_orth(const double* V, long ldv, int cnt, double* w, int N, 
 }
 
 // ---------------------------------------------------------------------------
+// What a Krylov solve reports beyond its status: the norm of the true
+// residual f - M x of the x it returns (every exit of CG, and of GMRES but
+// one, has just computed it; -1 where it has not) and the matvecs it made.
+struct KrylovOut {
+  double tres = -1.0;
+  int nmv = 0;
+};
+
 // Conjugate gradients on W.x (initial guess in W.x); vectors x, r, p, q, f
 // on-chip when OC.  Returns 0 on convergence, 1 if p^T M p <= 0 was met (not
 // SPD: caller switches to GMRES), 2 if maxit was reached.
 template <class ACC, bool OC>
 __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int maxit, ACC* lds, SolveLDS& S,
-                     int* iters) {
+                     int* iters, KrylovOut* out = nullptr) {
   using M = Mem<OC>;
   const int N = op.size();
   double *x = W.x, *r = W.r, *p = W.p, *q = W.q;
   const double* f = W.f;
+  KrylovOut o;
+  auto done = [&](int ret, double tres) {
+    if (out) {
+      out->tres = tres;
+      out->nmv += o.nmv;
+    }
+    return ret;
+  };
   double fn = 0.0, d1 = 0.0, d2 = 0.0;
   for (int e = threadIdx.x; e < N; e += NT) {
     const double fe = M::ld(f, e);
@@ -190,11 +206,13 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
   if (fn == 0.0) {
     for (int e = threadIdx.x; e < N; e += NT) M::st(x, e, 0.0);
     __syncthreads();
-    return 0;
+    return done(0, 0.0);
   }
   if (threadIdx.x == 0) S.vfl += 2.0 * N;
+  double tr = -1.0;
   for (int restart = 0; restart < 3; ++restart) {
     apply_op<double, ACC>(op, x, q, lds);
+    ++o.nmv;
     if (threadIdx.x == 0) S.vfl += 6.0 * N;  // this residual and the true-residual check below
     double rr = 0.0;
     for (int e = threadIdx.x; e < N; e += NT) {
@@ -204,17 +222,18 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
       rr += re * re;
     }
     wg_sum3(rr, d1, d2, S.red);
-    if (sqrt(rr) <= tol * fn) return 0;
+    if (sqrt(rr) <= tol * fn) return done(0, sqrt(rr));
     while (*iters < maxit) {
       long long t0 = ph_now();
       apply_op<double, ACC>(op, p, q, lds);
+      ++o.nmv;
       ph_add(S, PH_MATVEC, t0);
       t0 = ph_now();
       double pq = 0.0;
 #pragma unroll 8
       for (int e = threadIdx.x; e < N; e += NT) pq += M::ld(p, e) * M::ld(q, e);
       wg_sum3(pq, d1, d2, S.red);
-      if (!(pq > 0.0)) return 1;
+      if (!(pq > 0.0)) return done(1, -1.0);
       const double al = rr / pq;
       double rn = 0.0;
 #pragma unroll 4
@@ -237,43 +256,54 @@ __device__ int wg_cg(const LocalOp<double>& op, const Work& W, double tol, int m
     }
     // true residual check (the recurrence drifts)
     apply_op<double, ACC>(op, x, q, lds);
-    double tr = 0.0;
+    ++o.nmv;
+    tr = 0.0;
     for (int e = threadIdx.x; e < N; e += NT) {
       const double re = M::ld(f, e) - M::ld(q, e);
       M::st(r, e, re);
       tr += re * re;
     }
     wg_sum3(tr, d1, d2, S.red);
-    if (sqrt(tr) <= 10.0 * tol * fn) return 0;
-    if (*iters >= maxit) return 2;
+    tr = sqrt(tr);
+    if (tr <= 10.0 * tol * fn) return done(0, tr);
+    if (*iters >= maxit) return done(2, tr);
   }
-  return 2;
+  return done(2, tr);
 }
 
 // ---------------------------------------------------------------------------
 // Restarted GMRES(m) with DGKS-reorthogonalised Arnoldi; solution in W.x.
 template <class ACC>
 __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, int maxit, int m, ACC* lds,
-                        SolveLDS& S, int* iters) {
+                        SolveLDS& S, int* iters, KrylovOut* out = nullptr) {
   const int N = op.size();
   double *V = W.V, *x = W.x, *cur = W.cur, *w = W.w;
   const double* f = W.f;
+  KrylovOut o;
+  auto done = [&](int ret, double tres) {
+    if (out) {
+      out->tres = tres;
+      out->nmv += o.nmv;
+    }
+    return ret;
+  };
   if (m > KMAX) m = KMAX;
   if (m > N) m = N;
   const double fn = wg_nrm2(f, N, S.red);
   if (fn == 0.0) {
     wg_fill(x, 0.0, N);
     __syncthreads();
-    return 0;
+    return done(0, 0.0);
   }
   for (;;) {
     apply_op<double, ACC>(op, x, w, lds);
+    ++o.nmv;
     for (int e = threadIdx.x; e < N; e += NT) w[e] = f[e] - w[e];
     __syncthreads();
     const double beta = wg_nrm2(w, N, S.red);
     if (threadIdx.x == 0) S.vfl += 4.0 * N;
-    if (beta <= tol * fn) return 0;
-    if (*iters >= maxit) return 2;
+    if (beta <= tol * fn) return done(0, beta);
+    if (*iters >= maxit) return done(2, beta);
     const double ib = 1.0 / beta;
     for (int e = threadIdx.x; e < N; e += NT) {
       const double t = w[e] * ib;
@@ -289,6 +319,7 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     double res = beta;
     for (int j = 0; j < m; ++j) {
       apply_op<double, ACC>(op, cur, w, lds);
+      ++o.nmv;
       double w0;
       double hn = wg_orth(V, N, j + 1, w, N, S.h, S.H + j * (KMAX + 1), S.part, S.red, &w0, &S.vfl);
       if (!(hn > 1e-14 * w0)) hn = 0.0;  // (lucky) breakdown: Krylov space invariant
@@ -343,14 +374,17 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     }
     if (threadIdx.x == 0) S.vfl += 2.0 * N * jend + N;
     __syncthreads();
+    double tres = -1.0;  // x changed: its true residual is unknown until checked
     if (res <= tol * fn) {
       // confirm with the true residual once
       apply_op<double, ACC>(op, x, w, lds);
+      ++o.nmv;
       for (int e = threadIdx.x; e < N; e += NT) w[e] = f[e] - w[e];
       __syncthreads();
-      if (wg_nrm2(w, N, S.red) <= 10.0 * tol * fn) return 0;
+      tres = wg_nrm2(w, N, S.red);
+      if (tres <= 10.0 * tol * fn) return done(0, tres);
     }
-    if (*iters >= maxit) return 2;
+    if (*iters >= maxit) return done(2, tres);
   }
 }
 

@@ -219,9 +219,8 @@ def test_local_eig_nearest_interior_small(env):
 @pytest.mark.parametrize('m,c', [(12, 5), (5, 12), (200, 48), (7, 7), (1, 9), (9, 1), (32, 16), (64, 64),
                                  (64, 40), (30, 64), (65, 16), (200, 80), (130, 130), (100, 300), (520, 70)])
 def test_qr(env, m, c):
-    """Small panels (m, c <= 64) take the DPP-row QR; larger ones the
-    Householder sweep, whose trailing updates run column-blocked when 48 or
-    more columns remain (the last four shapes) and row-parallel below."""
+    """Small panels (m, c <= 64) take the DPP-row QR; larger tall ones
+    (m >= c) the blocked Householder QR; wide ones the column sweep."""
     lib, ctx, _ = env
     Y = np.random.default_rng(m * c).standard_normal((m, c))
     k = min(m, c)
@@ -235,6 +234,28 @@ def test_qr(env, m, c):
     # same factorisation as LAPACK up to the signs of R's rows
     Qr, Rl = sla.qr(Y, mode='economic')
     np.testing.assert_allclose(np.abs(Rr), np.abs(Rl), rtol=1e-10, atol=1e-12 * np.abs(Y).max())
+
+
+@pytest.mark.parametrize('m,c,rank,cond', [(300, 40, 20, 1.0), (520, 48, 48, 1e12), (768, 48, 47, 1.0)])
+def test_qr_ill_conditioned_and_rank_deficient(env, m, c, rank, cond):
+    """Tall panels of numerical rank below c or of condition number 1e12
+    (blocked Householder): Q R = Y and Q orthonormal to working accuracy; for
+    the full-rank panel R matches LAPACK's up to row signs."""
+    lib, ctx, _ = env
+    rng = np.random.default_rng(m + c + rank)
+    U, _ = np.linalg.qr(rng.standard_normal((m, rank)))
+    V, _ = np.linalg.qr(rng.standard_normal((c, rank)))
+    Y = (U * np.geomspace(1.0, 1.0 / cond, rank)) @ V.T
+    q, r = empty(m, c), empty(c, c)
+    dY = dev(Y)
+    assert lib.ttals_qr(ctx, None, m, c, ptr(dY), ptr(q), ptr(r)) == 0
+    Q, Rr = q.cpu().numpy(), r.cpu().numpy()
+    np.testing.assert_allclose(Q @ Rr, Y, atol=1e-13 * np.abs(Y).max() * m)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(c), atol=1e-12 * m)
+    assert np.allclose(np.tril(Rr, -1), 0.0)
+    if rank == c:
+        _, Rl = sla.qr(Y, mode='economic')
+        np.testing.assert_allclose(np.abs(Rr), np.abs(Rl), atol=1e-10 * np.abs(Rl).max())
 
 
 # --------------------------------------------------------------------------
@@ -435,12 +456,11 @@ def test_sle_errors(env):
     zero = [np.zeros_like(c) for c in A]
     with pytest.raises(np.linalg.LinAlgError):
         sle.als(_tt(zero), _tt(x0), _tt(b))
-    with pytest.raises(NotImplementedError):
-        evp.als(_tt(A), _tt(x0), number_ev=2)
-    with pytest.raises(NotImplementedError):
-        evp.als(_tt(A), _tt(x0), operator_gevp=_tt(A))
+    # (number_ev > 1 and operator_gevp run: tests/test_gpu_evp_options.py)
     with pytest.raises(NotImplementedError):
         evp.als(_tt(A), _tt(x0), solver='eigs')
+    with pytest.raises(NotImplementedError):
+        evp.als(_tt(A), _tt(x0), real=False)
 
 
 def _gapped_tt(rng, dims, rank, decay):
