@@ -1574,50 +1574,69 @@ __device__ void hh_block_t(const double* tau, const double* G, int kb, double* T
 }
 
 // X (rows x nc, global, ld ldx) <- (I - V T' V^T) X with T' = T^T (trans:
-// H^T X, the factorisation's trailing update) or T (forming Q).  Lanes over
-// 64-column chunks, waves over rows (HH_U rows loaded before any is used);
-// the waves' partial Y meet in Yb (NB x 64 doubles) in a fixed order.
-// rows in flight per wave (more push the engine kernels past 256 registers)
-constexpr int HH_U = 2;
+// H^T X, the factorisation's trailing update) or T (forming Q).  A wave's
+// lanes cover ncp columns (nc rounded up to 8, 16, 32 or 64; wider X in
+// chunks of 64) and 64 / ncp row groups; each lane streams its group's rows
+// with HH_U loads issued before the first is used (the passes are bound by
+// memory latency, not arithmetic).  Partial Y meet within the wave by lane
+// exchanges and across the waves in Yb (NB x 64 doubles), in a fixed order.
+// HH_U: 2 where the kernel must stay within 256 registers for two workgroups
+// per CU (k_als), 8 in a translation unit that defines TTALS_HH_ROWS for a
+// one-workgroup-per-CU kernel.
+#ifndef TTALS_HH_ROWS
+#define TTALS_HH_ROWS 2
+#endif
+constexpr int HH_U = TTALS_HH_ROWS;
 template <int NB>
 __device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const double* Pn, int kb, const double* T,
                                bool trans, double* Yb) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int c0 = 0; c0 < nc; c0 += 64) {
-    const int col = c0 + lane;
+  const int ncp = nc > 32 ? 64 : (nc > 16 ? 32 : (nc > 8 ? 16 : 8));
+  const int rg = 64 / ncp, cl = lane & (ncp - 1), sub = lane / ncp;
+  const int stride = NWAVE * rg;  // rows between a lane's consecutive rows
+  const int r0 = w * rg + sub;
+  for (int c0 = 0; c0 < nc; c0 += ncp) {
+    const int col = c0 + cl;
     const bool on = col < nc;
-    const int cc = on ? col : 0;
+    const int cc = on ? col : c0;
     double y[NB];
 #pragma unroll
     for (int p = 0; p < NB; ++p) y[p] = 0.0;
-    int i = w;
-    for (; i + (HH_U - 1) * NWAVE < rows; i += HH_U * NWAVE) {
+    int i = r0;
+    for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
       double x[HH_U];
 #pragma unroll
-      for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * NWAVE) * ldx + cc];
+      for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + cc];
 #pragma unroll
-      for (int u = 0; u < HH_U; ++u)
+      for (int u = 0; u < HH_U; ++u) {
+        __builtin_amdgcn_sched_barrier(0);  // one row's reflector entries live at a time
 #pragma unroll
         for (int p = 0; p < NB; ++p)
-          if (p < kb) y[p] += hh_v<NB>(Pn, i + u * NWAVE, p) * x[u];
+          if (p < kb) y[p] += hh_v<NB>(Pn, i + u * stride, p) * x[u];
+      }
     }
-    for (; i < rows; i += NWAVE) {
+    for (; i < rows; i += stride) {
       const double x = X[(long)i * ldx + cc];
 #pragma unroll
       for (int p = 0; p < NB; ++p)
         if (p < kb) y[p] += hh_v<NB>(Pn, i, p) * x;
     }
-    for (int u = 0; u < NWAVE; ++u) {
-      if (w == u) {
+    // row groups of the wave: lanes cl, cl + ncp, ... (fixed butterfly order)
+    for (int o = ncp; o < 64; o <<= 1) {
 #pragma unroll
-        for (int p = 0; p < NB; ++p) Yb[p * 64 + lane] = (u == 0 ? 0.0 : Yb[p * 64 + lane]) + y[p];
+      for (int p = 0; p < NB; ++p) y[p] += __shfl_xor(y[p], o, 64);
+    }
+    for (int u = 0; u < NWAVE; ++u) {
+      if (w == u && sub == 0) {
+#pragma unroll
+        for (int p = 0; p < NB; ++p) Yb[p * 64 + cl] = (u == 0 ? 0.0 : Yb[p * 64 + cl]) + y[p];
       }
       __syncthreads();
     }
     // Z = T' Y in place, one wave (a lane per column; Y's column in registers)
-    if (w == 0) {
+    if (w == 0 && sub == 0) {
 #pragma unroll
-      for (int p = 0; p < NB; ++p) y[p] = Yb[p * 64 + lane];
+      for (int p = 0; p < NB; ++p) y[p] = Yb[p * 64 + cl];
 #pragma unroll 1
       for (int p = 0; p < NB; ++p) {
         double sm = 0.0;
@@ -1626,29 +1645,30 @@ __device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const doubl
           const double tv = trans ? (l <= p ? T[l * NB + p] : 0.0) : (l >= p ? T[p * NB + l] : 0.0);
           sm += tv * y[l];
         }
-        Yb[p * 64 + lane] = sm;
+        Yb[p * 64 + cl] = sm;
       }
     }
     __syncthreads();
     double z[NB];
 #pragma unroll
-    for (int p = 0; p < NB; ++p) z[p] = Yb[p * 64 + lane];
+    for (int p = 0; p < NB; ++p) z[p] = Yb[p * 64 + cl];
     if (on) {
-      i = w;
-      for (; i + (HH_U - 1) * NWAVE < rows; i += HH_U * NWAVE) {
+      i = r0;
+      for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
         double x[HH_U];
 #pragma unroll
-        for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * NWAVE) * ldx + col];
+        for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + col];
 #pragma unroll
         for (int u = 0; u < HH_U; ++u) {
+          __builtin_amdgcn_sched_barrier(0);
           double sm = 0.0;
 #pragma unroll
           for (int p = 0; p < NB; ++p)
-            if (p < kb) sm += hh_v<NB>(Pn, i + u * NWAVE, p) * z[p];
-          X[(long)(i + u * NWAVE) * ldx + col] = x[u] - sm;
+            if (p < kb) sm += hh_v<NB>(Pn, i + u * stride, p) * z[p];
+          X[(long)(i + u * stride) * ldx + col] = x[u] - sm;
         }
       }
-      for (; i < rows; i += NWAVE) {
+      for (; i < rows; i += stride) {
         double sm = 0.0;
 #pragma unroll
         for (int p = 0; p < NB; ++p)
@@ -1660,9 +1680,24 @@ __device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const doubl
   }
 }
 
+// Pn (rows x kb, ld NB + 1) <- the block W[jb.., jb..jb+kb) (row stride c):
+// a thread per row, the row's kb loads issued together.
 template <int NB>
-__device__ __attribute__((noinline)) void hhqr_blocked_run(double* W, int m, int c, double* Qm, double* Rm,
-                                                           double* tws, double* lds, double* red) {
+__device__ __forceinline__ void hh_load_block(const double* W, int c, int jb, int rows, int kb, double* Pn) {
+  for (int i = threadIdx.x; i < rows; i += NT) {
+    const double* src = W + (long)(jb + i) * c + jb;
+    double v[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) v[q] = q < kb ? src[q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < kb) Pn[i * (NB + 1) + q] = v[q];
+  }
+}
+
+template <int NB>
+__device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double* Qm, double* Rm, double* tws,
+                                                 double* lds, double* red) {
   constexpr int ld = NB + 1;
   double* Pn = lds;
   double* dred = Pn + (long)m * ld;
@@ -1672,16 +1707,15 @@ __device__ __attribute__((noinline)) void hhqr_blocked_run(double* W, int m, int
   const int k = c;  // m >= c
   for (int jb = 0; jb < k; jb += NB) {
     const int kb = (k - jb) < NB ? (k - jb) : NB, rows = m - jb;
-    for (int e = threadIdx.x; e < rows * kb; e += NT) {
-      const int i = e / kb, q = e - i * kb;
-      Pn[i * ld + q] = W[(long)(jb + i) * c + jb + q];
-    }
+    hh_load_block<NB>(W, c, jb, rows, kb, Pn);
     __syncthreads();
     double* G = tws + (long)jb * NB;  // the block's Gram rows, then its T
     hh_block_factor<NB>(Pn, rows, kb, tau, G, dred, red);
-    for (int e = threadIdx.x; e < rows * kb; e += NT) {
-      const int i = e / kb, q = e - i * kb;
-      W[(long)(jb + i) * c + jb + q] = Pn[i * ld + q];
+    for (int i = threadIdx.x; i < rows; i += NT) {
+      double* dst = W + (long)(jb + i) * c + jb;
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        if (q < kb) dst[q] = Pn[i * ld + q];
     }
     hh_block_t<NB>(tau, G, kb, T);
     for (int e = threadIdx.x; e < NB * NB; e += NT) G[e] = T[e];
@@ -1701,10 +1735,7 @@ __device__ __attribute__((noinline)) void hhqr_blocked_run(double* W, int m, int
   const int nblk = (k + NB - 1) / NB;
   for (int b = nblk - 1; b >= 0; --b) {
     const int jb = b * NB, kb = (k - jb) < NB ? (k - jb) : NB, rows = m - jb;
-    for (int e = threadIdx.x; e < rows * kb; e += NT) {
-      const int i = e / kb, q = e - i * kb;
-      Pn[i * ld + q] = W[(long)(jb + i) * c + jb + q];
-    }
+    hh_load_block<NB>(W, c, jb, rows, kb, Pn);
     for (int e = threadIdx.x; e < NB * NB; e += NT) T[e] = tws[(long)jb * NB + e];
     __syncthreads();
     hh_block_apply<NB>(Qm + (long)jb * k + jb, k, rows, k - jb, Pn, kb, T, false, Yb);

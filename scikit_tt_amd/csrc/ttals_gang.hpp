@@ -302,6 +302,9 @@ struct GangCtx {
   double* lds;   // GEMM staging tiles first, lds_cap doubles in all
   long lds_cap;
   SolveLDS& S;
+  // set once a Cholesky QR of this system failed (a numerically
+  // rank-deficient panel): its later panels go to Householder directly
+  bool chol_off = false;
   __device__ bool sync() {
     const long long t0 = ph_now();
     const bool r = gang_sync(D.sync, G);
@@ -309,6 +312,24 @@ struct GangCtx {
     return r;
   }
 };
+
+// QR buffers of member 0's (whole-panel) Householder QR: the blocked form
+// when its block fits the member's LDS and its T storage the Gram slots
+// (free outside the Cholesky QR), else the column sweep.
+__device__ __forceinline__ QRBuf gang_qrbuf(const GangCtx& C, int c) {
+  const GangDesc& D = C.D;
+  QRBuf B{D.qw, D.qq, D.rf, D.tau};
+  if (((long)c + 16) * 16 <= (long)(C.G + 1) * D.rmax2) {
+    B.lds = C.lds;
+    B.lds_cap = C.lds_cap;
+    B.tws = D.gram;
+  }
+  return B;
+}
+__device__ __forceinline__ bool gang_blocked_qr_fits(const GangCtx& C, int m, int c) {
+  const long lim = C.lds_cap < HHQR_LDS_CAP ? C.lds_cap : HHQR_LDS_CAP;
+  return m >= c && HHBlockLDS<4>::need(m) <= lim && ((long)c + 16) * 16 <= (long)(C.G + 1) * C.D.rmax2;
+}
 
 // Local operator of position i: FP64 operands in global memory.
 struct GangLocal {
@@ -891,8 +912,10 @@ __device__ int gang_hhqr(GangCtx& C, const double* g, int m, int c, bool transpo
 // the split QR of core i's transposed unfolding (n b x a), Q^T into core i,
 // core i-1 <- core i-1 R^T (split over its rows, via g); a panel with fewer
 // rows than columns, or a failed pivot, runs Householder QR on member 0
-// (which may lower the rank, as the reference's QR does).  Returns false
-// once the gang has aborted.
+// (blocked where the panel fits; which may lower the rank, as the
+// reference's QR does) or, for a rank-deficient panel too tall for that, the
+// split column sweep.  After one failed pivot the system skips the Cholesky
+// QR.  Returns false once the gang has aborted.
 __device__ bool gang_ortho_right(GangCtx& C, double* fl) {
   const GangDesc& D = C.D;
   SolveLDS& S = C.S;
@@ -900,7 +923,7 @@ __device__ bool gang_ortho_right(GangCtx& C, double* fl) {
     const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
     const int m = n * b, c = a;
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
-    if (m >= c && (long)c * c <= D.rmax2) {
+    if (m >= c && (long)c * c <= D.rmax2 && !C.chol_off) {
       const double *Q, *R;
       const int q = gang_cholqr(C, D.xc[i], m, c, true, fl, &Q, &R);
       if (q < 0) return false;
@@ -923,35 +946,42 @@ __device__ bool gang_ortho_right(GangCtx& C, double* fl) {
         continue;
       }
       if (!C.sync()) return false;  // a pivot failed on every member alike
-      if (q == 1 && c <= 64) {
-        const double *Q2, *R2;
-        if (gang_hhqr(C, D.xc[i], m, c, true, fl, &Q2, &R2) < 0) return false;
-        int m0, m1;
-        gang_span(m, C.G, C.g, m0, m1);
-        for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
-          const long row = e / c, col = e - row * c;
-          D.xc[i][col * m + row] = Q2[e];
-        }
-        int r0, r1;
-        gang_span(cn, C.G, C.g, r0, r1);
-        if (r1 > r0)
-          wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(R2, c),
-                                  mrm<double>(D.g + (long)r0 * c, c), C.lds);
-        __syncthreads();
-        for (long e = (long)r0 * c + threadIdx.x; e < (long)r1 * c; e += NT) D.xc[i - 1][e] = D.g[e];
-        *fl += 2.0 * cn * c * a;
-        if (!C.sync()) return false;
-        continue;
+      if (q == 1) C.chol_off = true;
+    }
+    if (C.chol_off && m >= c && c <= 64 && (long)c * c <= D.rmax2 && !gang_blocked_qr_fits(C, m, c)) {
+      // a numerically rank-deficient panel too tall for member 0's blocked
+      // QR: the split column sweep
+      const double *Q2, *R2;
+      if (gang_hhqr(C, D.xc[i], m, c, true, fl, &Q2, &R2) < 0) return false;
+      int m0, m1;
+      gang_span(m, C.G, C.g, m0, m1);
+      for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+        const long row = e / c, col = e - row * c;
+        D.xc[i][col * m + row] = Q2[e];
       }
+      int r0, r1;
+      gang_span(cn, C.G, C.g, r0, r1);
+      if (r1 > r0)
+        wg_gemm<double, double>(r1 - r0, c, a, crm<double>(D.xc[i - 1] + (long)r0 * a, a), ctr<double>(R2, c),
+                                mrm<double>(D.g + (long)r0 * c, c), C.lds);
+      __syncthreads();
+      for (long e = (long)r0 * c + threadIdx.x; e < (long)r1 * c; e += NT) D.xc[i - 1][e] = D.g[e];
+      *fl += 2.0 * cn * c * a;
+      if (!C.sync()) return false;
+      continue;
     }
     if (C.g == 0) {
-      const QRBuf B{D.qw, D.qq, D.rf, D.tau};
+      const QRBuf B = gang_qrbuf(C, a);
       const int k = qr_factor(D.xc[i], m, a, true, B, S);
       for (long e = threadIdx.x; e < (long)k * m; e += NT) {
         const int rr = (int)(e / m), col = (int)(e % m);
         D.xc[i][e] = B.Q[(long)col * k + rr];
       }
-      ds_gemm<double, double>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+      if (B.lds)
+        wg_gemm<double, double>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k),
+                                C.lds);
+      else
+        ds_gemm<double, double>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
       wg_copy(D.g, D.xc[i - 1], cn * k);
       if (threadIdx.x == 0) D.rx[i] = k;
       *fl += fl_qr(m, a) + 2.0 * cn * k * a;
@@ -1056,17 +1086,20 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
   st.maxit_one = its > st.maxit_one ? its : st.maxit_one;
   ++st.nsolve;
   // QR of the new core and the warm start of the next position: split
-  // Cholesky QR over the gang for tall panels, else (or when a pivot fails)
-  // Householder QR on member 0
+  // Cholesky QR over the gang for tall panels, else (or when a pivot fails,
+  // and for the rest of the system's panels after one has) Householder QR on
+  // member 0, blocked where the panel fits, or the split column sweep for a
+  // rank-deficient panel too tall for that
   int status = ST_OK;
   const long long tq = ph_now();
   if (i > 0 || forward) {
     const int m = forward ? a * n : n * b, c = forward ? b : a;
-    if (m >= c && (long)c * c <= D.rmax2) {
+    if (m >= c && (long)c * c <= D.rmax2 && !(C.chol_off && (c > 64 || gang_blocked_qr_fits(C, m, c)))) {
       double qfl = 0.0;
       const double *Q = nullptr, *Rt = nullptr;
       const long long t_cq = ph_now();
-      const int q = gang_cholqr(C, D.g, m, c, !forward, &qfl, &Q, &Rt);
+      // Cholesky QR unless an earlier panel of this system was rank deficient
+      const int q = C.chol_off ? 1 : gang_cholqr(C, D.g, m, c, !forward, &qfl, &Q, &Rt);
       ph_add(S, PH_ORTH, t_cq);  // profiling builds: the split QR itself
       const long long t_ws = ph_now();
       if (q < 0) return ST_ABORT;
@@ -1105,9 +1138,10 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
         if (!C.sync()) return ST_ABORT;
         return status;
       }
-      // a pivot failed (or non-finite): every member got here alike
+      // a pivot failed (or non-finite, or skipped): every member got here alike
       if (!C.sync()) return ST_ABORT;
-      if (q == 1 && c <= 64) {
+      if (q == 1) C.chol_off = true;
+      if (q == 1 && c <= 64 && !gang_blocked_qr_fits(C, m, c)) {
         const double* Q2 = nullptr;
         const double* R2 = nullptr;
         if (gang_hhqr(C, D.g, m, c, !forward, &qfl, &Q2, &R2) < 0) return ST_ABORT;
@@ -1143,7 +1177,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
     }
   }
   if (C.g == 0) {
-    const QRBuf B{D.qw, D.qq, D.rf, D.tau};
+    const QRBuf B = gang_qrbuf(C, forward ? b : a);
     const double nv = wg_nrm2(D.g, o.N, S.red);
     if (!isfinite(nv)) {
       status = ST_NONFINITE;
@@ -1152,7 +1186,11 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       wg_copy(B.Q, D.xc[i], a * n * k);
       const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
       st.flops += fl_qr(a * n, b) + 2.0 * k * n1 * c1 * b;
-      ds_gemm<double, double>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc));
+      if (B.lds)
+        wg_gemm<double, double>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc),
+                                C.lds);
+      else
+        ds_gemm<double, double>(k, nc, b, crm<double>(B.R, b), crm<double>(D.xc[i + 1], nc), mrm<double>(D.g, nc));
       if (threadIdx.x == 0) D.rx[i + 1] = k;
     } else if (i > 0) {
       const int k = qr_factor(D.g, n * b, a, true, B, S);
@@ -1162,7 +1200,12 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
         D.xc[i][e] = B.Q[(long)cc * k + rr];
       }
       const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
-      ds_gemm<double, double>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k));
+      if (B.lds)
+        wg_gemm<double, double>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k),
+                                C.lds);
+      else
+        ds_gemm<double, double>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a),
+                                mrm<double>(D.g, k));
       if (threadIdx.x == 0) D.rx[i] = k;
     } else {
       wg_copy(D.g, D.xc[0], o.N);

@@ -3,7 +3,10 @@
 // A translation unit of its own: the engine's out-of-line device functions
 // are compiled here for k_hybrid's occupancy (one workgroup per CU, set by
 // the gang path's registers), while k_als keeps its own copies, compiled for
-// two resident workgroups per CU.
+// two resident workgroups per CU.  With registers to spare, the blocked
+// QR's streaming passes keep eight rows per lane in flight (same arithmetic
+// order as k_als's two).
+#define TTALS_HH_ROWS 8
 #include "ttals_hybrid.hpp"
 
 namespace ttals {
