@@ -200,7 +200,7 @@ __global__ __launch_bounds__(NT) void k_piece(PieceArgs A) {
         wg_qr_small(W, m, c, A.t2, A.out, A.v);
         break;
       }
-      if (wg_hhqr_blocked(W, m, c, A.out, A.v, A.t2 + (m > c ? m : c), dyn_lds, A.arena, S.red)) break;
+      if (wg_hhqr_blocked(W, m, c, A.out, A.v, A.t2 + (m > c ? m : c), A.t2, dyn_lds, A.arena, S.red)) break;
       wg_householder(W, m, c, A.t2, S.red);
       wg_form_q(W, m, c, A.t2, A.out);
       const int k = m < c ? m : c;

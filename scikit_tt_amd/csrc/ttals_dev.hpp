@@ -208,6 +208,52 @@ template <class T, class U>
 __device__ __forceinline__ void wg_copy(const T* x, U* y, int n) {
   for (int i = threadIdx.x; i < n; i += NT) y[i] = (U)x[i];
 }
+
+// y <- x (n elements, global memory); four loads in flight per thread before
+// their stores (x and y may alias as far as the compiler knows, which would
+// otherwise serialise every element on its load latency).
+template <class T, class U>
+__device__ __forceinline__ void wg_copy4(const T* x, U* y, int n) {
+  int i = threadIdx.x;
+  for (; i + 3 * NT < n; i += 4 * NT) {
+    const T a0 = x[i], a1 = x[i + NT], a2 = x[i + 2 * NT], a3 = x[i + 3 * NT];
+    y[i] = (U)a0;
+    y[i + NT] = (U)a1;
+    y[i + 2 * NT] = (U)a2;
+    y[i + 3 * NT] = (U)a3;
+  }
+  for (; i < n; i += NT) y[i] = (U)x[i];
+}
+
+// dst (cols x rows, row-major) <- the transpose of src (rows x cols): read
+// in src's order (coalesced), four loads in flight per thread, stores
+// scattered (they do not stall).
+__device__ __forceinline__ void wg_transpose(const double* src, int rows, int cols, double* dst) {
+  const long n = (long)rows * cols;
+  if (n < (1L << 22)) {
+    const float ic = 1.0f / (float)cols;
+    int e = threadIdx.x;
+    for (; e + 3 * NT < n; e += 4 * NT) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = src[e + u * NT];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int f = e + u * NT, i = (int)(((float)f + 0.5f) * ic), j = f - i * cols;
+        dst[(long)j * rows + i] = v[u];
+      }
+    }
+    for (; e < n; e += NT) {
+      const int i = (int)(((float)e + 0.5f) * ic), j = e - i * cols;
+      dst[(long)j * rows + i] = src[e];
+    }
+  } else {
+    for (long e = threadIdx.x; e < n; e += NT) {
+      const long i = e / cols, j = e - i * cols;
+      dst[j * rows + i] = src[e];
+    }
+  }
+}
 template <class T>
 __device__ __forceinline__ void wg_fill(T* y, T v, int n) {
   for (int i = threadIdx.x; i < n; i += NT) y[i] = v;
@@ -1697,7 +1743,7 @@ __device__ __forceinline__ void hh_load_block(const double* W, int c, int jb, in
 
 template <int NB>
 __device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double* Qm, double* Rm, double* tws,
-                                                 double* lds, double* red) {
+                                                 double* tau_all, double* lds, long lim, double* red) {
   constexpr int ld = NB + 1;
   double* Pn = lds;
   double* dred = Pn + (long)m * ld;
@@ -1717,16 +1763,59 @@ __device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double
       for (int q = 0; q < NB; ++q)
         if (q < kb) dst[q] = Pn[i * ld + q];
     }
+    if (threadIdx.x < kb) tau_all[jb + threadIdx.x] = tau[threadIdx.x];
     hh_block_t<NB>(tau, G, kb, T);
     for (int e = threadIdx.x; e < NB * NB; e += NT) G[e] = T[e];
     if (jb + kb < c) hh_block_apply<NB>(W + (long)jb * c + jb + kb, c, rows, c - jb - kb, Pn, kb, T, true, Yb);
     __syncthreads();
   }
-  // R, then Q = H_0 ... H_{k-1} I (backwards over the blocks)
+  // R
   for (long e = threadIdx.x; e < (long)k * c; e += NT) {
     const int i = (int)(e / c), j = (int)(e - (long)i * c);
     Rm[e] = (j >= i) ? W[e] : 0.0;
   }
+  __syncthreads();
+  if (GEMM_LDS_DOUBLES + 2L * k * k <= lim) {
+    // Q = H_0 ... H_{k-1} [I; 0] = I - V T V1^T in one step, from the compact
+    // WY form of all k reflectors (V: m x k, unit lower trapezoidal, made
+    // explicit in W; V1 its top k x k; T: k x k upper, dlarft's recurrence
+    // over the Gram matrix V^T V): Q = V (-T V1^T) + [I; 0], both products
+    // on the matrix cores.
+    double* Gs = lds + GEMM_LDS_DOUBLES;  // V^T V, then T in its upper part
+    double* Ms = Gs + (long)k * k;
+    for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+      const int i = (int)(e / c), j = (int)(e - (long)i * c);
+      if (j >= i) W[e] = (j == i) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    wg_gemm_mf(k, k, m, ctr<double>(W, c), crm<double>(W, c), mrm<double>(Gs, k), lds);
+    // T column by column over G's upper part: T[j][i] = -tau_i sum_{l=j}^{i-1}
+    // T[j][l] G[l][i] (j < i), T[i][i] = tau_i
+    for (int i = 0; i < k; ++i) {
+      const int j = threadIdx.x;
+      double v = 0.0;
+      if (j < i)
+        for (int l = j; l < i; ++l) v += Gs[j * k + l] * Gs[l * k + i];
+      __syncthreads();
+      if (j < i) Gs[j * k + i] = -tau_all[i] * v;
+      if (j == i) Gs[i * k + i] = tau_all[i];
+      __syncthreads();
+    }
+    for (int e = threadIdx.x; e < k * k; e += NT) {
+      const int i = e / k, j = e - i * k;
+      if (i > j) Gs[e] = 0.0;
+    }
+    __syncthreads();
+    // Ms = T V1^T, then Q = V (-Ms) + [I; 0]
+    wg_gemm_mf(k, k, k, crm<double>(Gs, k), ctr<double>(W, c), mrm<double>(Ms, k), lds);
+    for (int e = threadIdx.x; e < k * k; e += NT) Ms[e] = -Ms[e];
+    __syncthreads();
+    wg_gemm_mf(m, k, k, crm<double>(W, c), crm<double>(Ms, k), mrm<double>(Qm, k), lds);
+    for (int j = threadIdx.x; j < k; j += NT) Qm[(long)j * k + j] += 1.0;
+    __syncthreads();
+    return;
+  }
+  // else Q = H_0 ... H_{k-1} I backwards over the blocks
   for (long e = threadIdx.x; e < (long)m * k; e += NT) {
     const int i = (int)(e / k), j = (int)(e - (long)i * k);
     Qm[e] = (i == j) ? 1.0 : 0.0;
@@ -1744,20 +1833,21 @@ __device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double
 }
 
 // Blocked Householder QR (see above) of W (m x c, m >= c, global memory):
-// Q (m x c, ld c) into Qm, R (c x c) into Rm, reflectors left in W; tws
-// (global) holds (c + 8) * 8 doubles.  The block (8 or 4 columns) is the
+// Q (m x c, ld c) into Qm, R (c x c) into Rm; W is overwritten; tws (global)
+// holds (c + 8) * 8 doubles, tau_all c.  The block (8 or 4 columns) is the
 // larger whose LDS need fits min(cap, 61 KB) -- a bound independent of the
-// engine's arena, so every engine factors a panel alike.  Returns false when
-// neither fits (the caller runs the unblocked sweep).
+// engine's arena, so every engine factors a panel alike (and forms Q in one
+// step when 2 c^2 doubles fit beside the GEMM tiles).  Returns false when
+// neither block fits (the caller runs the unblocked sweep).
 constexpr long HHQR_LDS_CAP = 61 * 1024 / 8;
-__device__ bool wg_hhqr_blocked(double* W, int m, int c, double* Qm, double* Rm, double* tws, double* lds, long cap,
-                                double* red) {
+__device__ bool wg_hhqr_blocked(double* W, int m, int c, double* Qm, double* Rm, double* tws, double* tau_all,
+                                double* lds, long cap, double* red) {
   if (m < c || m >= (1 << 22) / 17) return false;
   const long lim = cap < HHQR_LDS_CAP ? cap : HHQR_LDS_CAP;
   if (HHBlockLDS<8>::need(m) <= lim)
-    hhqr_blocked_run<8>(W, m, c, Qm, Rm, tws, lds, red);
+    hhqr_blocked_run<8>(W, m, c, Qm, Rm, tws, tau_all, lds, lim, red);
   else if (HHBlockLDS<4>::need(m) <= lim)
-    hhqr_blocked_run<4>(W, m, c, Qm, Rm, tws, lds, red);
+    hhqr_blocked_run<4>(W, m, c, Qm, Rm, tws, tau_all, lds, lim, red);
   else
     return false;
   return true;

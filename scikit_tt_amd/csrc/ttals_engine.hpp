@@ -125,21 +125,14 @@ __device__ __forceinline__ double* qr_stage(const QRBuf& B, int m, int c, long l
 
 // W <- the (m x c) view of v: v itself (row-major) or, transposed, v (c x m).
 __device__ __forceinline__ void qr_load_panel(double* W, const double* v, int m, int c, bool transposed) {
-  const long mc = (long)m * c;
   if (!transposed) {
-    for (long e = threadIdx.x; e < mc; e += NT) W[e] = v[e];
-  } else if (mc < (1L << 22)) {
-    // W[i][j] = v[j][i], v is (c x m)
-    const float ic = 1.0f / (float)c;
-    for (int e = threadIdx.x; e < (int)mc; e += NT) {
-      const int i = fquot(e, ic), j = e - i * c;
-      W[e] = v[(long)j * m + i];
+    if ((long)m * c < (1L << 31)) {
+      wg_copy4(v, W, m * c);
+    } else {
+      for (long e = threadIdx.x; e < (long)m * c; e += NT) W[e] = v[e];
     }
   } else {
-    for (long e = threadIdx.x; e < mc; e += NT) {
-      const int i = (int)(e / c), j = (int)(e % c);
-      W[e] = v[(long)j * m + i];
-    }
+    wg_transpose(v, c, m, W);  // W[i][j] = v[j][i]
   }
   __syncthreads();
 }
@@ -157,7 +150,7 @@ __device__ int qr_factor(const double* v, int m, int c, bool transposed, const Q
     wg_qr_small(W, m, c, B.tau, B.Q, B.R);
     return k;
   }
-  if (B.lds && wg_hhqr_blocked(W, m, c, B.Q, B.R, B.tws, B.lds, B.lds_cap, S.red)) return k;
+  if (B.lds && wg_hhqr_blocked(W, m, c, B.Q, B.R, B.tws, B.tau, B.lds, B.lds_cap, S.red)) return k;
   wg_householder(W, m, c, B.tau, S.red);
   wg_form_q(W, m, c, B.tau, B.Q);
   for (long e = threadIdx.x; e < (long)k * c; e += NT) {
@@ -489,7 +482,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     // QR of (a*n, b): core i = Q, warm start for i+1 = R * core_{i+1}
     const QRBuf B = qr_buffers(D, a * n, b, reinterpret_cast<double*>(lds), arena);
     const int k = qr_factor(D.g, a * n, b, false, B, S);
-    wg_copy(B.Q, D.xc[i], a * n * k);
+    wg_copy4(B.Q, D.xc[i], a * n * k);
     const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
     st.flops += fl_qr(a * n, b) + 2.0 * k * n1 * c1 * b;
     // warm start g = R core_{i+1}: on the matrix cores when R is on-chip and
@@ -515,19 +508,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     const int k = qr_factor(D.g, n * b, a, true, B, S);
     st.flops += fl_qr(n * b, a) + 2.0 * D.rx[i - 1] * D.n[i - 1] * k * a;
     // core i (k, n*b) = Q^T, Q is (n*b x k)
-    if ((long)k * n * b < (1L << 22)) {
-      const int nb = n * b;
-      const float inb = 1.0f / (float)nb;
-      for (int e = threadIdx.x; e < k * nb; e += NT) {
-        const int r = fquot(e, inb), c = e - r * nb;
-        D.xc[i][e] = B.Q[(long)c * k + r];
-      }
-    } else {
-      for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
-        const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
-        D.xc[i][e] = B.Q[(long)c * k + r];
-      }
-    }
+    wg_transpose(B.Q, n * b, k, D.xc[i]);
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1], cn = c0 * n0;
     // g[(c0 n0), k] = core_{i-1}[(c0 n0), a] * R[k, a]^T (matrix cores as above)
     double* xs = qr_stage(B, n * b, a, (long)cn * a + (long)cn * k + 768, reinterpret_cast<double*>(lds), arena);
@@ -560,10 +541,7 @@ __device__ void ortho_right_x(const SysDesc& D, ACC* lds, long arena, SolveLDS& 
     const int a = D.rx[i], n = D.n[i], b = D.rx[i + 1];
     const QRBuf B = qr_buffers(D, n * b, a, reinterpret_cast<double*>(lds), arena);
     const int k = qr_factor(D.xc[i], n * b, a, true, B, S);
-    for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
-      const int r = (int)(e / (n * b)), c = (int)(e % (n * b));
-      D.xc[i][e] = B.Q[(long)c * k + r];
-    }
+    wg_transpose(B.Q, n * b, k, D.xc[i]);
     const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
     if (B.lds)
       wg_gemm<double, ACC>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k), lds);

@@ -973,10 +973,7 @@ __device__ bool gang_ortho_right(GangCtx& C, double* fl) {
     if (C.g == 0) {
       const QRBuf B = gang_qrbuf(C, a);
       const int k = qr_factor(D.xc[i], m, a, true, B, S);
-      for (long e = threadIdx.x; e < (long)k * m; e += NT) {
-        const int rr = (int)(e / m), col = (int)(e % m);
-        D.xc[i][e] = B.Q[(long)col * k + rr];
-      }
+      wg_transpose(B.Q, m, k, D.xc[i]);
       if (B.lds)
         wg_gemm<double, double>(cn, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k),
                                 C.lds);
@@ -1183,7 +1180,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       status = ST_NONFINITE;
     } else if (forward) {
       const int k = qr_factor(D.g, a * n, b, false, B, S);
-      wg_copy(B.Q, D.xc[i], a * n * k);
+      wg_copy4(B.Q, D.xc[i], a * n * k);
       const int n1 = D.n[i + 1], c1 = D.rx[i + 2], nc = n1 * c1;
       st.flops += fl_qr(a * n, b) + 2.0 * k * n1 * c1 * b;
       if (B.lds)
@@ -1195,10 +1192,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
     } else if (i > 0) {
       const int k = qr_factor(D.g, n * b, a, true, B, S);
       st.flops += fl_qr(n * b, a) + 2.0 * D.rx[i - 1] * D.n[i - 1] * k * a;
-      for (long e = threadIdx.x; e < (long)k * n * b; e += NT) {
-        const int rr = (int)(e / (n * b)), cc = (int)(e % (n * b));
-        D.xc[i][e] = B.Q[(long)cc * k + rr];
-      }
+      wg_transpose(B.Q, n * b, k, D.xc[i]);
       const int c0 = D.rx[i - 1], n0 = D.n[i - 1];
       if (B.lds)
         wg_gemm<double, double>(c0 * n0, k, a, crm<double>(D.xc[i - 1], a), ctr<double>(B.R, a), mrm<double>(D.g, k),
