@@ -403,11 +403,20 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
 }
 
 template <class T, class ACC>
+__device__ void wg_gemm_valu(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds);
+
+// GEMM over global operands: the matrix cores for FP64, the vector units
+// (register tiles) otherwise.
+template <class T, class ACC>
 __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
-  if constexpr (std::is_same<T, double>::value && std::is_same<ACC, double>::value) {
+  if constexpr (std::is_same<T, double>::value && std::is_same<ACC, double>::value)
     wg_gemm_mf(M, N, K, A, B, C, lds);
-    return;
-  }
+  else
+    wg_gemm_valu<T, ACC>(M, N, K, A, B, C, lds);
+}
+
+template <class T, class ACC>
+__device__ void wg_gemm_valu(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
   ACC* As = lds;                      // [GBK][GBM+1]
   ACC* Bs = lds + GBK * (GBM + 1);    // [GBK][GBN+1]
   const int tid = threadIdx.x;
@@ -1524,25 +1533,44 @@ __device__ __forceinline__ double hh_v(const double* Pn, int i, int p) {
 
 // Factor the block Pn (rows x kb, ld NB + 1) in place (R on and above the
 // diagonal, reflectors below), tau[0..kb) and the upper part of G = V^T V
-// (G[l * NB + j], l < j).  dred: NWAVE * NB doubles; red: NWAVE doubles.
-// Thread t owns the rows t, t + NT, ... in every column step (rows below the
-// step's pivot only): a row is written and read by one thread, so the steps
-// need no barrier beyond their reductions'.
+// (G[l * NB + j], l < j).  dred: NWAVE * NB doubles.  One reduction per
+// column: over the rows below the pivot, with x the column, |x|^2, x^T X_q
+// (later columns) and x^T V_l (earlier reflectors) together; then with
+// v = (1, sc x) the update's w_q = X_jq + sc x^T X_q and the Gram entries
+// G[l][j] = V_jl + sc x^T V_l follow without a second pass.  Thread t owns
+// the rows t, t + NT, ... in every step, so a row is written and read by one
+// thread; the pivot row j, which every thread reads after the step's
+// barrier, collects its new values in prow (NB * NB + NB doubles of LDS,
+// with the step's w_q) until the block is done.  Two barriers per column.
 template <int NB>
-__device__ void hh_block_factor(double* Pn, int rows, int kb, double* tau, double* G, double* dred, double* red) {
+__device__ void hh_block_factor(double* Pn, int rows, int kb, double* tau, double* G, double* dred, double* prow) {
   constexpr int ld = NB + 1;
+  double* wqs = prow + NB * NB;  // the step's w_q
   static_assert(NB < NT, "a block's pivot rows lie in the first row of every thread's set");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int tid = threadIdx.x;
   for (int j = 0; j < kb; ++j) {
-    // first own row > j / >= j (j < NB < NT: only the first set can lie below)
-    const int i_gt = tid > j ? tid : tid + NT, i_ge = tid >= j ? tid : tid + NT;
-    double s2 = 0.0;
+    const int i_gt = tid > j ? tid : tid + NT;  // first own row below the pivot
+    double acc[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) acc[q] = 0.0;
     for (int i = i_gt; i < rows; i += NT) {
-      const double t = Pn[i * ld + j];
-      s2 += t * t;
+      const double x = Pn[i * ld + j];
+#pragma unroll
+      for (int q = 0; q < NB; ++q)
+        if (q < kb) acc[q] += x * (q == j ? x : Pn[i * ld + q]);
     }
-    s2 = wg_sum(s2, red);  // (barriers: the previous column's updates are visible)
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < kb) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) dred[w * NB + q] = acc[q];
+    }
+    __syncthreads();
+    double s2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < NWAVE; ++u) s2 += dred[u * NB + j];
     const double alpha = Pn[j * ld + j];
     double t = 0.0, beta = alpha, sc = 0.0;
     if (s2 != 0.0) {
@@ -1551,51 +1579,33 @@ __device__ void hh_block_factor(double* Pn, int rows, int kb, double* tau, doubl
       t = (beta - alpha) / beta;
       sc = 1.0 / (alpha - beta);
     }
-    // own rows: scale the reflector; partial dots of v with the later
-    // columns (q > j, for the update) and of the earlier reflectors with v
-    // (l < j, the Gram entries G[l][j])
-    double acc[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q) acc[q] = 0.0;
-    for (int i = i_ge; i < rows; i += NT) {
-      double v = 1.0;
-      if (i > j) {
-        v = (t != 0.0) ? Pn[i * ld + j] * sc : 0.0;
-        if (t != 0.0) Pn[i * ld + j] = v;
-      }
-#pragma unroll
-      for (int q = 0; q < NB; ++q)
-        if (q != j && q < kb) acc[q] += v * Pn[i * ld + q];
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (q != j && q < kb) acc[q] = wave_sum(acc[q]);
-    if (lane == 0) {
-#pragma unroll
-      for (int q = 0; q < NB; ++q) dred[w * NB + q] = acc[q];
-    }
-    __syncthreads();
-    double wq[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
+    // thread q < kb: w_q, the pivot row's new entry q and G[q][j] (q < j);
+    // the pivot row itself is left alone (every thread reads it above)
+    if (tid < kb) {
       double sm = 0.0;
 #pragma unroll
-      for (int u = 0; u < NWAVE; ++u) sm += dred[u * NB + q];
-      wq[q] = sm;
+      for (int u = 0; u < NWAVE; ++u) sm += dred[u * NB + tid];
+      const double xq = Pn[j * ld + tid], wq = xq + sc * sm;
+      wqs[tid] = wq;
+      prow[j * NB + tid] = tid < j ? xq : (tid == j ? beta : xq - t * wq);
+      if (tid < j) G[tid * NB + j] = wq;
     }
+    if (tid == 0) tau[j] = t;
+    __syncthreads();
     if (t != 0.0) {
-      for (int i = i_ge; i < rows; i += NT) {
-        const double v = (i == j) ? 1.0 : Pn[i * ld + j];
+      for (int i = i_gt; i < rows; i += NT) {
+        const double v = Pn[i * ld + j] * sc;
+        Pn[i * ld + j] = v;
 #pragma unroll
         for (int q = 0; q < NB; ++q)
-          if (q > j && q < kb) Pn[i * ld + q] -= t * v * wq[q];
+          if (q > j && q < kb) Pn[i * ld + q] -= t * v * wqs[q];
       }
     }
-    if (tid == j) Pn[j * ld + j] = beta;  // (the owner of row j < NT)
-    if (threadIdx.x == 0) {
-      tau[j] = t;
-      for (int l = 0; l < j; ++l) G[l * NB + j] = wq[l];
-    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kb * kb; e += NT) {
+    const int i = e / kb, q = e - i * kb;
+    Pn[i * ld + q] = prow[i * NB + q];
   }
   __syncthreads();
 }
@@ -1756,7 +1766,7 @@ __device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double
     hh_load_block<NB>(W, c, jb, rows, kb, Pn);
     __syncthreads();
     double* G = tws + (long)jb * NB;  // the block's Gram rows, then its T
-    hh_block_factor<NB>(Pn, rows, kb, tau, G, dred, red);
+    hh_block_factor<NB>(Pn, rows, kb, tau, G, dred, Yb);
     for (int i = threadIdx.x; i < rows; i += NT) {
       double* dst = W + (long)(jb + i) * c + jb;
 #pragma unroll
@@ -1788,31 +1798,38 @@ __device__ __forceinline__ void hhqr_blocked_run(double* W, int m, int c, double
       if (j >= i) W[e] = (j == i) ? 1.0 : 0.0;
     }
     __syncthreads();
-    wg_gemm_mf(k, k, m, ctr<double>(W, c), crm<double>(W, c), mrm<double>(Gs, k), lds);
-    // T column by column over G's upper part: T[j][i] = -tau_i sum_{l=j}^{i-1}
-    // T[j][l] G[l][i] (j < i), T[i][i] = tau_i
-    for (int i = 0; i < k; ++i) {
-      const int j = threadIdx.x;
-      double v = 0.0;
-      if (j < i)
-        for (int l = j; l < i; ++l) v += Gs[j * k + l] * Gs[l * k + i];
-      __syncthreads();
-      if (j < i) Gs[j * k + i] = -tau_all[i] * v;
-      if (j == i) Gs[i * k + i] = tau_all[i];
+    // three products through one GEMM call site (one inlined copy of its
+    // registers): G = V^T V; Ms = T V1^T; Q = V (-Ms)
+    for (int g = 0; g < 3; ++g) {
+      const int gm = g == 2 ? m : k, gk = g == 0 ? m : k;
+      const CView<double> A = g == 0 ? ctr<double>(W, c) : (g == 1 ? crm<double>(Gs, k) : crm<double>(W, c));
+      const CView<double> Bv = g == 0 ? crm<double>(W, c) : (g == 1 ? ctr<double>(W, c) : crm<double>(Ms, k));
+      const MView<double> Cv = g == 0 ? mrm<double>(Gs, k) : (g == 1 ? mrm<double>(Ms, k) : mrm<double>(Qm, k));
+      wg_gemm_mf(gm, k, gk, A, Bv, Cv, lds);
+      if (g == 0) {
+        // T column by column over G's upper part: T[j][i] = -tau_i
+        // sum_{l=j}^{i-1} T[j][l] G[l][i] (j < i), T[i][i] = tau_i
+        for (int i = 0; i < k; ++i) {
+          const int j = threadIdx.x;
+          double v = 0.0;
+          if (j < i)
+            for (int l = j; l < i; ++l) v += Gs[j * k + l] * Gs[l * k + i];
+          __syncthreads();
+          if (j < i) Gs[j * k + i] = -tau_all[i] * v;
+          if (j == i) Gs[i * k + i] = tau_all[i];
+          __syncthreads();
+        }
+        for (int e = threadIdx.x; e < k * k; e += NT) {
+          const int i = e / k, j = e - i * k;
+          if (i > j) Gs[e] = 0.0;
+        }
+      } else if (g == 1) {
+        for (int e = threadIdx.x; e < k * k; e += NT) Ms[e] = -Ms[e];
+      } else {
+        for (int j = threadIdx.x; j < k; j += NT) Qm[(long)j * k + j] += 1.0;
+      }
       __syncthreads();
     }
-    for (int e = threadIdx.x; e < k * k; e += NT) {
-      const int i = e / k, j = e - i * k;
-      if (i > j) Gs[e] = 0.0;
-    }
-    __syncthreads();
-    // Ms = T V1^T, then Q = V (-Ms) + [I; 0]
-    wg_gemm_mf(k, k, k, crm<double>(Gs, k), ctr<double>(W, c), mrm<double>(Ms, k), lds);
-    for (int e = threadIdx.x; e < k * k; e += NT) Ms[e] = -Ms[e];
-    __syncthreads();
-    wg_gemm_mf(m, k, k, crm<double>(W, c), crm<double>(Ms, k), mrm<double>(Qm, k), lds);
-    for (int j = threadIdx.x; j < k; j += NT) Qm[(long)j * k + j] += 1.0;
-    __syncthreads();
     return;
   }
   // else Q = H_0 ... H_{k-1} I backwards over the blocks
