@@ -1659,16 +1659,41 @@ __device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const doubl
 #pragma unroll
     for (int p = 0; p < NB; ++p) y[p] = 0.0;
     int i = r0;
-    for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
-      double x[HH_U];
+    if constexpr (HH_U >= 4) {
+      // registers to spare: the next group's loads are issued before this
+      // group is used, and the reflector reads are left to the scheduler
+      double xn[HH_U];
+      if (i + (HH_U - 1) * stride < rows) {
 #pragma unroll
-      for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + cc];
+        for (int u = 0; u < HH_U; ++u) xn[u] = X[(long)(i + u * stride) * ldx + cc];
+      }
+      for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
+        double x[HH_U];
 #pragma unroll
-      for (int u = 0; u < HH_U; ++u) {
-        __builtin_amdgcn_sched_barrier(0);  // one row's reflector entries live at a time
+        for (int u = 0; u < HH_U; ++u) x[u] = xn[u];
+        const int inext = i + HH_U * stride;
+        if (inext + (HH_U - 1) * stride < rows) {
 #pragma unroll
-        for (int p = 0; p < NB; ++p)
-          if (p < kb) y[p] += hh_v<NB>(Pn, i + u * stride, p) * x[u];
+          for (int u = 0; u < HH_U; ++u) xn[u] = X[(long)(inext + u * stride) * ldx + cc];
+        }
+#pragma unroll
+        for (int u = 0; u < HH_U; ++u)
+#pragma unroll
+          for (int p = 0; p < NB; ++p)
+            if (p < kb) y[p] += hh_v<NB>(Pn, i + u * stride, p) * x[u];
+      }
+    } else {
+      for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
+        double x[HH_U];
+#pragma unroll
+        for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + cc];
+#pragma unroll
+        for (int u = 0; u < HH_U; ++u) {
+          __builtin_amdgcn_sched_barrier(0);  // one row's reflector entries live at a time
+#pragma unroll
+          for (int p = 0; p < NB; ++p)
+            if (p < kb) y[p] += hh_v<NB>(Pn, i + u * stride, p) * x[u];
+        }
       }
     }
     for (; i < rows; i += stride) {
@@ -1710,18 +1735,45 @@ __device__ void hh_block_apply(double* X, int ldx, int rows, int nc, const doubl
     for (int p = 0; p < NB; ++p) z[p] = Yb[p * 64 + cl];
     if (on) {
       i = r0;
-      for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
-        double x[HH_U];
+      if constexpr (HH_U >= 4) {
+        double xn[HH_U];
+        if (i + (HH_U - 1) * stride < rows) {
 #pragma unroll
-        for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + col];
+          for (int u = 0; u < HH_U; ++u) xn[u] = X[(long)(i + u * stride) * ldx + col];
+        }
+        for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
+          double x[HH_U];
 #pragma unroll
-        for (int u = 0; u < HH_U; ++u) {
-          __builtin_amdgcn_sched_barrier(0);
-          double sm = 0.0;
+          for (int u = 0; u < HH_U; ++u) x[u] = xn[u];
+          // (rows of the next group are disjoint from this group's stores)
+          const int inext = i + HH_U * stride;
+          if (inext + (HH_U - 1) * stride < rows) {
 #pragma unroll
-          for (int p = 0; p < NB; ++p)
-            if (p < kb) sm += hh_v<NB>(Pn, i + u * stride, p) * z[p];
-          X[(long)(i + u * stride) * ldx + col] = x[u] - sm;
+            for (int u = 0; u < HH_U; ++u) xn[u] = X[(long)(inext + u * stride) * ldx + col];
+          }
+#pragma unroll
+          for (int u = 0; u < HH_U; ++u) {
+            double sm = 0.0;
+#pragma unroll
+            for (int p = 0; p < NB; ++p)
+              if (p < kb) sm += hh_v<NB>(Pn, i + u * stride, p) * z[p];
+            X[(long)(i + u * stride) * ldx + col] = x[u] - sm;
+          }
+        }
+      } else {
+        for (; i + (HH_U - 1) * stride < rows; i += HH_U * stride) {
+          double x[HH_U];
+#pragma unroll
+          for (int u = 0; u < HH_U; ++u) x[u] = X[(long)(i + u * stride) * ldx + col];
+#pragma unroll
+          for (int u = 0; u < HH_U; ++u) {
+            __builtin_amdgcn_sched_barrier(0);
+            double sm = 0.0;
+#pragma unroll
+            for (int p = 0; p < NB; ++p)
+              if (p < kb) sm += hh_v<NB>(Pn, i + u * stride, p) * z[p];
+            X[(long)(i + u * stride) * ldx + col] = x[u] - sm;
+          }
         }
       }
       for (; i < rows; i += stride) {
