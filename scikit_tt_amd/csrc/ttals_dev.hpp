@@ -286,6 +286,15 @@ __device__ __forceinline__ int gmap_at(const GMap& g, int i) {
 
 typedef double dx4_t __attribute__((ext_vector_type(4)));
 
+// K-steps of operands in flight during a K-step's MFMAs: 1 where the kernel
+// must stay within 256 registers for two workgroups per CU (k_als), 2 in a
+// translation unit that defines TTALS_GEMM_PREFETCH for a one-workgroup-per-CU
+// kernel (the same arithmetic either way).
+#ifndef TTALS_GEMM_PREFETCH
+#define TTALS_GEMM_PREFETCH 1
+#endif
+constexpr int GEMM_PREFETCH = TTALS_GEMM_PREFETCH;
+
 // FP64 GEMM over global (L2/HBM) operands on the matrix cores.  64x64 output
 // tiles, K-steps of 16 staged through LDS ([GBK][GBM+1] and [GBK][GBN+1]:
 // the odd stride keeps the transposing stores of k-fast operands off a single
@@ -315,11 +324,10 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
   const int b_dn = b_nfast ? 0 : 16, b_dk = b_nfast ? 4 : 0;
   const int tn = (N + GBN - 1) / GBN, ks = (K + GBK - 1) / GBK;
   const int steps = ((M + GBM - 1) / GBM) * tn * ks;
-  double ra[4], rb[4];
   // loader position (tile origin, K offset) and compute position, advanced without division
   int lm0 = 0, ln0 = 0, lk0 = 0, ltj = 0;
   int cm0 = 0, cn0 = 0, ckk = 0, ctj = 0;
-  auto load = [&]() {
+  auto load = [&](double (&ra)[4], double (&rb)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gm = lm0 + a_i0 + j * a_di, gk = lk0 + a_k0 + j * a_dk;
@@ -338,7 +346,7 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
       }
     }
   };
-  auto stage = [&]() {
+  auto stage = [&](const double (&ra)[4], const double (&rb)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       As[(a_k0 + j * a_dk) * (GBM + 1) + a_i0 + j * a_di] = ra[j];
@@ -350,13 +358,8 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int g = 0; g < 2; ++g) acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
-  if (steps > 0) {
-    load();
-    stage();
-  }
-  __syncthreads();
-  for (int s = 0; s < steps; ++s) {
-    if (s + 1 < steps) load();
+  // the MFMAs of the staged K-step, and the tile's store after its last one
+  auto compute = [&]() {
 #pragma unroll
     for (int q = 0; q < GBK / 4; ++q) {
       const int kr = (4 * q + lk);
@@ -394,10 +397,81 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
         cm0 += GBM;
       }
     }
+  };
+  if constexpr (GEMM_PREFETCH == 2) {
+    // two register sets: while step s computes, steps s+1 and s+2 are loading
+    double ra0[4], rb0[4], ra1[4], rb1[4];
+    if (steps > 0) {
+      load(ra0, rb0);
+      stage(ra0, rb0);
+    }
+    if (steps > 1) load(ra1, rb1);
     __syncthreads();
-    if (s + 1 < steps) {
-      stage();
+    for (int s = 0; s < steps; s += 2) {
+      if (s + 2 < steps) load(ra0, rb0);
+      compute();
       __syncthreads();
+      if (s + 1 >= steps) break;
+      stage(ra1, rb1);
+      __syncthreads();
+      if (s + 3 < steps) load(ra1, rb1);
+      compute();
+      __syncthreads();
+      if (s + 2 < steps) {
+        stage(ra0, rb0);
+        __syncthreads();
+      }
+    }
+  } else {
+    double ra[4], rb[4];
+    if (steps > 0) {
+      load(ra, rb);
+      stage(ra, rb);
+    }
+    __syncthreads();
+    for (int s = 0; s < steps; ++s) {
+      if (s + 1 < steps) load(ra, rb);
+#pragma unroll
+      for (int q = 0; q < GBK / 4; ++q) {
+        const int kr = (4 * q + lk);
+        const double a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
+        const double b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+      ckk += GBK;
+      if (ckk >= K) {
+        const GMap Cr = gmap(C.r), Cc = gmap(C.c);
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          const int gn = cn0 + wn + 16 * g + li;
+          const bool nv = gn < N;
+          const int cc = nv ? gmap_at(Cc, gn) : 0;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int gm = cm0 + wm + 16 * h + lk + 4 * e;
+              if (nv && gm < M) C.p[gmap_at(Cr, gm) + cc] = acc[h][g][e];
+            }
+            acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+          }
+        }
+        ckk = 0;
+        cn0 += GBN;
+        if (++ctj == tn) {
+          ctj = 0;
+          cn0 = 0;
+          cm0 += GBM;
+        }
+      }
+      __syncthreads();
+      if (s + 1 < steps) {
+        stage(ra, rb);
+        __syncthreads();
+      }
     }
   }
 }

@@ -4,9 +4,10 @@
 // are compiled here for k_hybrid's occupancy (one workgroup per CU, set by
 // the gang path's registers), while k_als keeps its own copies, compiled for
 // two resident workgroups per CU.  With registers to spare, the blocked
-// QR's streaming passes keep eight rows per lane in flight (same arithmetic
-// order as k_als's two).
+// QR's streaming passes keep eight rows per lane in flight and the global
+// GEMMs two K-steps (same arithmetic order as k_als's two rows and one step).
 #define TTALS_HH_ROWS 8
+#define TTALS_GEMM_PREFETCH 2
 #include "ttals_hybrid.hpp"
 
 namespace ttals {
