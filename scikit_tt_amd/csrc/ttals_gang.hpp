@@ -908,6 +908,119 @@ __device__ int gang_hhqr(GangCtx& C, const double* g, int m, int c, bool transpo
   return 0;
 }
 
+// Split blocked Householder QR of the panel (m x c, m >= c) over the gang:
+// the structure of wg_hhqr_blocked, with every member factoring each column
+// block in its own LDS (identical inputs and order: identical reflectors, no
+// exchange), the trailing update split over the members by columns (one gang
+// barrier per block), and Q formed in one step from the compact WY form of
+// all reflectors, each member computing its own rows.  Rows of the panel as
+// in gang_cholqr; *Q in D.qq (every row), *R in D.rf.  Returns 0, or -1 when
+// the gang aborted.  Callers check gang_split_qr_fits first.
+__device__ __forceinline__ bool gang_split_qr_fits(const GangCtx& C, int m, int c) {
+  const long lim = C.lds_cap < HHQR_LDS_CAP ? C.lds_cap : HHQR_LDS_CAP;
+  return m >= c && c <= NT && m < (1 << 22) / 17 && HHBlockLDS<4>::need(m) <= lim &&
+         GEMM_LDS_DOUBLES + 2L * c * c <= lim;
+}
+
+template <int NB>
+__device__ int gang_hhqr_run(GangCtx& C, int m, int c) {
+  const GangDesc& D = C.D;
+  constexpr int ld = NB + 1;
+  double* W = D.qw;
+  double* Pn = C.lds;
+  double* dred = Pn + (long)m * ld;
+  double* Yb = dred + NWAVE * NB;
+  double* T = Yb + NB * 64;
+  double* tau = T + NB * NB;
+  double* G = Yb + NB * NB + NB;  // the block's Gram rows (after the pivot rows and w_q)
+  const int k = c;
+  for (int jb = 0; jb < k; jb += NB) {
+    const int kb = (k - jb) < NB ? (k - jb) : NB, rows = m - jb;
+    hh_load_block<NB>(W, c, jb, rows, kb, Pn);
+    __syncthreads();
+    hh_block_factor<NB>(Pn, rows, kb, tau, G, dred, Yb);
+    if (C.g == 0) {
+      for (int i = threadIdx.x; i < rows; i += NT) {
+        double* dst = W + (long)(jb + i) * c + jb;
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          if (q < kb) dst[q] = Pn[i * ld + q];
+      }
+      if (threadIdx.x < kb) D.tau[jb + threadIdx.x] = tau[threadIdx.x];
+    }
+    hh_block_t<NB>(tau, G, kb, T);
+    const int c2 = c - jb - kb;
+    if (c2 > 0) {
+      int j0, j1;
+      gang_span(c2, C.G, C.g, j0, j1);
+      if (j1 > j0) hh_block_apply<NB>(W + (long)jb * c + jb + kb + j0, c, rows, j1 - j0, Pn, kb, T, true, Yb);
+    }
+    __syncthreads();
+    if (!C.sync()) return -1;
+  }
+  // member 0: R, and W made the explicit V (unit diagonal, zeros above)
+  if (C.g == 0) {
+    for (long e = threadIdx.x; e < (long)k * c; e += NT) {
+      const int i = (int)(e / c), j = (int)(e - (long)i * c);
+      D.rf[e] = (j >= i) ? W[e] : 0.0;
+      if (j >= i) W[e] = (j == i) ? 1.0 : 0.0;
+    }
+  }
+  if (!C.sync()) return -1;
+  // every member: G = V^T V, T (dlarft), Ms = -T V1^T; then its own rows of
+  // Q = V Ms + [I; 0]
+  double* Gs = C.lds + GEMM_LDS_DOUBLES;
+  double* Ms = Gs + (long)k * k;
+  wg_gemm_mf(k, k, m, ctr<double>(W, c), crm<double>(W, c), mrm<double>(Gs, k), C.lds);
+  for (int i = 0; i < k; ++i) {
+    const int j = threadIdx.x;
+    double v = 0.0;
+    if (j < i)
+      for (int l = j; l < i; ++l) v += Gs[j * k + l] * Gs[l * k + i];
+    __syncthreads();
+    if (j < i) Gs[j * k + i] = -D.tau[i] * v;
+    if (j == i) Gs[i * k + i] = D.tau[i];
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < k * k; e += NT) {
+    const int i = e / k, j = e - i * k;
+    if (i > j) Gs[e] = 0.0;
+  }
+  __syncthreads();
+  wg_gemm_mf(k, k, k, crm<double>(Gs, k), ctr<double>(W, c), mrm<double>(Ms, k), C.lds);
+  for (int e = threadIdx.x; e < k * k; e += NT) Ms[e] = -Ms[e];
+  __syncthreads();
+  int m0, m1;
+  gang_span(m, C.G, C.g, m0, m1);
+  if (m1 > m0)
+    wg_gemm_mf(m1 - m0, k, k, crm<double>(W + (long)m0 * c, c), crm<double>(Ms, k), mrm<double>(D.qq + (long)m0 * k, k),
+               C.lds);
+  for (int j = m0 + (int)threadIdx.x; j < (m1 < k ? m1 : k); j += NT) D.qq[(long)j * k + j] += 1.0;
+  __syncthreads();
+  return 0;
+}
+
+__device__ int gang_hhqr_blocked(GangCtx& C, const double* g, int m, int c, bool transposed, double* fl,
+                                 const double** Qout, const double** Rout) {
+  const GangDesc& D = C.D;
+  int m0, m1;
+  gang_span(m, C.G, C.g, m0, m1);
+  for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
+    const long row = e / c, col = e - row * c;
+    D.qw[e] = transposed ? g[col * m + row] : g[e];
+  }
+  __syncthreads();
+  if (!C.sync()) return -1;
+  const long lim = C.lds_cap < HHQR_LDS_CAP ? C.lds_cap : HHQR_LDS_CAP;
+  const int r = HHBlockLDS<8>::need(m) <= lim ? gang_hhqr_run<8>(C, m, c) : gang_hhqr_run<4>(C, m, c);
+  if (r < 0) return -1;
+  if (!C.sync()) return -1;  // Q rows and R complete
+  *fl += 4.0 * (double)m * c * c - 4.0 / 3.0 * (double)c * c * c;
+  *Qout = D.qq;
+  *Rout = D.rf;
+  return 0;
+}
+
 // Right-orthonormal start x0 (the sle engine solves from it): for i = d-1 .. 1
 // the split QR of core i's transposed unfolding (n b x a), Q^T into core i,
 // core i-1 <- core i-1 R^T (split over its rows, via g); a panel with fewer
@@ -948,11 +1061,15 @@ __device__ bool gang_ortho_right(GangCtx& C, double* fl) {
       if (!C.sync()) return false;  // a pivot failed on every member alike
       if (q == 1) C.chol_off = true;
     }
-    if (C.chol_off && m >= c && c <= 64 && (long)c * c <= D.rmax2 && !gang_blocked_qr_fits(C, m, c)) {
-      // a numerically rank-deficient panel too tall for member 0's blocked
-      // QR: the split column sweep
+    const bool split_hh = gang_split_qr_fits(C, m, c);
+    if (C.chol_off && m >= c &&
+        (split_hh || (c <= 64 && (long)c * c <= D.rmax2 && !gang_blocked_qr_fits(C, m, c)))) {
+      // a numerically rank-deficient panel: the split blocked QR where it
+      // fits, else the split column sweep
       const double *Q2, *R2;
-      if (gang_hhqr(C, D.xc[i], m, c, true, fl, &Q2, &R2) < 0) return false;
+      if ((split_hh ? gang_hhqr_blocked(C, D.xc[i], m, c, true, fl, &Q2, &R2)
+                    : gang_hhqr(C, D.xc[i], m, c, true, fl, &Q2, &R2)) < 0)
+        return false;
       int m0, m1;
       gang_span(m, C.G, C.g, m0, m1);
       for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
@@ -1091,7 +1208,9 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
   const long long tq = ph_now();
   if (i > 0 || forward) {
     const int m = forward ? a * n : n * b, c = forward ? b : a;
-    if (m >= c && (long)c * c <= D.rmax2 && !(C.chol_off && (c > 64 || gang_blocked_qr_fits(C, m, c)))) {
+    const bool split_hh = gang_split_qr_fits(C, m, c);
+    if (m >= c && (long)c * c <= D.rmax2 &&
+        !(C.chol_off && !split_hh && (c > 64 || gang_blocked_qr_fits(C, m, c)))) {
       double qfl = 0.0;
       const double *Q = nullptr, *Rt = nullptr;
       const long long t_cq = ph_now();
@@ -1138,10 +1257,12 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       // a pivot failed (or non-finite, or skipped): every member got here alike
       if (!C.sync()) return ST_ABORT;
       if (q == 1) C.chol_off = true;
-      if (q == 1 && c <= 64 && !gang_blocked_qr_fits(C, m, c)) {
+      if (q == 1 && (split_hh || (c <= 64 && !gang_blocked_qr_fits(C, m, c)))) {
         const double* Q2 = nullptr;
         const double* R2 = nullptr;
-        if (gang_hhqr(C, D.g, m, c, !forward, &qfl, &Q2, &R2) < 0) return ST_ABORT;
+        if ((split_hh ? gang_hhqr_blocked(C, D.g, m, c, !forward, &qfl, &Q2, &R2)
+                      : gang_hhqr(C, D.g, m, c, !forward, &qfl, &Q2, &R2)) < 0)
+          return ST_ABORT;
         int m0, m1;
         gang_span(m, C.G, C.g, m0, m1);
         for (long e = (long)m0 * c + threadIdx.x; e < (long)m1 * c; e += NT) {
