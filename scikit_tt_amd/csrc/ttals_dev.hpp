@@ -327,13 +327,19 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
   // loader position (tile origin, K offset) and compute position, advanced without division
   int lm0 = 0, ln0 = 0, lk0 = 0, ltj = 0;
   int cm0 = 0, cn0 = 0, ckk = 0, ctj = 0;
+  // plain (single-stride) index maps -- most operands -- skip the split-map
+  // arithmetic (a wave-uniform branch)
+  const bool a_plain = A.r.div >= (1 << 22) && A.c.div >= (1 << 22);
+  const bool b_plain = B.r.div >= (1 << 22) && B.c.div >= (1 << 22);
   auto load = [&](double (&ra)[4], double (&rb)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gm = lm0 + a_i0 + j * a_di, gk = lk0 + a_k0 + j * a_dk;
-      ra[j] = (gm < M && gk < K) ? A.p[gmap_at(Ar, gm) + gmap_at(Ac, gk)] : 0.0;
+      const int ao = a_plain ? gm * A.r.lo + gk * A.c.lo : gmap_at(Ar, gm) + gmap_at(Ac, gk);
+      ra[j] = (gm < M && gk < K) ? A.p[ao] : 0.0;
       const int gn = ln0 + b_n0 + j * b_dn, gk2 = lk0 + b_k0 + j * b_dk;
-      rb[j] = (gn < N && gk2 < K) ? B.p[gmap_at(Br, gk2) + gmap_at(Bc, gn)] : 0.0;
+      const int bo = b_plain ? gk2 * B.r.lo + gn * B.c.lo : gmap_at(Br, gk2) + gmap_at(Bc, gn);
+      rb[j] = (gn < N && gk2 < K) ? B.p[bo] : 0.0;
     }
     lk0 += GBK;
     if (lk0 >= K) {
