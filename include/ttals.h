@@ -70,6 +70,8 @@ typedef struct ttals_result {
   double cycles[7];        /* shader clock cycles: total, matvec, Gram-Schmidt, tridiagonal eigen,
                               QR + warm start, stack updates, on-chip set-up (diagnostics; only in
                               a library built with -DTTALS_PROFILE=1, else 0) */
+  double device_ms;        /* device time of this system's solve inside its launch (the constant
+                              clock, first to last step of its workgroup or gang's member 0) */
 } ttals_result;
 
 /* ---- context ------------------------------------------------------------ */
@@ -223,6 +225,11 @@ int ttals_sle_plan(ttals_ctx* ctx, int count, const ttals_tt* A, const ttals_tt*
  * systems (0: one workgroup).  No counterpart in the reference: a
  * diagnostic of the mixed schedule. */
 int ttals_last_plan(ttals_ctx* ctx, int* gangs, int count);
+
+/* Launch time (ms) the model predicted for the last mixed-schedule sle
+ * batch on this context (0 when the last batch was not one).  No
+ * counterpart in the reference: a diagnostic of the mixed schedule. */
+int ttals_last_plan_ms(ttals_ctx* ctx, double* ms);
 
 /* Device time (ms) of the engine kernel of the context's last finished
  * submission, from HIP events recorded on its stream immediately before and

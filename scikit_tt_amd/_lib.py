@@ -39,7 +39,8 @@ class Result(ctypes.Structure):
                 ('lambda_', ctypes.c_double),
                 ('worst_residual', ctypes.c_double),
                 ('flops', ctypes.c_double),
-                ('cycles', ctypes.c_double * 7)]
+                ('cycles', ctypes.c_double * 7),
+                ('device_ms', ctypes.c_double)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -70,6 +71,7 @@ SIGNATURES = {
     'ttals_get_cost_model': (_i, [_vp, _vp, _i]),
     'ttals_sle_cost': (_i, [_vp, _i, _TTp, _TTp, _vp]),
     'ttals_last_plan': (_i, [_vp, _vp, _i]),
+    'ttals_last_plan_ms': (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     'ttals_sle_plan': (_i, [_vp, _i, _TTp, _TTp, _i, _i, _vp, _vp]),
     'ttals_stack_left_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     'ttals_stack_right_op': (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),

@@ -1038,6 +1038,7 @@ struct ttals_ctx {
   long arena = 61 * 1024 / 8;
   bool arena_set = false;    // the caller chose the arena (ttals_set_arena_kb)
   long arena_hybrid = 0;     // k_hybrid's arena in doubles (0: not yet sized)
+  double wall_khz = 0.0;     // the device's constant clock (result rows' device times)
   int tol_policy = 0;        // k > 0: adaptive evp local tolerance capped at 1e-k (SysDesc::adapt)
   int matvec_variant = 0;    // SysDesc::mvar
   int schedule = 0;          // sle batches: 0 one workgroup per system, 1 gangs for the systems that set the launch time
@@ -1045,6 +1046,7 @@ struct ttals_ctx {
   long long budget = 0;      // SysDesc/GangDesc::budget of later submissions (100 MHz clock ticks; 0: none)
   CostModel cost;
   std::vector<int> last_plan;  // gang size per system of the last sle batch (0: one workgroup)
+  double last_plan_ms = 0.0;   // modelled makespan of the last mixed-schedule launch (0: none)
   std::string err;
 };
 
@@ -1295,7 +1297,7 @@ void take_offsets(int kind, const SysSizes& z, Layout& hl, Layout& dl, SysOff& o
   o.w_tau = dl.take(8 * z.rmax);
   o.w_rf = dl.take(8 * z.rmax * z.rmax);
   o.w_best = dl.take(kind == 1 ? 8 * z.xsz : 8);
-  o.w_out = dl.take(8 * 16);
+  o.w_out = dl.take(8 * RES_HEAD);
   o.w_sym = dl.take(kind == 1 ? 8 * 4 * (size_t)z.ramax * z.ramax : 8);
 }
 
@@ -1449,7 +1451,7 @@ int enqueue_batch(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, Placemen
   // compact results: one row of 16 scalars + (dmax+1) ranks per system, on
   // the device after the workspace and in the staging buffer after the
   // metadata
-  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t res_stride = RES_HEAD + (size_t)dmax + 1;
   const size_t w_res = dl.take(8 * res_stride * count);
   Layout rl;
   rl.bytes = meta_bytes;
@@ -1498,9 +1500,11 @@ int enqueue_batch(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, Placemen
   return 0;
 }
 
-// Unpacks one system's compact result row; returns its status.
-int unpack_result(const double* o, int d, int* ranks, ttals_result& R) {
-  for (int i = 0; i <= d; ++i) ranks[i] = (int)o[16 + i];
+// Unpacks one system's compact result row (khz: the constant clock's rate);
+// returns its status.
+int unpack_result(const double* o, int d, int* ranks, ttals_result& R, double khz) {
+  for (int i = 0; i <= d; ++i) ranks[i] = (int)o[RES_HEAD + i];
+  R.device_ms = khz > 0.0 ? o[16] / khz : 0.0;
   R.lambda = o[0];
   R.sweeps = (int)o[1];
   R.status = (int)o[2];
@@ -1535,7 +1539,10 @@ int submit_batch(ttals_ctx* c, hipStream_t stream, int kind, int count, const tt
   size_t r_res = 0, stride = 0;
   int rc = (kind == 0 && c->schedule == 1) ? enqueue_hybrid(c, stream, B, &r_res, &stride)
                                            : enqueue_batch(c, stream, B, pl, &r_res, &stride, true);
-  if (!rc && !(kind == 0 && c->schedule == 1)) c->last_plan.assign(kind == 0 ? count : 0, 0);
+  if (!rc && !(kind == 0 && c->schedule == 1)) {
+    c->last_plan.assign(kind == 0 ? count : 0, 0);
+    c->last_plan_ms = 0.0;
+  }
   if (rc) return rc;
   if (!c->done) HIPCHK(c, hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
   HIPCHK(c, hipEventRecord(c->done, stream));
@@ -1553,7 +1560,7 @@ int finish_batch(ttals_ctx* c, ttals_result* res) {
   int worst = 0;
   for (size_t s = 0; s < c->pending.size(); ++s) {
     const Pending& pd = c->pending[s];
-    const int st = unpack_result(hptr<double>(c, pd.r_res), pd.d, pd.ranks, res[s]);
+    const int st = unpack_result(hptr<double>(c, pd.r_res), pd.d, pd.ranks, res[s], c->wall_khz);
     if (st == ST_ABORT) return fail(c, TTALS_E_HIP, "gang barrier timed out (system " + std::to_string(s) + ")");
     worst = worst_status(worst, st);
   }
@@ -1903,7 +1910,7 @@ int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t*
     if ((rc = size_system(c, 0, B.A[s], B.x0[s], &B.b[s], B.xo[s], s, z))) return rc;
   std::vector<int> G;
   std::vector<double> t1;
-  (void)plan_batch(c->cost, arena, c->matvec_variant, count, B.A, B.x0, B.repeats, slots, G, t1);
+  c->last_plan_ms = plan_batch(c->cost, arena, c->matvec_variant, count, B.A, B.x0, B.repeats, slots, G, t1);
   int ng = 0, wg = 0;
   for (int s = 0; s < count; ++s)
     if (G[s] > 0) {
@@ -1930,7 +1937,7 @@ int enqueue_hybrid(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, size_t*
   const size_t m_sdesc = hl.take(sizeof(SysDesc) * std::max(nf, 1));
   int dmax = 0;
   for (int s = 0; s < count; ++s) dmax = std::max(dmax, B.A[s].order);
-  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t res_stride = RES_HEAD + (size_t)dmax + 1;
   const size_t w_sync = dl.take(sizeof(GangSync) * ng);
   const size_t w_res = dl.take(8 * res_stride * count);
   GOff go;
@@ -2027,7 +2034,7 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
     return fail(c, TTALS_E_ARG, "the gangs need " + std::to_string(total) + " resident workgroups; the device holds " +
                                     std::to_string(cap));
   const size_t m_blocks = hl.take(sizeof(int) * total);
-  const size_t res_stride = 16 + (size_t)dmax + 1;
+  const size_t res_stride = RES_HEAD + (size_t)dmax + 1;
   const size_t w_sync = dl.take(sizeof(GangSync) * count);
   const size_t w_res = dl.take(8 * res_stride * count);
   GOff go;
@@ -2085,7 +2092,7 @@ int gang_solve(ttals_ctx* c, hipStream_t stream, int count, const ttals_tt* A, c
   int worst = 0;
   for (int s = 0; s < count; ++s) {
     const int st = unpack_result(reinterpret_cast<const double*>(hbase + r_res + 8 * res_stride * s), A[s].order,
-                                 xo[s].ranks, res[s]);
+                                 xo[s].ranks, res[s], c->wall_khz);
     if (st == ST_ABORT) return fail(c, TTALS_E_HIP, "gang barrier timed out (system " + std::to_string(s) + ")");
     worst = worst_status(worst, st);
   }
@@ -2123,6 +2130,9 @@ int ttals_ctx_create(int device, ttals_ctx** out) {
   }
   ttals_ctx* c = new ttals_ctx();
   c->device = device;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+    c->wall_khz = (double)khz;
   if (const char* e = std::getenv("TTALS_ARENA_KB")) {
     const long kb = std::atol(e);
     if (kb >= 0 && kb <= 61) c->arena = kb * 1024 / 8;
@@ -2279,6 +2289,12 @@ int ttals_sle_plan(ttals_ctx* c, int count, const ttals_tt* A, const ttals_tt* x
   return 0;
 }
 
+int ttals_last_plan_ms(ttals_ctx* c, double* ms) {
+  if (!c || !ms) return c ? fail(c, TTALS_E_ARG, "null argument") : TTALS_E_ARG;
+  *ms = c->last_plan_ms;
+  return 0;
+}
+
 int ttals_last_plan(ttals_ctx* c, int* gangs, int count) {
   if (!c) return TTALS_E_ARG;
   if (!gangs || count != (int)c->last_plan.size())
@@ -2325,7 +2341,7 @@ int ttals_sle_als_batch_static(ttals_ctx* c, void* stream, int count, const ttal
   int worst = 0;
   for (int s = 0; s < count; ++s)
     worst = worst_status(worst, unpack_result(reinterpret_cast<const double*>(g_static_host + r_res + 8 * stride * s),
-                                              A[s].order, x_out[s].ranks, res[s]));
+                                              A[s].order, x_out[s].ranks, res[s], c->wall_khz));
   return worst;
 }
 

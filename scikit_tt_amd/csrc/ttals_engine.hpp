@@ -34,6 +34,12 @@ __device__ __forceinline__ bool over_budget(long long budget, unsigned long long
 // value of an exactly symmetric operator near sqrt(eps) ~ 1e-8.
 constexpr double SYM_TOL = 1e-6;
 
+// Leading doubles of a system's compact result row (then its d+1 final
+// ranks): lambda, sweeps, status, Krylov iterations, unconverged solves,
+// local solves, most iterations in one solve, worst residual, flops, the
+// seven cycle counters of profiling builds, constant-clock ticks of the solve.
+constexpr int RES_HEAD = 17;
+
 struct SysDesc {
   int d, kind, solver, repeats;
   int ortho_x0, sym, gm_restart, maxit;
@@ -70,7 +76,7 @@ struct SysDesc {
   double* rf;                 // rmax^2 (clean R factor)
   double* out;                // [16] lambda, sweeps, status, krylov iters, unconverged, local solves, ...
   const double* const* x0c;   // [d] initial-guess cores, copied into xc at the start (skipped where equal)
-  double* res;                // compact result row: out[0..15], then the d+1 final ranks
+  double* res;                // compact result row: out[0..16], then the d+1 final ranks
   double* trace;              // optional: 6 doubles per local solve (nullptr = off)
   long trace_cap;             // capacity in local solves
   double* symw;               // evp: 4 * (max operator rank)^2 doubles for the symmetry test
@@ -754,6 +760,7 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     D.out[8] = st.flops;
     D.out[9] = (double)(ph_now() - tstart);
     for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
+    D.out[16] = (double)(wall_clock64() - t_begin);  // constant-clock ticks of the solve
   }
 }
 
@@ -773,7 +780,7 @@ __device__ __forceinline__ void als_block(const SysDesc& D, double* dyn_lds, lon
   als_solve<double>(D, dyn_lds, arena, S);
   __syncthreads();
   // one compact row per system, so the host fetches every result with one copy
-  for (int k = threadIdx.x; k < 17 + D.d; k += NT) D.res[k] = k < 16 ? D.out[k] : (double)D.rx[k - 16];
+  for (int k = threadIdx.x; k < RES_HEAD + 1 + D.d; k += NT) D.res[k] = k < RES_HEAD ? D.out[k] : (double)D.rx[k - RES_HEAD];
 }
 
 }  // namespace ttals

@@ -1438,8 +1438,9 @@ __device__ void gang_als(GangCtx& C) {
     D.out[9] = (double)(ph_now() - tstart);
     for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
     D.out[15] = (double)S.prof[PH_OTHER];
-    for (int k = 0; k < 16; ++k) D.res[k] = D.out[k];
-    for (int k = 0; k <= d; ++k) D.res[16 + k] = (double)D.rx[k];
+    D.out[16] = (double)(wall_clock64() - t_begin);  // constant-clock ticks of the gang's solve
+    for (int k = 0; k < RES_HEAD; ++k) D.res[k] = D.out[k];
+    for (int k = 0; k <= d; ++k) D.res[RES_HEAD + k] = (double)D.rx[k];
   }
 }
 

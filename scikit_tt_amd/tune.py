@@ -115,8 +115,25 @@ def _kernel_ms(ctx):
     return ms.value
 
 
+def _fit_rates(C, t, repeats, fallback):
+    """on, off, solve rates: least squares of repeats * C @ rates = t in
+    relative error, rates >= 0 (a rate with no cost behind it keeps its
+    fallback value)."""
+    M = repeats * C / t[:, None]
+    try:
+        from scipy.optimize import nnls
+        rates, _ = nnls(M, np.ones(len(t)))
+    except ImportError:
+        rates = np.linalg.lstsq(M, np.ones(len(t)), rcond=None)[0]
+    rates = np.maximum(rates, 1e-15)
+    for j in range(3):
+        if not np.any(C[:, j] > 0):
+            rates[j] = fallback[j]
+    return rates
+
+
 def calibrate(operators, initial_guesses, right_hand_sides, repeats=1, sample=10, gang_sample=3, gangs=GANGS[1:],
-              precision='fp64', install=True):
+              precision='fp64', install=True, load_passes=3):
     """Fit the model on the device from timings of systems of this batch.
 
     ``sample`` systems spread over the batch's range of modelled cost (the
@@ -124,8 +141,18 @@ def calibrate(operators, initial_guesses, right_hand_sides, repeats=1, sample=10
     and solve rates are their least-squares fit in relative error.  The
     ``gang_sample`` largest are timed on gangs of each size in ``gangs``
     (at most their widest rank); gang[k] is the median over them of
-    (gang time - modelled per-solve time) / off-chip-priced matvec time.  Returns a dict with the fitted
-    parameters and the timings; ``install`` sets them on the context.
+    (gang time - modelled per-solve time) / off-chip-priced matvec time.
+
+    A system runs slower inside a full launch than alone (the systems share
+    L2 and HBM), and not uniformly: large global-memory systems more than
+    gangs and small on-chip systems.  ``load_passes`` times the batch itself
+    that many times with the current model and, from every system's device
+    time inside the launch (``device_ms`` of its report), scales the
+    off-chip rate by the median slowdown of the large single-workgroup
+    systems and the gang factors by the gangs' slowdown relative to theirs.
+    Returns a dict with the fitted parameters, the timings and each load
+    pass's planned and measured launch; ``install`` sets them on the context
+    (load passes need it).
     """
     import scikit_tt_amd.solvers.sle as sle
     count = len(operators)
@@ -152,18 +179,7 @@ def calibrate(operators, initial_guesses, right_hand_sides, repeats=1, sample=10
         t1.append(best)
     t1 = np.array(t1)
     C = c[picks]
-    # least squares in relative error, rates >= 0 (tiny floor keeps them valid)
-    M = repeats * C / t1[:, None]
-    try:
-        from scipy.optimize import nnls
-        rates, _ = nnls(M, np.ones(len(picks)))
-    except ImportError:
-        rates = np.linalg.lstsq(M, np.ones(len(picks)), rcond=None)[0]
-    rates = np.maximum(rates, 1e-15)
-    if not np.any(C[:, 0] > 0):
-        rates[0] = p0[0]
-    if not np.any(C[:, 1] > 0):
-        rates[1] = p0[1]
+    rates = _fit_rates(C, t1, repeats, p0)
     params = p0.copy()
     params[:3] = rates
     # gangs on the largest systems
@@ -192,8 +208,43 @@ def calibrate(operators, initial_guesses, right_hand_sides, repeats=1, sample=10
     if install:
         set_model(params, dev)
     fit = predict(C, repeats, params)
-    return {'params': dict(zip(PARAMS, params.tolist())),
-            'systems_timed': len(picks),
-            'one_workgroup_ms': t1.tolist(),
-            'fit_rel_err_max': float(np.max(np.abs(fit - t1) / t1)),
-            'gang_ms': {'%d/G%d' % key: v for key, v in gang_ms.items()}}
+    out = {'params_alone': dict(zip(PARAMS, params.tolist())),
+           'systems_timed': len(picks),
+           'one_workgroup_ms': t1.tolist(),
+           'fit_rel_err_max': float(np.max(np.abs(fit - t1) / t1)),
+           'gang_ms': {'%d/G%d' % key: v for key, v in gang_ms.items()},
+           'load_passes': []}
+    if install:
+        lib = _lib.load()
+        for _ in range(int(load_passes)):
+            g = np.zeros(count, dtype=np.int32)
+            _, infos = sle.als_batch(operators, initial_guesses, right_hand_sides, repeats=repeats, schedule='auto',
+                                     gang_precision=precision, return_info=True)
+            launch = _kernel_ms(ctx)
+            planned = ctypes.c_double()
+            _lib.check(lib.ttals_last_plan_ms(ctx, ctypes.byref(planned)), ctx, 'last_plan_ms')
+            _lib.check(lib.ttals_last_plan(ctx, g.ctypes.data, count), ctx, 'last_plan')
+            t = infos.array['device_ms'].copy()
+            # in-launch over modelled time: the large single-workgroup systems
+            # (global-memory bound, the ones a plan leaves ungated) and the gangs
+            t_model = predict(c, repeats, params)
+            tmv = repeats * params[1] * (c[:, 0] + c[:, 1])
+            tov = repeats * params[2] * c[:, 2]
+            gang_model = np.zeros(count)
+            for j, G in enumerate(GANGS):
+                sel = g == G
+                if j > 0 and sel.any():
+                    gang_model[sel] = tmv[sel] * params[3 + j] + tov[sel]
+            big = (g == 0) & (t > 0) & (t_model >= 0.3 * t_model.max())
+            ganged = (g > 0) & (t > 0) & (gang_model > 0)
+            if big.sum() >= 3:
+                f_s = float(np.median(t[big] / t_model[big]))
+                f_g = float(np.median(t[ganged] / gang_model[ganged])) if ganged.any() else f_s
+                params[1] *= f_s          # off-chip rate of single systems under load
+                params[4:] *= f_g / f_s   # gangs keep their own slowdown
+            set_model(params, dev)
+            out['load_passes'].append({'planned_ms': planned.value, 'launch_ms': launch,
+                                       'gang_sizes': {int(k): int(v) for k, v in zip(*np.unique(g, return_counts=True))}})
+    out['params'] = dict(zip(PARAMS, params.tolist()))
+    return out
+
