@@ -76,8 +76,19 @@ def test_calibrated_model_and_auto_schedule(env):
     print('  fit', fit, flush=True)
     assert np.all(np.isfinite(p)) and np.all(p > 0)
     np.testing.assert_array_equal(tune.get_model(0), p)
+    # the load passes launched the batch itself and read its planned makespan
+    assert len(fit['load_passes']) >= 1
+    for lp in fit['load_passes']:
+        assert lp['launch_ms'] > 0 and lp['planned_ms'] > 0, lp
     sols, infos = sle.als_batch(As, Xs, Bs, repeats=1, schedule='auto', return_info=True)
     assert (infos.array['status'] == 0).all()
+    # every system reports its own device time inside the launch, no longer than the launch
+    from scikit_tt_amd import _lib
+    import ctypes
+    ms = ctypes.c_double()
+    assert _lib.load().ttals_last_launch_ms(_lib.context(0), ctypes.byref(ms)) == 0
+    t = infos.array['device_ms']
+    assert np.all(t > 0) and np.all(t <= ms.value * 1.01 + 0.05), (t.max(), ms.value)
     res, bn = residual_norms(As, sols, Bs)
     ref = sle.als_batch(As, Xs, Bs, repeats=1)
     res1, _ = residual_norms(As, ref, Bs)
