@@ -19,7 +19,10 @@ Also printed: the same batch's rel.res distribution and rank-floor check
 solution), p50/p99 step latency, the roofline of the engine kernel
 (algorithmic FP64 flops counted by the engine from shapes and the Krylov
 work it did, over the kernel time measured with HIP events on the launch
-stream), a secondary evp workload (2048 TFIM chains, d=32, rank 16, 6
+stream; beside it the algorithmic bytes of the local matvecs, the stacks,
+operator core and iterate read once per matvec plus the input and output
+vectors where they live in global memory, against the measured traffic), a
+secondary evp workload (2048 TFIM chains, d=32, rank 16, 6
 sweeps) with its eigenvalue error against the reference-written value of
 instance 0 and against the FP64 fixed-tolerance run, a serving phase, and
 the CPU baseline (the numpy oracle, whole solves, on the subset of the
@@ -373,6 +376,7 @@ def main():
     elapsed = time.perf_counter() - t0
     arr = infos.array
     flops = float(arr['flops'].sum())
+    mv_bytes = float(arr['bytes'].sum())  # algorithmic bytes of the launch's local matvecs
     status_ok = arr['status'] == 0
     gangs = np.asarray(infos.gangs)
     # the same batch on one workgroup per system, once (reported beside the schedule used)
@@ -410,10 +414,15 @@ def main():
     plain_time = time.perf_counter() - tp
     pres, _ = residual_norms([As[s] for s in idx], plain_sols, [Bs[s] for s in idx])
     prel = pres / bn[idx]
-    # systems the batch ran on one workgroup: the same engine, bit for bit;
-    # ganged systems: the same iterates to the engines' agreement (~1e-9)
+    # the plain path runs k_als; the batch's one-workgroup systems run inside
+    # k_hybrid (larger on-chip arena, other QR row grouping) and its ganged
+    # systems on gangs: the same iterates to rounding, not bit for bit
     one = gangs[idx] == 0
-    plain_same = bool(np.array_equal(prel[one], relres[idx][one]))
+
+    def relres_ratio(sel):
+        if not sel.any():
+            return None
+        return float(np.max(np.maximum(prel[sel], 1e-300) / np.maximum(relres[idx][sel], 1e-300)))
     plain_ok = int((prel <= GATE).sum())
     plain_rate = plain_ok / plain_time  # gated solves per second of the sampled systems
     log('plain path: %d systems in %.1f s, %d meet the gate' % (P, plain_time, plain_ok))
@@ -459,16 +468,19 @@ def main():
         'plain_fp64_systems_timed': P,
         'plain_fp64_seconds': round(plain_time, 3),
         'plain_fp64_solves_per_s': round(plain_rate, 3),
-        'plain_relres_identical_to_batched_one_workgroup_systems': plain_same,
-        'plain_vs_batched_ganged_systems_max_relres_ratio': (
-            float(np.max(np.maximum(prel[~one], 1e-300) / np.maximum(relres[idx][~one], 1e-300))) if (~one).any()
-            else None),
+        'plain_gate_agrees_with_batched': bool(np.array_equal(prel <= GATE, relres[idx] <= GATE)),
+        'plain_vs_batched_one_workgroup_systems_max_relres_ratio': relres_ratio(one),
+        'plain_vs_batched_ganged_systems_max_relres_ratio': relres_ratio(~one),
         'speedup_batched_vs_plain': round(value / plain_rate, 2) if plain_rate > 0 else None,
         'latency_ms_p50': round(float(np.percentile(lat, 50)), 3),
         'latency_ms_p99': round(float(np.percentile(lat, 99)), 3),
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_FP64_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / PEAK_FP64_TFLOPS, 'traffic': traffic, 'traffic_source': traffic_src,
-                     'kernel': kernel, 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3, **counters},
+                     'kernel': kernel, 'flops_per_launch': flops, 'launch_ms': avg_kernel_s * 1e3,
+                     'algorithmic_bytes_per_launch': mv_bytes,
+                     'algorithmic_hbm_frac': mv_bytes / avg_kernel_s / (PEAK_HBM_GBS * 1e9),
+                     'traffic_over_algorithmic_bytes': (traffic / mv_bytes) if (traffic and mv_bytes > 0) else None,
+                     **counters},
     }
     if args.tfim_batch > 0:
         log('tfim evp batch %d' % args.tfim_batch)

@@ -72,6 +72,11 @@ typedef struct ttals_result {
                               a library built with -DTTALS_PROFILE=1, else 0) */
   double device_ms;        /* device time of this system's solve inside its launch (the constant
                               clock, first to last step of its workgroup or gang's member 0) */
+  double bytes;            /* algorithmic bytes of the local solves: per matvec the stacks,
+                              operator core and iterate read once (+ the input and output vectors
+                              where they live in global memory), one element per vector flop on
+                              global-memory vectors; dense local solves: the operator pieces once
+                              + the micro matrix written and read once */
 } ttals_result;
 
 /* ---- context ------------------------------------------------------------ */

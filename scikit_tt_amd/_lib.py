@@ -40,7 +40,8 @@ class Result(ctypes.Structure):
                 ('worst_residual', ctypes.c_double),
                 ('flops', ctypes.c_double),
                 ('cycles', ctypes.c_double * 7),
-                ('device_ms', ctypes.c_double)]
+                ('device_ms', ctypes.c_double),
+                ('bytes', ctypes.c_double)]
 
 
 # exported symbol -> (restype, argtypes)

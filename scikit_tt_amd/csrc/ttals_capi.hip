@@ -1505,6 +1505,7 @@ int enqueue_batch(ttals_ctx* c, hipStream_t stream, const BatchArgs& B, Placemen
 int unpack_result(const double* o, int d, int* ranks, ttals_result& R, double khz) {
   for (int i = 0; i <= d; ++i) ranks[i] = (int)o[RES_HEAD + i];
   R.device_ms = khz > 0.0 ? o[16] / khz : 0.0;
+  R.bytes = o[17];
   R.lambda = o[0];
   R.sweeps = (int)o[1];
   R.status = (int)o[2];

@@ -37,8 +37,9 @@ constexpr double SYM_TOL = 1e-6;
 // Leading doubles of a system's compact result row (then its d+1 final
 // ranks): lambda, sweeps, status, Krylov iterations, unconverged solves,
 // local solves, most iterations in one solve, worst residual, flops, the
-// seven cycle counters of profiling builds, constant-clock ticks of the solve.
-constexpr int RES_HEAD = 17;
+// seven cycle counters of profiling builds, constant-clock ticks of the solve,
+// algorithmic bytes of the local solves.
+constexpr int RES_HEAD = 18;
 
 struct SysDesc {
   int d, kind, solver, repeats;
@@ -173,6 +174,14 @@ __device__ __forceinline__ double fl_op(double a, double P, double n, double Q, 
   // stack update or local matvec: 2abn(aP + PnQ + bQ)
   return 2.0 * a * b * n * (a * P + P * n * Q + b * Q);
 }
+// algorithmic bytes of one local matvec at s bytes per element: the left and
+// right stacks, the operator core and the iterate read once, plus the input
+// and output vectors (2N) when they live in global memory.  The solvers'
+// vector work (Gram-Schmidt, updates) on global-memory vectors is charged
+// one element moved per vector flop on top.
+__device__ __forceinline__ double by_op(double a, double P, double n, double Q, double b, double s, bool global_vec) {
+  return s * (a * P * a + a * n * b + P * n * n * Q + b * Q * b + (global_vec ? 2.0 * a * n * b : 0.0));
+}
 __device__ __forceinline__ double fl_rhs(double a, double p, double n, double q, double b) {
   return 2.0 * a * n * q * (p + b);
 }
@@ -293,6 +302,7 @@ struct StepStats {
   double last_res = 0.0;
   long nsolve = 0;
   double flops = 0.0;  // algorithmic flops of the whole solve
+  double mvb = 0.0;    // algorithmic bytes of the local solves (matvecs and vector work)
   double tol = 0.0;    // local Krylov tolerance of the current sweep
 };
 
@@ -428,6 +438,8 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       int* piv = reinterpret_cast<int*>(D.dense + (long)NDENSE * NDENSE);
       if (!wg_lu_solve(D.dense, N, D.g, piv, S.red)) return ST_SINGULAR;
       st.flops += 3.0 * (double)N * N * P * Q + (2.0 / 3.0) * (double)N * N * N + 2.0 * (double)N * N;
+      // the operator pieces read once, the micro matrix written and read once
+      st.mvb += by_op(a, P, n, Q, b, 8.0, false) + 8.0 * (2.0 * (double)N * N - N);
     } else {
       int r = 1;
       KrylovOut ko;
@@ -455,6 +467,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
       // the matvecs made + the vector work the solver counted as it ran
       // (Gram-Schmidt passes as made)
       st.flops += ko.nmv * fmv + S.vfl + 6.0 * N;
+      st.mvb += ko.nmv * by_op(a, P, n, Q, b, 8.0, !onchip) + (onchip ? 0.0 : 8.0 * S.vfl);
     }
   } else {
     const int mode = (D.solver == 1) ? 1 : 2;
@@ -463,6 +476,7 @@ __device__ int local_step(int i, bool forward, const SysDesc& D, bool sym, ACC* 
     // matvecs + the vector work the solver counted as it ran (local passes
     // every step, full Gram-Schmidt only where it was made, Ritz vectors)
     st.flops += its * fmv + S.vfl;
+    st.mvb += its * by_op(a, P, n, Q, b, 8.0, !onchip) + (onchip ? 0.0 : 8.0 * S.vfl);
     if (r == 3) return ST_UNSUPPORTED;
     if (r == 4) return ST_NONFINITE;
     if (r != 0) ++st.nunconv;
@@ -761,6 +775,7 @@ __device__ void als_solve(const SysDesc& D, ACC* lds, long arena, SolveLDS& S) {
     D.out[9] = (double)(ph_now() - tstart);
     for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
     D.out[16] = (double)(wall_clock64() - t_begin);  // constant-clock ticks of the solve
+    D.out[17] = st.mvb;
   }
 }
 

@@ -1129,6 +1129,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
     if (r == 1) r = gang_gmres<double>(C, o, o.L, o.A, o.R, D.g, D.f, D.q, D.V, D.tol, D.maxit, D.gm_restart, &its,
                                         &vfl);
     st.flops += (its + 2) * fmv + vfl;
+    st.mvb += (its + 2) * by_op(a, o.P, n, o.Q, b, 8.0, true) + 8.0 * vfl;
   } else {
     // FP32 copies of the local operator
     {
@@ -1165,6 +1166,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       const double fn = sqrt(S.scal[2]), rn = sqrt(S.scal[3]);
       vfl += 5.0 * o.N;
       st.flops += fmv;
+      st.mvb += by_op(a, o.P, n, o.Q, b, 8.0, true) + 40.0 * o.N;
       if (fn == 0.0 || rn <= D.tol * fn) {
         r = 0;
         break;
@@ -1179,6 +1181,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       }
       if (!C.sync()) return ST_ABORT;
       int ii = 0, ri = 1;
+      const double vfl0 = vfl;
       if (sym)
         ri = gang_cg<float>(C, o, D.L32, D.A32, D.R32, D.x32, D.r32, D.p32, D.q32, reinterpret_cast<float*>(D.V),
                             D.tol_inner, D.maxit_inner, &ii, &vfl);
@@ -1188,6 +1191,7 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       if (ri < 0) return ST_ABORT;
       inner_total += ii;
       st.flops += (ii + 2) * fmv;
+      st.mvb += (ii + 2) * by_op(a, o.P, n, o.Q, b, 4.0, true) + 4.0 * (vfl - vfl0);
       for (long e = o.e0 + threadIdx.x; e < o.e1; e += NT) D.g[e] += rn * (double)D.x32[e];
       if (!C.sync()) return ST_ABORT;
     }
@@ -1439,6 +1443,7 @@ __device__ void gang_als(GangCtx& C) {
     for (int k = 0; k < 6; ++k) D.out[10 + k] = (double)S.prof[k];
     D.out[15] = (double)S.prof[PH_OTHER];
     D.out[16] = (double)(wall_clock64() - t_begin);  // constant-clock ticks of the gang's solve
+    D.out[17] = st.mvb;
     for (int k = 0; k < RES_HEAD; ++k) D.res[k] = D.out[k];
     for (int k = 0; k <= d; ++k) D.res[RES_HEAD + k] = (double)D.rx[k];
   }

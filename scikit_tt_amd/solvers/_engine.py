@@ -14,7 +14,7 @@ class SolveInfo:
     """Per-system engine report: status, sweeps, Krylov iterations, eigenvalue."""
 
     __slots__ = ('status', 'sweeps', 'unconverged', 'max_iters', 'krylov_iters', 'local_solves', 'eigenvalue',
-                 'worst_residual', 'flops', 'device_ms', '_r')
+                 'worst_residual', 'flops', 'device_ms', 'bytes', '_r')
     _CYCLE_NAMES = ('total', 'matvec', 'gram_schmidt', 'tridiag', 'qr', 'stacks', 'setup')
 
     def __init__(self, r):
@@ -28,6 +28,7 @@ class SolveInfo:
         self.worst_residual = float(r.worst_residual)
         self.flops = float(r.flops)
         self.device_ms = float(r.device_ms)
+        self.bytes = float(r.bytes)
         self._r = r  # keeps the result record for the (rarely read) cycle counters
 
     @property
@@ -43,7 +44,7 @@ class SolveInfo:
 RESULT_DTYPE = np.dtype([('status', '<i4'), ('sweeps', '<i4'), ('unconverged', '<i4'), ('max_iters', '<i4'),
                          ('krylov_iters', '<i8'), ('local_solves', '<i8'), ('eigenvalue', '<f8'),
                          ('worst_residual', '<f8'), ('flops', '<f8'), ('cycles', '<f8', (7,)),
-                         ('device_ms', '<f8')])
+                         ('device_ms', '<f8'), ('bytes', '<f8')])
 assert RESULT_DTYPE.itemsize == ctypes.sizeof(_lib.Result)
 
 

@@ -69,7 +69,7 @@ def _same(a, b):
     for x, y in zip(ca, cb):
         for u, v in zip(x, y):
             np.testing.assert_array_equal(u, v)
-    for f in ('status', 'sweeps', 'krylov_iters', 'local_solves', 'worst_residual', 'flops'):
+    for f in ('status', 'sweeps', 'krylov_iters', 'local_solves', 'worst_residual', 'flops', 'bytes'):
         np.testing.assert_array_equal(ra[f], rb[f])
 
 
