@@ -435,10 +435,13 @@ __device__ int gang_cg(GangCtx& C, const GangLocal& o, const T* L, const T* A, c
 
 // Restarted GMRES(m) on M x = f with classical Gram-Schmidt applied twice
 // (each pass one gang reduction).  Same conventions as gang_cg.  w is a
-// work vector, V the (m + 1) x N basis.
+// work vector, V the (m + 1) x N basis.  A run that converges in one or two
+// steps returns without the confirming residual matvec (the least-squares
+// residual of an orthonormal basis is the true one to rounding) and counts
+// it in *unconfirmed.
 template <class T>
 __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A, const T* R, T* x, const T* f,
-                          T* w, T* V, double tol, int maxit, int m, int* iters, double* vfl) {
+                          T* w, T* V, double tol, int maxit, int m, int* iters, double* vfl, int* unconfirmed) {
   const int G = C.G, g = C.g;
   const int ld = GANG_LD;
   double* part = C.D.part;
@@ -574,6 +577,10 @@ __device__ int gang_gmres(GangCtx& C, const GangLocal& o, const T* L, const T* A
     }
     if (threadIdx.x == 0) *vfl += 2.0 * N * jend;
     if (!C.sync()) return -1;  // x complete
+    if (res <= tol * fn && jend <= 2) {
+      ++*unconfirmed;
+      return 0;
+    }
     if (*iters >= maxit) {
       // final residual decides
       gang_matvec<T>(o.a, o.P, o.n, o.Q, o.b, L, A, R, x, w, o.r0, o.r1, t1, t2, lds);
@@ -1122,14 +1129,15 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
   st.flops += fl_rhs(a, D.rb[i], n, D.rb[i + 1], b);
   if (!C.sync()) return ST_ABORT;
   int its = 0, r = 0;
+  int unconf = 0;  // GMRES runs that ended without the confirming matvec
   double vfl = 0.0;
   if (D.prec == 0) {
     r = 1;
     if (sym) r = gang_cg<double>(C, o, o.L, o.A, o.R, D.g, D.f, D.r, D.p, D.q, D.tol, D.maxit, &its, &vfl);
     if (r == 1) r = gang_gmres<double>(C, o, o.L, o.A, o.R, D.g, D.f, D.q, D.V, D.tol, D.maxit, D.gm_restart, &its,
-                                        &vfl);
-    st.flops += (its + 2) * fmv + vfl;
-    st.mvb += (its + 2) * by_op(a, o.P, n, o.Q, b, 8.0, true) + 8.0 * vfl;
+                                        &vfl, &unconf);
+    st.flops += (its + 2 - unconf) * fmv + vfl;
+    st.mvb += (its + 2 - unconf) * by_op(a, o.P, n, o.Q, b, 8.0, true) + 8.0 * vfl;
   } else {
     // FP32 copies of the local operator
     {
@@ -1181,17 +1189,18 @@ __device__ int gang_step(GangCtx& C, int i, bool forward, bool sym, StepStats& s
       }
       if (!C.sync()) return ST_ABORT;
       int ii = 0, ri = 1;
+      unconf = 0;
       const double vfl0 = vfl;
       if (sym)
         ri = gang_cg<float>(C, o, D.L32, D.A32, D.R32, D.x32, D.r32, D.p32, D.q32, reinterpret_cast<float*>(D.V),
                             D.tol_inner, D.maxit_inner, &ii, &vfl);
       if (ri == 1)
         ri = gang_gmres<float>(C, o, D.L32, D.A32, D.R32, D.x32, D.r32, D.q32, D.V32, D.tol_inner, D.maxit_inner,
-                               D.gm_restart, &ii, &vfl);
+                               D.gm_restart, &ii, &vfl, &unconf);
       if (ri < 0) return ST_ABORT;
       inner_total += ii;
-      st.flops += (ii + 2) * fmv;
-      st.mvb += (ii + 2) * by_op(a, o.P, n, o.Q, b, 4.0, true) + 4.0 * (vfl - vfl0);
+      st.flops += (ii + 2 - unconf) * fmv;
+      st.mvb += (ii + 2 - unconf) * by_op(a, o.P, n, o.Q, b, 4.0, true) + 4.0 * (vfl - vfl0);
       for (long e = o.e0 + threadIdx.x; e < o.e1; e += NT) D.g[e] += rn * (double)D.x32[e];
       if (!C.sync()) return ST_ABORT;
     }

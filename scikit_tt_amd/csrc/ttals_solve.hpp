@@ -375,6 +375,9 @@ __device__ int wg_gmres(const LocalOp<double>& op, const Work& W, double tol, in
     if (threadIdx.x == 0) S.vfl += 2.0 * N * jend + N;
     __syncthreads();
     double tres = -1.0;  // x changed: its true residual is unknown until checked
+    // after one or two steps from an orthonormal basis the least-squares
+    // residual is the true residual to rounding: no confirming matvec
+    if (res <= tol * fn && jend <= 2) return done(0, res);
     if (res <= tol * fn) {
       // confirm with the true residual once
       apply_op<double, ACC>(op, x, w, lds);
