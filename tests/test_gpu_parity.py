@@ -138,6 +138,33 @@ def test_local_solve(env, arena, a, P, n, Q, b, method):
     np.testing.assert_allclose(x, np.linalg.solve(M, f), rtol=1e-8, atol=1e-9 * np.abs(x).max())
 
 
+@pytest.mark.parametrize('eps', [1e-14, 1e-7])
+def test_local_gmres_short_run_true_residual(env, arena, eps):
+    """GMRES runs of one or two steps return without a confirming matvec: the
+    least-squares residual they accept must be the true residual.  M = I +
+    eps (I x E x I) converges in one step (eps = 1e-14) or two (1e-7); the
+    true residual of the returned x is checked on the host against the
+    solver's acceptance bound (10 tol)."""
+    lib, ctx, _ = env
+    a, P, n, Q, b = 8, 1, 8, 1, 8
+    rng = np.random.default_rng(3)
+    L = np.eye(a)[:, None, :].copy()
+    R = np.eye(b)[:, None, :].copy()
+    Ac = (np.eye(n) + eps * rng.standard_normal((n, n)))[None, :, :, None].copy()
+    M = sle_als.micro_matrix(L, Ac, R)
+    N = M.shape[0]
+    f = rng.standard_normal(N)
+    v = torch.zeros(N, dtype=torch.float64, device='cuda')
+    its = ctypes.c_int()
+    keep = [dev(L), dev(Ac), dev(R), dev(f)]
+    rc = lib.ttals_local_solve(ctx, None, a, P, n, Q, b, *[ptr(t) for t in keep], ptr(v), 3, 0, 1e-12, 4000,
+                               ctypes.byref(its))
+    assert rc == 0
+    assert 1 <= its.value <= 2, its.value
+    x = v.cpu().numpy()
+    assert np.linalg.norm(M @ x - f) <= 10 * 1e-12 * np.linalg.norm(f)
+
+
 def test_local_lu_singular_detected(env):
     lib, ctx, _lib = env
     a, P, n, Q, b = 2, 1, 3, 1, 2
