@@ -285,6 +285,23 @@ __device__ __forceinline__ int gmap_at(const GMap& g, int i) {
 }
 
 typedef double dx4_t __attribute__((ext_vector_type(4)));
+typedef float fx4_t __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step in FP64 or FP32 (operands: lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]) and the row of result element e
+// held by lane group lk = l / 16 (column l % 16): the FP64 instruction
+// interleaves the rows (lk + 4e), the FP32 one gives each group four
+// consecutive rows (4 lk + e).
+__device__ __forceinline__ dx4_t mfma16x16x4(double a, double b, dx4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ fx4_t mfma16x16x4(float a, float b, fx4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <class T>
+__device__ __forceinline__ int mfma16_row(int lk, int e) {
+  return std::is_same<T, double>::value ? lk + 4 * e : 4 * lk + e;
+}
 
 // K-steps of operands in flight during a K-step's MFMAs: 1 where the kernel
 // must stay within 256 registers for two workgroups per CU (k_als), 2 in a
@@ -305,10 +322,11 @@ constexpr int GEMM_PREFETCH = TTALS_GEMM_PREFETCH;
 // of L2/HBM overlaps the arithmetic instead of adding to it.  Elements past
 // M, N or K are staged as zeros (their products vanish; rows/columns past M,
 // N are not written).
-__device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A, CView<double> B,
-                                           MView<double> C, double* lds_) {
-  TT_LDS double* As = (TT_LDS double*)lds_;              // [GBK][GBM+1]
-  TT_LDS double* Bs = (TT_LDS double*)lds_ + GBK * (GBM + 1);  // [GBK][GBN+1]
+template <class T>
+__device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, T* lds_) {
+  typedef typename std::conditional<std::is_same<T, double>::value, dx4_t, fx4_t>::type acc_t;
+  TT_LDS T* As = (TT_LDS T*)lds_;              // [GBK][GBM+1]
+  TT_LDS T* Bs = (TT_LDS T*)lds_ + GBK * (GBM + 1);  // [GBK][GBN+1]
   static_assert(GBM == 64 && GBN == 64 && GBK == 16 && NT == 256, "tile shape assumed below");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -331,15 +349,15 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
   // arithmetic (a wave-uniform branch)
   const bool a_plain = A.r.div >= (1 << 22) && A.c.div >= (1 << 22);
   const bool b_plain = B.r.div >= (1 << 22) && B.c.div >= (1 << 22);
-  auto load = [&](double (&ra)[4], double (&rb)[4]) {
+  auto load = [&](T (&ra)[4], T (&rb)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int gm = lm0 + a_i0 + j * a_di, gk = lk0 + a_k0 + j * a_dk;
       const int ao = a_plain ? gm * A.r.lo + gk * A.c.lo : gmap_at(Ar, gm) + gmap_at(Ac, gk);
-      ra[j] = (gm < M && gk < K) ? A.p[ao] : 0.0;
+      ra[j] = (gm < M && gk < K) ? A.p[ao] : (T)0;
       const int gn = ln0 + b_n0 + j * b_dn, gk2 = lk0 + b_k0 + j * b_dk;
       const int bo = b_plain ? gk2 * B.r.lo + gn * B.c.lo : gmap_at(Br, gk2) + gmap_at(Bc, gn);
-      rb[j] = (gn < N && gk2 < K) ? B.p[bo] : 0.0;
+      rb[j] = (gn < N && gk2 < K) ? B.p[bo] : (T)0;
     }
     lk0 += GBK;
     if (lk0 >= K) {
@@ -352,33 +370,33 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
       }
     }
   };
-  auto stage = [&](const double (&ra)[4], const double (&rb)[4]) {
+  auto stage = [&](const T (&ra)[4], const T (&rb)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       As[(a_k0 + j * a_dk) * (GBM + 1) + a_i0 + j * a_di] = ra[j];
       Bs[(b_k0 + j * b_dk) * (GBN + 1) + b_n0 + j * b_dn] = rb[j];
     }
   };
-  dx4_t acc[2][2];
+  acc_t acc[2][2];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int g = 0; g < 2; ++g) acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < 2; ++g) acc[h][g] = acc_t{0, 0, 0, 0};
   // the MFMAs of the staged K-step, and the tile's store after its last one
   auto compute = [&]() {
 #pragma unroll
     for (int q = 0; q < GBK / 4; ++q) {
       const int kr = (4 * q + lk);
-      const double a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
-      const double b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      const T a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
+      const T b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
+      acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
+      acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+      acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+      acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
     }
     ckk += GBK;
     if (ckk >= K) {
-      // tile done: lane l holds C[i0 + l/16 + 4e][j0 + l%16] of each 16x16 block
+      // tile done: lane l holds C[i0 + mfma16_row(l/16, e)][j0 + l%16] of each 16x16 block
       const GMap Cr = gmap(C.r), Cc = gmap(C.c);
 #pragma unroll
       for (int g = 0; g < 2; ++g) {
@@ -389,10 +407,10 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
         for (int h = 0; h < 2; ++h) {
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int gm = cm0 + wm + 16 * h + lk + 4 * e;
+            const int gm = cm0 + wm + 16 * h + mfma16_row<T>(lk, e);
             if (nv && gm < M) C.p[gmap_at(Cr, gm) + cc] = acc[h][g][e];
           }
-          acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+          acc[h][g] = acc_t{0, 0, 0, 0};
         }
       }
       ckk = 0;
@@ -406,7 +424,7 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
   };
   if constexpr (GEMM_PREFETCH == 2) {
     // two register sets: while step s computes, steps s+1 and s+2 are loading
-    double ra0[4], rb0[4], ra1[4], rb1[4];
+    T ra0[4], rb0[4], ra1[4], rb1[4];
     if (steps > 0) {
       load(ra0, rb0);
       stage(ra0, rb0);
@@ -429,7 +447,7 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
       }
     }
   } else {
-    double ra[4], rb[4];
+    T ra[4], rb[4];
     if (steps > 0) {
       load(ra, rb);
       stage(ra, rb);
@@ -440,12 +458,12 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
 #pragma unroll
       for (int q = 0; q < GBK / 4; ++q) {
         const int kr = (4 * q + lk);
-        const double a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
-        const double b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
-        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        const T a0 = As[kr * (GBM + 1) + wm + li], a1 = As[kr * (GBM + 1) + wm + 16 + li];
+        const T b0 = Bs[kr * (GBN + 1) + wn + li], b1 = Bs[kr * (GBN + 1) + wn + 16 + li];
+        acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
+        acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+        acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+        acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
       }
       ckk += GBK;
       if (ckk >= K) {
@@ -459,10 +477,10 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
           for (int h = 0; h < 2; ++h) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              const int gm = cm0 + wm + 16 * h + lk + 4 * e;
+              const int gm = cm0 + wm + 16 * h + mfma16_row<T>(lk, e);
               if (nv && gm < M) C.p[gmap_at(Cr, gm) + cc] = acc[h][g][e];
             }
-            acc[h][g] = dx4_t{0.0, 0.0, 0.0, 0.0};
+            acc[h][g] = acc_t{0, 0, 0, 0};
           }
         }
         ckk = 0;
@@ -485,12 +503,12 @@ __device__ __forceinline__ void wg_gemm_mf(int M, int N, int K, CView<double> A,
 template <class T, class ACC>
 __device__ void wg_gemm_valu(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds);
 
-// GEMM over global operands: the matrix cores for FP64, the vector units
-// (register tiles) otherwise.
+// GEMM over global operands: the matrix cores in FP64 or FP32 (accumulating
+// in the operands' precision), the vector units (register tiles) otherwise.
 template <class T, class ACC>
 __device__ void wg_gemm(int M, int N, int K, CView<T> A, CView<T> B, MView<T> C, ACC* lds) {
-  if constexpr (std::is_same<T, double>::value && std::is_same<ACC, double>::value)
-    wg_gemm_mf(M, N, K, A, B, C, lds);
+  if constexpr ((std::is_same<T, double>::value || std::is_same<T, float>::value) && std::is_same<ACC, T>::value)
+    wg_gemm_mf<T>(M, N, K, A, B, C, lds);
   else
     wg_gemm_valu<T, ACC>(M, N, K, A, B, C, lds);
 }
